@@ -1,0 +1,620 @@
+// Device index, search launch and the C ABI entry points for search.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "gvs_internal.hpp"
+#include "search_query.hpp"
+
+using namespace gvsk;
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+template <int NCH, int STORE, int METRIC>
+__global__ __launch_bounds__(256) void k_search(KParams P)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int wib = threadIdx.x >> 6;
+    unsigned char *lds = smem + (size_t)wib * P.lds_per_wave;
+    const int64_t gw = (int64_t)blockIdx.x * P.waves_per_block + wib;
+    if (P.persistent) {
+        // device work queue: each wave takes the next query index
+        for (;;) {
+            uint32_t q = 0;
+            if (lane_id() == 0) q = atomicAdd(&P.counters[0], 1u);
+            q = __builtin_amdgcn_readfirstlane(q);
+            if ((int64_t)q >= P.nq) break;
+            search_query<NCH, STORE, METRIC>(P, (int64_t)q, gw, lds);
+        }
+    } else {
+        const int64_t qi = P.q_base + gw;
+        if (qi >= P.nq) return;
+        search_query<NCH, STORE, METRIC>(P, qi, gw, lds);
+    }
+}
+
+typedef void (*kfn_t)(KParams);
+
+template <int STORE, int METRIC>
+static kfn_t pick_nch(int nch)
+{
+    switch (nch) {
+    case 1: return k_search<1, STORE, METRIC>;
+    case 2: return k_search<2, STORE, METRIC>;
+    case 3: return k_search<3, STORE, METRIC>;
+    case 4: return k_search<4, STORE, METRIC>;
+    case 6: if constexpr (STORE != STORE_FP8) return k_search<6, STORE, METRIC>; else return nullptr;
+    case 8: if constexpr (STORE != STORE_FP8) return k_search<8, STORE, METRIC>; else return nullptr;
+    case 12: if constexpr (STORE == STORE_F32) return k_search<12, STORE, METRIC>; else return nullptr;
+    case 16: if constexpr (STORE == STORE_F32) return k_search<16, STORE, METRIC>; else return nullptr;
+    default: return nullptr;
+    }
+}
+
+static kfn_t pick_kernel(int storage, int metric, int nch)
+{
+    if (storage == GVS_STORE_F32)
+        return metric == GVS_METRIC_L2 ? pick_nch<STORE_F32, METRIC_L2>(nch) : pick_nch<STORE_F32, METRIC_IP>(nch);
+    if (storage == GVS_STORE_BF16)
+        return metric == GVS_METRIC_L2 ? pick_nch<STORE_BF16, METRIC_L2>(nch) : pick_nch<STORE_BF16, METRIC_IP>(nch);
+    if (storage == GVS_STORE_FP8)
+        return metric == GVS_METRIC_L2 ? pick_nch<STORE_FP8, METRIC_L2>(nch) : pick_nch<STORE_FP8, METRIC_IP>(nch);
+    return nullptr;
+}
+
+// ---------------------------------------------------------------------------
+// host helpers
+// ---------------------------------------------------------------------------
+namespace gvs {
+
+bool hip_ok(hipError_t e, const char *what)
+{
+    if (e == hipSuccess) return true;
+    set_error("%s: %s", what, hipGetErrorString(e));
+    return false;
+}
+
+static int ilog2_ceil(int64_t v)
+{
+    int l = 0;
+    while ((int64_t(1) << l) < v) l++;
+    return l;
+}
+
+// bf16 / e4m3 encoders for the low-precision copies (round to nearest even)
+static uint16_t f32_to_bf16(float f)
+{
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+static float e4m3_value(int c)
+{
+    int e = (c >> 3) & 15, m = c & 7;
+    return e ? std::ldexp(1.0f + m / 8.0f, e - 7) : (float)m * 0x1p-9f;
+}
+static uint8_t f32_to_e4m3(float f)
+{
+    uint8_t s = std::signbit(f) ? 0x80 : 0;
+    float a = std::fabs(f);
+    if (a >= 448.0f) return s | 0x7E;
+    // binary search over the monotone grid of codes 0..0x7E
+    int lo = 0, hi = 0x7E;
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (e4m3_value(mid) <= a) lo = mid;
+        else hi = mid;
+    }
+    float dl = a - e4m3_value(lo), dh = e4m3_value(hi) - a;
+    int c = (dl < dh) ? lo : (dh < dl) ? hi : ((lo & 1) ? hi : lo);
+    return s | (uint8_t)c;
+}
+
+}  // namespace gvs
+
+using gvs::hip_ok;
+
+static void free_index(gvs_index *ix)
+{
+    if (!ix) return;
+    (void)hipFree(ix->d_f32);
+    (void)hipFree(ix->d_bf16);
+    (void)hipFree(ix->d_fp8);
+    (void)hipFree(ix->d_scale);
+    (void)hipFree(ix->d_adj0);
+    (void)hipFree(ix->d_adj_up);
+    (void)hipFree(ix->d_upper_off);
+    (void)hipFree(ix->d_labels);
+    (void)hipFree(ix->d_vtab);
+    (void)hipFree(ix->d_counters);
+    (void)hipFree(ix->d_status);
+    delete ix;
+}
+
+extern "C" int gvs_index_create(const gvs_graph *g, uint32_t storage_mask, gvs_index **out)
+{
+    if (!g || !out || g->n <= 0 || g->n > INT32_MAX || g->dim <= 0 || !g->vectors || !g->adj0 ||
+        g->r0 <= 0 || g->ld_vectors < g->dim || g->entry < 0 || g->entry >= g->n ||
+        (g->metric != GVS_METRIC_L2 && g->metric != GVS_METRIC_IP) || g->max_level < 0) {
+        gvs::set_error("gvs_index_create: invalid graph description");
+        return GVS_ERR_ARG;
+    }
+    if (g->r0 > WAVE * MAX_ROWS) {
+        gvs::set_error("gvs_index_create: r0=%d exceeds %d", g->r0, WAVE * MAX_ROWS);
+        return GVS_ERR_UNSUPPORTED;
+    }
+    if (g->max_level > 0 && (!g->upper_off || !g->adj_up || g->ru <= 0 || g->ru > WAVE)) {
+        gvs::set_error("gvs_index_create: upper layers need upper_off, adj_up and 0 < ru <= 64");
+        return GVS_ERR_ARG;
+    }
+    // validate ids (a bad id would make the kernel read out of bounds)
+    const int64_t n = g->n;
+    for (int64_t i = 0; i < n * g->r0; i++)
+        if (g->adj0[i] < -1 || g->adj0[i] >= n) {
+            gvs::set_error("gvs_index_create: adj0[%lld] = %d out of range", (long long)i, g->adj0[i]);
+            return GVS_ERR_ARG;
+        }
+    if (g->max_level > 0) {
+        for (int64_t i = 0; i < g->up_rows * g->ru; i++)
+            if (g->adj_up[i] < -1 || g->adj_up[i] >= n) {
+                gvs::set_error("gvs_index_create: adj_up[%lld] out of range", (long long)i);
+                return GVS_ERR_ARG;
+            }
+        for (int64_t i = 0; i < n; i++)
+            if (g->upper_off[i] < -1 || g->upper_off[i] >= g->up_rows) {
+                gvs::set_error("gvs_index_create: upper_off[%lld] out of range", (long long)i);
+                return GVS_ERR_ARG;
+            }
+        // node levels from the row layout (rows of a node are consecutive);
+        // the descent at level l only follows links to nodes of level >= l
+        std::vector<std::pair<int64_t, int64_t>> offs;
+        for (int64_t i = 0; i < n; i++)
+            if (g->upper_off[i] >= 0) offs.emplace_back(g->upper_off[i], i);
+        std::sort(offs.begin(), offs.end());
+        std::vector<int32_t> lev((size_t)n, 0);
+        std::vector<int64_t> owner((size_t)g->up_rows, -1);
+        for (size_t j = 0; j < offs.size(); j++) {
+            int64_t end = j + 1 < offs.size() ? offs[j + 1].first : g->up_rows;
+            lev[(size_t)offs[j].second] = (int32_t)(end - offs[j].first);
+            for (int64_t r = offs[j].first; r < end; r++) owner[(size_t)r] = offs[j].second;
+        }
+        if (lev[(size_t)g->entry] < g->max_level) {
+            gvs::set_error("gvs_index_create: entry point level %d < max_level %d", lev[(size_t)g->entry],
+                           g->max_level);
+            return GVS_ERR_ARG;
+        }
+        for (int64_t r = 0; r < g->up_rows; r++) {
+            if (owner[(size_t)r] < 0) continue;
+            int l = (int)(r - g->upper_off[(size_t)owner[(size_t)r]]) + 1;
+            for (int j = 0; j < g->ru; j++) {
+                int32_t u = g->adj_up[r * g->ru + j];
+                if (u >= 0 && lev[(size_t)u] < l) {
+                    gvs::set_error("gvs_index_create: level-%d link to node %d of level %d", l, u, lev[(size_t)u]);
+                    return GVS_ERR_ARG;
+                }
+            }
+        }
+    }
+
+    gvs_index *ix = new gvs_index();
+    ix->n = n;
+    ix->dim = g->dim;
+    ix->metric = g->metric;
+    ix->r0 = g->r0;
+    ix->ru = g->ru;
+    ix->max_level = g->max_level;
+    ix->entry = g->entry;
+    ix->up_rows = g->max_level > 0 ? g->up_rows : 0;
+    ix->storage_mask = storage_mask | (1u << GVS_STORE_F32);
+    int gran = (ix->storage_mask & (1u << GVS_STORE_FP8)) ? 128 : (ix->storage_mask & (1u << GVS_STORE_BF16)) ? 64 : 32;
+    ix->dim_pad = (g->dim + gran - 1) / gran * gran;
+    (void)hipGetDevice(&ix->device);
+    hipDeviceProp_t prop;
+    if (!hip_ok(hipGetDeviceProperties(&prop, ix->device), "hipGetDeviceProperties")) {
+        delete ix;
+        return GVS_ERR_HIP;
+    }
+    ix->cu_count = prop.multiProcessorCount;
+
+    const int dp = ix->dim_pad;
+    std::vector<float> padded;
+    bool ok = true;
+    // fp32 copy, zero padded rows
+    {
+        padded.assign((size_t)n * dp, 0.0f);
+        for (int64_t i = 0; i < n; i++)
+            std::memcpy(&padded[(size_t)i * dp], g->vectors + i * g->ld_vectors, sizeof(float) * g->dim);
+        ok = ok && hip_ok(hipMalloc(&ix->d_f32, padded.size() * 4), "hipMalloc vectors") &&
+             hip_ok(hipMemcpy(ix->d_f32, padded.data(), padded.size() * 4, hipMemcpyHostToDevice), "copy vectors");
+    }
+    if (ok && (ix->storage_mask & (1u << GVS_STORE_BF16))) {
+        std::vector<uint16_t> h((size_t)n * dp);
+        for (size_t i = 0; i < h.size(); i++) h[i] = gvs::f32_to_bf16(padded[i]);
+        ok = hip_ok(hipMalloc(&ix->d_bf16, h.size() * 2), "hipMalloc bf16") &&
+             hip_ok(hipMemcpy(ix->d_bf16, h.data(), h.size() * 2, hipMemcpyHostToDevice), "copy bf16");
+    }
+    if (ok && (ix->storage_mask & (1u << GVS_STORE_FP8))) {
+        std::vector<uint8_t> h((size_t)n * dp);
+        std::vector<float> sc((size_t)n);
+        for (int64_t i = 0; i < n; i++) {
+            const float *row = &padded[(size_t)i * dp];
+            float amax = 0.0f;
+            for (int d = 0; d < dp; d++) amax = std::max(amax, std::fabs(row[d]));
+            float s = amax > 0.0f ? amax / 448.0f : 1.0f;
+            sc[(size_t)i] = s;
+            for (int d = 0; d < dp; d++) h[(size_t)i * dp + d] = gvs::f32_to_e4m3(row[d] / s);
+        }
+        ok = hip_ok(hipMalloc(&ix->d_fp8, h.size()), "hipMalloc fp8") &&
+             hip_ok(hipMemcpy(ix->d_fp8, h.data(), h.size(), hipMemcpyHostToDevice), "copy fp8") &&
+             hip_ok(hipMalloc(&ix->d_scale, sc.size() * 4), "hipMalloc scale") &&
+             hip_ok(hipMemcpy(ix->d_scale, sc.data(), sc.size() * 4, hipMemcpyHostToDevice), "copy scale");
+    }
+    padded.clear();
+    padded.shrink_to_fit();
+    ok = ok && hip_ok(hipMalloc(&ix->d_adj0, sizeof(int32_t) * n * g->r0), "hipMalloc adj0") &&
+         hip_ok(hipMemcpy(ix->d_adj0, g->adj0, sizeof(int32_t) * n * g->r0, hipMemcpyHostToDevice), "copy adj0");
+    if (ok && g->max_level > 0) {
+        ok = hip_ok(hipMalloc(&ix->d_adj_up, sizeof(int32_t) * g->up_rows * g->ru), "hipMalloc adj_up") &&
+             hip_ok(hipMemcpy(ix->d_adj_up, g->adj_up, sizeof(int32_t) * g->up_rows * g->ru, hipMemcpyHostToDevice),
+                    "copy adj_up") &&
+             hip_ok(hipMalloc(&ix->d_upper_off, sizeof(int64_t) * n), "hipMalloc upper_off") &&
+             hip_ok(hipMemcpy(ix->d_upper_off, g->upper_off, sizeof(int64_t) * n, hipMemcpyHostToDevice),
+                    "copy upper_off");
+    }
+    if (ok && g->labels) {
+        ok = hip_ok(hipMalloc(&ix->d_labels, sizeof(int64_t) * n), "hipMalloc labels") &&
+             hip_ok(hipMemcpy(ix->d_labels, g->labels, sizeof(int64_t) * n, hipMemcpyHostToDevice), "copy labels");
+    }
+    ok = ok && hip_ok(hipMalloc(&ix->d_counters, 64), "hipMalloc counters") &&
+         hip_ok(hipMemset(ix->d_counters, 0, 64), "memset counters");
+    if (!ok) {
+        free_index(ix);
+        return GVS_ERR_HIP;
+    }
+    *out = ix;
+    return GVS_OK;
+}
+
+extern "C" int gvs_index_destroy(gvs_index *ix)
+{
+    if (!ix) return GVS_ERR_ARG;
+    (void)hipDeviceSynchronize();
+    free_index(ix);
+    return GVS_OK;
+}
+
+extern "C" int gvs_index_device_bytes(const gvs_index *ix, int64_t *graph_bytes, int64_t *vector_bytes)
+{
+    if (!ix) return GVS_ERR_ARG;
+    if (graph_bytes)
+        *graph_bytes = ix->n * ix->r0 * 4 + ix->up_rows * ix->ru * 4 + (ix->max_level > 0 ? ix->n * 8 : 0);
+    if (vector_bytes) {
+        int64_t v = ix->n * ix->dim_pad * 4;
+        if (ix->d_bf16) v += ix->n * ix->dim_pad * 2;
+        if (ix->d_fp8) v += ix->n * ix->dim_pad + ix->n * 4;
+        *vector_bytes = v;
+    }
+    return GVS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// launch planning
+// ---------------------------------------------------------------------------
+namespace gvs {
+
+int plan_search(const gvs_index *ix, const gvs_params *p, int max_ef, int max_k, LaunchPlan *lp)
+{
+    if (!p) {
+        set_error("gvs_search: params is NULL");
+        return GVS_ERR_ARG;
+    }
+    if (max_k < 1 || max_k > 4096 || max_ef < 1 || max_ef > 4096 || p->mc < 1 || p->mc > 8 || p->mg < 1 ||
+        p->mg > 1 || p->waves_per_query != 1) {
+        set_error("gvs_search: unsupported k=%d ef=%d mc=%d mg=%d waves_per_query=%d", max_k, max_ef, p->mc,
+                  p->mg, p->waves_per_query);
+        return GVS_ERR_UNSUPPORTED;
+    }
+    if (p->storage < 0 || p->storage > 2 || !(ix->storage_mask & (1u << p->storage))) {
+        set_error("gvs_search: storage %d not built for this index", p->storage);
+        return GVS_ERR_ARG;
+    }
+    const int rpc = (ix->r0 + WAVE - 1) / WAVE;
+    if (p->mc * rpc > MAX_ROWS) {
+        set_error("gvs_search: mc * ceil(r0/64) must be <= %d", MAX_ROWS);
+        return GVS_ERR_UNSUPPORTED;
+    }
+    const int E = p->storage == GVS_STORE_F32 ? 4 : p->storage == GVS_STORE_BF16 ? 8 : 16;
+    const int nch = ix->dim_pad / (8 * E);
+    kfn_t fn = (ix->dim_pad % (8 * E) == 0) ? pick_kernel(p->storage, ix->metric, nch) : nullptr;
+    if (!fn) {
+        set_error("gvs_search: dim %d (padded %d) not compiled for storage %d", ix->dim, ix->dim_pad, p->storage);
+        return GVS_ERR_UNSUPPORTED;
+    }
+    int efe = std::max(max_ef, max_k);
+    lp->fn = (void *)fn;
+    lp->ef_cap = (efe + 63) / 64 * 64;
+    lp->lcap = p->mc * rpc * WAVE;
+    lp->blog2 = p->bloom_log2 > 0 ? p->bloom_log2 : std::min(12, std::max(8, ilog2_ceil((int64_t)efe * 16)));
+    int vmin = ilog2_ceil(4 * (int64_t)lp->lcap + 1024);
+    lp->vlog2 = p->visited_log2 > 0 ? std::max(p->visited_log2, vmin)
+                                    : std::max(vmin, std::min(20, ilog2_ceil((int64_t)efe * 128)));
+    if (lp->blog2 > 14 || lp->vlog2 > 26) {
+        set_error("gvs_search: bloom_log2 <= 14 and visited_log2 <= 26 required");
+        return GVS_ERR_ARG;
+    }
+    int bytes = 16 * lp->ef_cap + 8 * lp->lcap + 4 * (1 << lp->blog2) + 4 * lp->lcap + 256;
+    lp->lds_per_wave = (bytes + 15) / 16 * 16;
+    lp->wpb = std::max(1, std::min(4, 65536 / lp->lds_per_wave));
+    if (lp->lds_per_wave > 65536) {
+        set_error("gvs_search: per-query LDS %d bytes exceeds 64 KiB (ef too large)", lp->lds_per_wave);
+        return GVS_ERR_UNSUPPORTED;
+    }
+    int blocks_per_cu = 0;
+    if (!hip_ok(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks_per_cu, (const void *)fn, lp->wpb * WAVE,
+                                                             (size_t)lp->wpb * lp->lds_per_wave),
+                "hipOccupancyMaxActiveBlocksPerMultiprocessor"))
+        return GVS_ERR_HIP;
+    lp->resident_waves = std::max(1, blocks_per_cu) * ix->cu_count * lp->wpb;
+    return GVS_OK;
+}
+
+int ensure_scratch(gvs_index *ix, int64_t slots, int vlog2, int64_t nstatus)
+{
+    size_t words = (size_t)slots << vlog2;
+    if (words > ix->vtab_words) {
+        (void)hipFree(ix->d_vtab);
+        ix->d_vtab = nullptr;
+        ix->vtab_words = 0;
+        if (!hip_ok(hipMalloc(&ix->d_vtab, words * 4), "hipMalloc visited tables") ||
+            !hip_ok(hipMemset(ix->d_vtab, 0, words * 4), "hipMemset visited tables"))
+            return GVS_ERR_NOMEM;
+        ix->vtab_words = words;
+    }
+    if (nstatus > ix->status_cap) {
+        (void)hipFree(ix->d_status);
+        ix->d_status = nullptr;
+        ix->status_cap = 0;
+        if (!hip_ok(hipMalloc(&ix->d_status, sizeof(int32_t) * nstatus), "hipMalloc status")) return GVS_ERR_NOMEM;
+        ix->status_cap = nstatus;
+    }
+    return GVS_OK;
+}
+
+void fill_params(const gvs_index *ix, const gvs_params *p, const LaunchPlan &lp, KParams &P)
+{
+    std::memset(&P, 0, sizeof P);
+    P.vec = p->storage == GVS_STORE_F32 ? (const void *)ix->d_f32
+          : p->storage == GVS_STORE_BF16 ? (const void *)ix->d_bf16 : (const void *)ix->d_fp8;
+    P.vscale = ix->d_scale;
+    P.vec_f32 = ix->d_f32;
+    P.row_elems = ix->dim_pad;
+    P.row_elems_f32 = ix->dim_pad;
+    P.dim = ix->dim;
+    P.r0 = ix->r0;
+    P.adj0 = ix->d_adj0;
+    P.adj_up = ix->d_adj_up;
+    P.upper_off = ix->d_upper_off;
+    P.ru = ix->ru;
+    P.max_level = ix->max_level;
+    P.entry = ix->entry;
+    P.labels = ix->d_labels;
+    P.mc = p->mc;
+    P.rerank = p->rerank;
+    P.vtab = ix->d_vtab;
+    P.vlog2 = lp.vlog2;
+    P.blog2 = lp.blog2;
+    P.bloom_seed = 0x2545F491u;
+    P.ef_cap = lp.ef_cap;
+    P.lcap = lp.lcap;
+    P.lds_per_wave = lp.lds_per_wave;
+    P.waves_per_block = lp.wpb;
+    P.counters = ix->d_counters;
+    P.status = ix->d_status;
+}
+
+int launch_search(gvs_index *ix, const gvs_params *p, const LaunchPlan &lp, KParams P, int64_t nq,
+                  hipStream_t stream)
+{
+    kfn_t fn = (kfn_t)lp.fn;
+    const size_t shm = (size_t)lp.wpb * lp.lds_per_wave;
+    if (p->persistent) {
+        int64_t gw = p->grid_waves > 0 ? p->grid_waves : lp.resident_waves;
+        gw = std::max<int64_t>(lp.wpb, std::min<int64_t>(gw, (nq + lp.wpb - 1) / lp.wpb * lp.wpb));
+        gw = (gw + lp.wpb - 1) / lp.wpb * lp.wpb;
+        int rc = ensure_scratch(ix, gw, lp.vlog2, nq);
+        if (rc) return rc;
+        P.vtab = ix->d_vtab;
+        P.status = ix->d_status;
+        P.persistent = 1;
+        P.nq = nq;
+        P.q_base = 0;
+        if (!hip_ok(hipMemsetAsync(ix->d_counters, 0, 8, stream), "hipMemsetAsync counters")) return GVS_ERR_HIP;
+        hipLaunchKernelGGL(fn, dim3((unsigned)(gw / lp.wpb)), dim3(lp.wpb * WAVE), shm, stream, P);
+        if (!hip_ok(hipGetLastError(), "launch k_search (persistent)")) return GVS_ERR_HIP;
+        return GVS_OK;
+    }
+    // one wave per query; queries processed in chunks of at most `slots`
+    int64_t slots = std::min<int64_t>(nq, std::max<int64_t>(lp.wpb, ((int64_t)1 << 30) >> lp.vlog2));
+    slots = std::min<int64_t>(slots, 65536);
+    slots = (slots + lp.wpb - 1) / lp.wpb * lp.wpb;
+    int rc = ensure_scratch(ix, slots, lp.vlog2, nq);
+    if (rc) return rc;
+    P.vtab = ix->d_vtab;
+    P.status = ix->d_status;
+    P.persistent = 0;
+    if (!hip_ok(hipMemsetAsync(ix->d_counters, 0, 8, stream), "hipMemsetAsync counters")) return GVS_ERR_HIP;
+    for (int64_t b = 0; b < nq; b += slots) {
+        int64_t cnt = std::min(slots, nq - b);
+        P.q_base = b;
+        P.nq = b + cnt;
+        hipLaunchKernelGGL(fn, dim3((unsigned)((cnt + lp.wpb - 1) / lp.wpb)), dim3(lp.wpb * WAVE), shm, stream, P);
+        if (!hip_ok(hipGetLastError(), "launch k_search")) return GVS_ERR_HIP;
+    }
+    return GVS_OK;
+}
+
+}  // namespace gvs
+
+static int search_impl(gvs_index *ix, const float *queries, int64_t nq, int64_t ldq, const int32_t *entries,
+                       const gvs_params *p, int64_t *labels, float *dists, int32_t *stats, hipStream_t stream)
+{
+    if (!ix || !p || nq < 0 || (nq > 0 && (!queries || !labels || !dists)) || ldq < ix->dim) {
+        gvs::set_error("gvs_search: invalid argument");
+        return GVS_ERR_ARG;
+    }
+    if (nq == 0) return GVS_OK;
+    if (nq > INT32_MAX) {
+        gvs::set_error("gvs_search: nq too large");
+        return GVS_ERR_ARG;
+    }
+    gvs::LaunchPlan lp;
+    int rc = gvs::plan_search(ix, p, p->ef, p->k, &lp);
+    if (rc) return rc;
+    KParams P;
+    gvs::fill_params(ix, p, lp, P);
+    P.queries = queries;
+    P.ldq = ldq;
+    P.entries = entries;
+    P.k = p->k;
+    P.ef = std::max(p->ef, p->k);
+    P.out_labels = labels;
+    P.out_dists = dists;
+    P.out_stats = stats;
+    rc = gvs::launch_search(ix, p, lp, P, nq, stream);
+    if (rc) return rc;
+    // visited-table overflow: rerun the affected queries with 4x larger tables
+    uint32_t novf = 0;
+    if (!hip_ok(hipMemcpyAsync(&novf, ix->d_counters + 1, 4, hipMemcpyDeviceToHost, stream), "read overflow count") ||
+        !hip_ok(hipStreamSynchronize(stream), "hipStreamSynchronize"))
+        return GVS_ERR_HIP;
+    for (int attempt = 0; novf > 0 && attempt < 3; attempt++) {
+        std::vector<int32_t> st((size_t)nq);
+        if (!hip_ok(hipMemcpy(st.data(), ix->d_status, sizeof(int32_t) * nq, hipMemcpyDeviceToHost), "read status"))
+            return GVS_ERR_HIP;
+        std::vector<int64_t> idx;
+        for (int64_t i = 0; i < nq; i++)
+            if (st[(size_t)i]) idx.push_back(i);
+        gvs_params p2 = *p;
+        p2.visited_log2 = lp.vlog2 + 2 * (attempt + 1);
+        p2.persistent = 0;
+        gvs::LaunchPlan lp2;
+        rc = gvs::plan_search(ix, &p2, p2.ef, p2.k, &lp2);
+        if (rc) return rc;
+        KParams P2;
+        gvs::fill_params(ix, &p2, lp2, P2);
+        int64_t *d_idx = nullptr;
+        if (!hip_ok(hipMalloc(&d_idx, sizeof(int64_t) * idx.size()), "hipMalloc retry") ||
+            !hip_ok(hipMemcpy(d_idx, idx.data(), sizeof(int64_t) * idx.size(), hipMemcpyHostToDevice), "copy retry")) {
+            (void)hipFree(d_idx);
+            return GVS_ERR_HIP;
+        }
+        // rerun the failed rows, writing into temporaries, then scatter back
+        int64_t m = (int64_t)idx.size();
+        int64_t *t_lab = nullptr;
+        float *t_d = nullptr;
+        int32_t *t_st = nullptr;
+        int32_t *t_ent = nullptr;
+        bool ok = hip_ok(hipMalloc(&t_lab, sizeof(int64_t) * m * p->k), "hipMalloc") &&
+                  hip_ok(hipMalloc(&t_d, sizeof(float) * m * p->k), "hipMalloc") &&
+                  hip_ok(hipMalloc(&t_st, sizeof(int32_t) * m * 4), "hipMalloc");
+        if (ok && entries) {
+            std::vector<int32_t> he((size_t)m);
+            for (int64_t i = 0; i < m; i++)
+                (void)hipMemcpy(&he[(size_t)i], entries + idx[(size_t)i], 4, hipMemcpyDeviceToHost);
+            ok = hip_ok(hipMalloc(&t_ent, sizeof(int32_t) * m), "hipMalloc") &&
+                 hip_ok(hipMemcpy(t_ent, he.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice), "copy");
+        }
+        if (ok) {
+            P2.queries = queries;
+            P2.ldq = ldq;
+            P2.q_index = d_idx;
+            P2.entries = t_ent;
+            P2.k = p->k;
+            P2.ef = std::max(p->ef, p->k);
+            P2.out_labels = t_lab;
+            P2.out_dists = t_d;
+            P2.out_stats = t_st;
+            ok = hip_ok(hipMemset(ix->d_counters, 0, 8), "memset") &&
+                 gvs::launch_search(ix, &p2, lp2, P2, m, stream) == GVS_OK &&
+                 hip_ok(hipStreamSynchronize(stream), "sync retry");
+            for (int64_t i = 0; ok && i < m; i++) {
+                int64_t q = idx[(size_t)i];
+                ok = hip_ok(hipMemcpy(labels + q * p->k, t_lab + i * p->k, sizeof(int64_t) * p->k,
+                                      hipMemcpyDeviceToDevice), "scatter") &&
+                     hip_ok(hipMemcpy(dists + q * p->k, t_d + i * p->k, sizeof(float) * p->k,
+                                      hipMemcpyDeviceToDevice), "scatter") &&
+                     (!stats || hip_ok(hipMemcpy(stats + q * 4, t_st + i * 4, 16, hipMemcpyDeviceToDevice), "scatter"));
+            }
+            if (ok) ok = hip_ok(hipMemcpy(&novf, ix->d_counters + 1, 4, hipMemcpyDeviceToHost), "read overflow");
+        }
+        (void)hipFree(d_idx);
+        (void)hipFree(t_lab);
+        (void)hipFree(t_d);
+        (void)hipFree(t_st);
+        (void)hipFree(t_ent);
+        if (!ok) return GVS_ERR_HIP;
+    }
+    if (novf > 0) {
+        gvs::set_error("gvs_search: %u queries overflowed the visited table", novf);
+        return GVS_ERR_OVERFLOW;
+    }
+    return GVS_OK;
+}
+
+extern "C" int gvs_search_device(gvs_index *ix, const float *queries, int64_t nq, int64_t ldq,
+                                 const int32_t *entries, const gvs_params *p, int64_t *labels, float *dists,
+                                 int32_t *stats, void *stream)
+{
+    return search_impl(ix, queries, nq, ldq, entries, p, labels, dists, stats, (hipStream_t)stream);
+}
+
+extern "C" int gvs_search_host(gvs_index *ix, const float *queries, int64_t nq, int64_t ldq,
+                               const int32_t *entries, const gvs_params *p, int64_t *labels, float *dists,
+                               int32_t *stats)
+{
+    if (!ix || !p || nq < 0 || (nq > 0 && (!queries || !labels || !dists)) || ldq < (ix ? ix->dim : 1)) {
+        gvs::set_error("gvs_search_host: invalid argument");
+        return GVS_ERR_ARG;
+    }
+    if (nq == 0) return GVS_OK;
+    float *dq = nullptr;
+    int32_t *de = nullptr, *ds = nullptr;
+    int64_t *dl = nullptr;
+    float *dd = nullptr;
+    int rc = GVS_ERR_HIP;
+    const int k = p->k;
+    if (k < 1) {
+        gvs::set_error("gvs_search_host: k must be >= 1");
+        return GVS_ERR_ARG;
+    }
+    bool ok = hip_ok(hipMalloc(&dq, sizeof(float) * nq * ldq), "hipMalloc queries") &&
+              hip_ok(hipMemcpy(dq, queries, sizeof(float) * nq * ldq, hipMemcpyHostToDevice), "copy queries") &&
+              hip_ok(hipMalloc(&dl, sizeof(int64_t) * nq * k), "hipMalloc labels") &&
+              hip_ok(hipMalloc(&dd, sizeof(float) * nq * k), "hipMalloc dists");
+    if (ok && entries)
+        ok = hip_ok(hipMalloc(&de, sizeof(int32_t) * nq), "hipMalloc entries") &&
+             hip_ok(hipMemcpy(de, entries, sizeof(int32_t) * nq, hipMemcpyHostToDevice), "copy entries");
+    if (ok && stats) ok = hip_ok(hipMalloc(&ds, sizeof(int32_t) * nq * 4), "hipMalloc stats");
+    if (ok) {
+        rc = search_impl(ix, dq, nq, ldq, de, p, dl, dd, ds, nullptr);
+        if (rc == GVS_OK) {
+            ok = hip_ok(hipDeviceSynchronize(), "hipDeviceSynchronize") &&
+                 hip_ok(hipMemcpy(labels, dl, sizeof(int64_t) * nq * k, hipMemcpyDeviceToHost), "copy labels") &&
+                 hip_ok(hipMemcpy(dists, dd, sizeof(float) * nq * k, hipMemcpyDeviceToHost), "copy dists") &&
+                 (!stats || hip_ok(hipMemcpy(stats, ds, sizeof(int32_t) * nq * 4, hipMemcpyDeviceToHost), "copy stats"));
+            if (!ok) rc = GVS_ERR_HIP;
+        }
+    }
+    (void)hipFree(dq);
+    (void)hipFree(de);
+    (void)hipFree(ds);
+    (void)hipFree(dl);
+    (void)hipFree(dd);
+    return rc;
+}

@@ -1,0 +1,41 @@
+// Internal declarations shared by the device-side translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "host_graph.hpp"
+
+struct gvs_index {
+    int device = 0;
+    int cu_count = 0;
+    int64_t n = 0;
+    int dim = 0, dim_pad = 0, metric = 0;
+    int r0 = 0, ru = 0, max_level = 0, entry = 0;
+    int64_t up_rows = 0;
+    uint32_t storage_mask = 1;
+    float *d_f32 = nullptr;
+    uint16_t *d_bf16 = nullptr;
+    uint8_t *d_fp8 = nullptr;
+    float *d_scale = nullptr;
+    int32_t *d_adj0 = nullptr;
+    int32_t *d_adj_up = nullptr;
+    int64_t *d_upper_off = nullptr;
+    int64_t *d_labels = nullptr;
+    // scratch: per-slot visited tables, work-queue counters, per-query status
+    uint32_t *d_vtab = nullptr;
+    size_t vtab_words = 0;
+    uint32_t *d_counters = nullptr;
+    int32_t *d_status = nullptr;
+    int64_t status_cap = 0;
+};
+
+namespace gvs {
+bool hip_ok(hipError_t e, const char *what);
+
+struct LaunchPlan {
+    void *fn = nullptr;
+    int ef_cap = 0, lcap = 0, blog2 = 0, vlog2 = 0, lds_per_wave = 0, wpb = 1;
+    int64_t resident_waves = 0;
+};
+}  // namespace gvs

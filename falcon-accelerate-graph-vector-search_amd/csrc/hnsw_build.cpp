@@ -1,0 +1,433 @@
+// HNSW graph construction with hnswlib's insertion semantics, multithreaded.
+//
+// Follows networked_FPGA/CPU_programs_oldest/hnswlib/hnswalg.h (HNSWLIB):
+//   constructor fields (M, maxM = M, maxM0 = 2M, efc = max(efc, M),
+//     mult = 1/ln(M), level RNG seeded with random_seed)   :27-66
+//   getRandomLevel  -ln(U(0,1)) * mult                      :148-152
+//   searchBaseLayer (construction search, efc)              :153-236
+//   getNeighborsByHeuristic2                                :328-369
+//   mutuallyConnectNewElement                               :386-501
+//   addPoint (descent, per-level search + connect)          :982-1112
+// Levels are drawn in index order before the parallel insertion, so one
+// thread reproduces hnswlib's single-thread graph draw for draw.  Locks guard
+// each node's lists; unlike hnswlib no lock is held across another lock.
+// Priority queues are std::priority_queue with the same comparators, so the
+// order of equal-distance pops matches hnswlib's as well.
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <queue>
+#include <random>
+#include <thread>
+#include <vector>
+
+#include "host_graph.hpp"
+
+namespace {
+
+typedef std::pair<float, uint32_t> pf;
+struct CmpFirst {
+    bool operator()(const pf &a, const pf &b) const noexcept { return a.first < b.first; }
+};
+typedef std::priority_queue<pf, std::vector<pf>, CmpFirst> pq_t;
+
+// hnswlib scalar order (space_l2.h:6-20 / space_ip.h:7-19); built with
+// -ffp-contract=off so the multiply and add round separately.
+float dist_exact(const float *a, const float *b, int d, int metric)
+{
+    float res = 0.0f;
+    if (metric == GVS_METRIC_L2) {
+        for (int i = 0; i < d; i++) {
+            float t = a[i] - b[i];
+            res += t * t;
+        }
+        return res;
+    }
+    for (int i = 0; i < d; i++) res += a[i] * b[i];
+    return 1.0f - res;
+}
+
+// 16 independent partial sums (vectorises to AVX2 without reassociation)
+float dist_fast(const float *a, const float *b, int d, int metric)
+{
+    float s[16] = {0};
+    int i = 0;
+    if (metric == GVS_METRIC_L2) {
+        for (; i + 16 <= d; i += 16)
+            for (int j = 0; j < 16; j++) {
+                float t = a[i + j] - b[i + j];
+                s[j] += t * t;
+            }
+        float r = 0.0f;
+        for (; i < d; i++) {
+            float t = a[i] - b[i];
+            r += t * t;
+        }
+        for (int j = 0; j < 16; j++) r += s[j];
+        return r;
+    }
+    for (; i + 16 <= d; i += 16)
+        for (int j = 0; j < 16; j++) s[j] += a[i + j] * b[i + j];
+    float r = 0.0f;
+    for (; i < d; i++) r += a[i] * b[i];
+    for (int j = 0; j < 16; j++) r += s[j];
+    return 1.0f - r;
+}
+
+struct Builder {
+    const float *x;
+    int64_t n;
+    int dim, metric;
+    bool exact;
+    size_t M, maxM, maxM0, efc;
+    double mult;
+    std::vector<uint32_t> l0;                  // n * (1 + maxM0): count, ids
+    std::vector<std::vector<uint32_t>> up;     // per node: level * (1 + maxM)
+    std::vector<int> levels;
+    std::vector<std::mutex> locks;
+    std::mutex global;
+    int maxlevel = -1;
+    int64_t entry = -1;
+
+    Builder(const float *x_, int64_t n_, int dim_, int metric_, size_t M_, size_t efc_,
+            uint64_t seed, bool exact_)
+        : x(x_), n(n_), dim(dim_), metric(metric_), exact(exact_), M(M_), maxM(M_),
+          maxM0(2 * M_), efc(std::max(efc_, M_)), l0((size_t)n_ * (1 + 2 * M_), 0),
+          up((size_t)n_), levels((size_t)n_), locks((size_t)n_)
+    {
+        mult = 1.0 / std::log(1.0 * M);
+        std::default_random_engine gen;
+        gen.seed(seed);
+        for (int64_t i = 0; i < n; i++) {
+            std::uniform_real_distribution<double> distribution(0.0, 1.0);
+            double r = -std::log(distribution(gen)) * mult;
+            levels[(size_t)i] = (int)r;
+        }
+    }
+
+    inline const float *vec(uint32_t i) const { return x + (size_t)i * dim; }
+    inline float dist(const float *a, const float *b) const
+    {
+        return exact ? dist_exact(a, b, dim, metric) : dist_fast(a, b, dim, metric);
+    }
+    inline uint32_t *list(uint32_t i, int level)
+    {
+        if (level == 0) return &l0[(size_t)i * (1 + maxM0)];
+        return &up[i][(size_t)(level - 1) * (1 + maxM)];
+    }
+    std::vector<uint32_t> copy_list(uint32_t i, int level)
+    {
+        std::lock_guard<std::mutex> lk(locks[i]);
+        uint32_t *l = list(i, level);
+        return std::vector<uint32_t>(l + 1, l + 1 + l[0]);
+    }
+
+    // HNSWLIB/hnswalg.h:153-236
+    pq_t search_layer(uint32_t ep, const float *q, int layer, std::vector<uint32_t> &tag, uint32_t &epoch)
+    {
+        epoch++;
+        if (epoch == 0) {
+            std::fill(tag.begin(), tag.end(), 0);
+            epoch = 1;
+        }
+        pq_t top, cand;
+        float d = dist(q, vec(ep));
+        top.emplace(d, ep);
+        float lower = d;
+        cand.emplace(-d, ep);
+        tag[ep] = epoch;
+        std::vector<uint32_t> nb;
+        while (!cand.empty()) {
+            pf cur = cand.top();
+            if ((-cur.first) > lower && top.size() == efc) break;
+            cand.pop();
+            nb = copy_list(cur.second, layer);
+            for (uint32_t c : nb) {
+                if (tag[c] == epoch) continue;
+                tag[c] = epoch;
+                float d1 = dist(q, vec(c));
+                if (top.size() < efc || lower > d1) {
+                    cand.emplace(-d1, c);
+                    top.emplace(d1, c);
+                    if (top.size() > efc) top.pop();
+                    if (!top.empty()) lower = top.top().first;
+                }
+            }
+        }
+        return top;
+    }
+
+    // HNSWLIB/hnswalg.h:328-369
+    void heuristic(pq_t &top, size_t m)
+    {
+        if (top.size() < m) return;
+        std::priority_queue<pf> closest;
+        std::vector<pf> ret;
+        while (!top.empty()) {
+            closest.emplace(-top.top().first, top.top().second);
+            top.pop();
+        }
+        while (!closest.empty()) {
+            if (ret.size() >= m) break;
+            pf cur = closest.top();
+            float dq = -cur.first;
+            closest.pop();
+            bool good = true;
+            for (const pf &s : ret) {
+                float dd = dist(vec(s.second), vec(cur.second));
+                if (dd < dq) {
+                    good = false;
+                    break;
+                }
+            }
+            if (good) ret.push_back(cur);
+        }
+        for (const pf &c : ret) top.emplace(-c.first, c.second);
+    }
+
+    // HNSWLIB/hnswalg.h:386-501 (isUpdate = false)
+    uint32_t connect(uint32_t cur, pq_t &top, int level)
+    {
+        size_t mcur = level ? maxM : maxM0;
+        heuristic(top, M);
+        std::vector<uint32_t> sel;
+        sel.reserve(M);
+        while (!top.empty()) {
+            sel.push_back(top.top().second);
+            top.pop();
+        }
+        uint32_t next = sel.back();
+        {
+            std::lock_guard<std::mutex> lk(locks[cur]);
+            uint32_t *l = list(cur, level);
+            l[0] = (uint32_t)sel.size();
+            for (size_t i = 0; i < sel.size(); i++) l[1 + i] = sel[i];
+        }
+        for (uint32_t s : sel) {
+            std::lock_guard<std::mutex> lk(locks[s]);
+            uint32_t *l = list(s, level);
+            size_t sz = l[0];
+            if (sz < mcur) {
+                l[1 + sz] = cur;
+                l[0] = (uint32_t)(sz + 1);
+            } else {
+                float dmax = dist(vec(cur), vec(s));
+                pq_t cands;
+                cands.emplace(dmax, cur);
+                for (size_t j = 0; j < sz; j++) cands.emplace(dist(vec(l[1 + j]), vec(s)), l[1 + j]);
+                heuristic(cands, mcur);
+                size_t idx = 0;
+                while (!cands.empty()) {
+                    l[1 + idx] = cands.top().second;
+                    cands.pop();
+                    idx++;
+                }
+                l[0] = (uint32_t)idx;
+            }
+        }
+        return next;
+    }
+
+    // HNSWLIB/hnswalg.h:982-1112
+    void insert(uint32_t cur, std::vector<uint32_t> &tag, uint32_t &epoch)
+    {
+        int curlevel = levels[cur];
+        if (curlevel > 0) up[cur].assign((size_t)curlevel * (1 + maxM), 0);
+        std::unique_lock<std::mutex> glk(global);
+        int maxlevelcopy = maxlevel;
+        int64_t ep = entry;
+        if (curlevel <= maxlevelcopy) glk.unlock();
+        if (ep < 0) {  // first element
+            entry = cur;
+            maxlevel = curlevel;
+            return;
+        }
+        const float *q = vec(cur);
+        uint32_t obj = (uint32_t)ep;
+        if (curlevel < maxlevelcopy) {
+            float curdist = dist(q, vec(obj));
+            for (int level = maxlevelcopy; level > curlevel; level--) {
+                bool changed = true;
+                while (changed) {
+                    changed = false;
+                    std::vector<uint32_t> nb = copy_list(obj, level);
+                    for (uint32_t c : nb) {
+                        float d = dist(q, vec(c));
+                        if (d < curdist) {
+                            curdist = d;
+                            obj = c;
+                            changed = true;
+                        }
+                    }
+                }
+            }
+        }
+        for (int level = std::min(curlevel, maxlevelcopy); level >= 0; level--) {
+            pq_t top = search_layer(obj, q, level, tag, epoch);
+            obj = connect(cur, top, level);
+        }
+        if (curlevel > maxlevelcopy) {
+            entry = cur;
+            maxlevel = curlevel;
+        }
+    }
+};
+
+}  // namespace
+
+extern "C" int gvs_hnsw_build(const float *x, int64_t n, int32_t dim, int32_t metric, int32_t M,
+                              int32_t ef_construction, uint64_t seed, int32_t nthreads,
+                              int32_t exact_order, gvs_host_graph **out)
+{
+    if (!x || n <= 0 || dim <= 0 || M < 2 || ef_construction < 1 || !out || n > INT32_MAX ||
+        (metric != GVS_METRIC_L2 && metric != GVS_METRIC_IP)) {
+        gvs::set_error("gvs_hnsw_build: invalid argument");
+        return GVS_ERR_ARG;
+    }
+    if (M > 1024) {
+        gvs::set_error("gvs_hnsw_build: M too large");
+        return GVS_ERR_ARG;
+    }
+    try {
+        Builder b(x, n, dim, metric, (size_t)M, (size_t)ef_construction, seed, exact_order != 0);
+        int nt = std::max(1, (int)nthreads);
+        {
+            std::vector<uint32_t> tag((size_t)n, 0);
+            uint32_t epoch = 0;
+            b.insert(0, tag, epoch);
+        }
+        std::atomic<int64_t> next(1);
+        auto work = [&]() {
+            std::vector<uint32_t> tag((size_t)n, 0);
+            uint32_t epoch = 0;
+            for (;;) {
+                int64_t i = next.fetch_add(1);
+                if (i >= n) break;
+                b.insert((uint32_t)i, tag, epoch);
+            }
+        };
+        if (nt == 1) {
+            work();
+        } else {
+            std::vector<std::thread> th;
+            for (int t = 0; t < nt; t++) th.emplace_back(work);
+            for (auto &t : th) t.join();
+        }
+
+        gvs_host_graph *g = new gvs_host_graph();
+        g->n = n;
+        g->dim = dim;
+        g->metric = metric;
+        g->vectors.assign(x, x + (size_t)n * dim);
+        g->r0 = (int32_t)b.maxM0;
+        g->adj0.assign((size_t)n * b.maxM0, -1);
+        for (int64_t i = 0; i < n; i++) {
+            const uint32_t *l = &b.l0[(size_t)i * (1 + b.maxM0)];
+            for (uint32_t j = 0; j < l[0]; j++) g->adj0[(size_t)i * b.maxM0 + j] = (int32_t)l[1 + j];
+        }
+        g->max_level = b.maxlevel;
+        g->entry = (int32_t)b.entry;
+        g->ru = (int32_t)b.maxM;
+        g->levels.assign(b.levels.begin(), b.levels.end());
+        g->upper_off.assign((size_t)n, -1);
+        int64_t rows = 0;
+        for (int64_t i = 0; i < n; i++)
+            if (b.levels[(size_t)i] > 0) {
+                g->upper_off[(size_t)i] = rows;
+                rows += b.levels[(size_t)i];
+            }
+        g->adj_up.assign((size_t)rows * b.maxM, -1);
+        for (int64_t i = 0; i < n; i++) {
+            int L = b.levels[(size_t)i];
+            for (int l = 1; l <= L; l++) {
+                const uint32_t *src = &b.up[(size_t)i][(size_t)(l - 1) * (1 + b.maxM)];
+                int32_t *dst = &g->adj_up[(size_t)(g->upper_off[(size_t)i] + l - 1) * b.maxM];
+                for (uint32_t j = 0; j < src[0]; j++) dst[j] = (int32_t)src[1 + j];
+            }
+        }
+        g->labels.resize((size_t)n);
+        for (int64_t i = 0; i < n; i++) g->labels[(size_t)i] = i;
+        g->M = M;
+        g->ef_construction = (int32_t)b.efc;
+        g->mult = b.mult;
+        *out = g;
+        return GVS_OK;
+    } catch (const std::bad_alloc &) {
+        gvs::set_error("gvs_hnsw_build: out of host memory");
+        return GVS_ERR_NOMEM;
+    }
+}
+
+// Flat (NSG-style) graph from the HNSW base layer: rows pruned to <= R with
+// the occlusion rule of getNeighborsByHeuristic2, entry = navigating node
+// (the node closest to the centroid of the data), no upper layers.
+extern "C" int gvs_nsg_from_hnsw(const gvs_host_graph *from, int32_t R, gvs_host_graph **out)
+{
+    if (!from || !out || R < 1 || R > 1024) {
+        gvs::set_error("gvs_nsg_from_hnsw: invalid argument");
+        return GVS_ERR_ARG;
+    }
+    const int64_t n = from->n;
+    const int dim = from->dim;
+    gvs_host_graph *g = new gvs_host_graph();
+    g->n = n;
+    g->dim = dim;
+    g->metric = from->metric;
+    g->vectors = from->vectors;
+    g->r0 = R;
+    g->adj0.assign((size_t)n * R, -1);
+    auto vec = [&](int64_t i) { return from->vectors.data() + (size_t)i * dim; };
+    for (int64_t i = 0; i < n; i++) {
+        const int32_t *row = &from->adj0[(size_t)i * from->r0];
+        int cnt = 0;
+        while (cnt < from->r0 && row[cnt] >= 0) cnt++;
+        if (cnt <= R) {
+            for (int j = 0; j < cnt; j++) g->adj0[(size_t)i * R + j] = row[j];
+            continue;
+        }
+        std::vector<std::pair<float, int32_t>> c;
+        for (int j = 0; j < cnt; j++) c.emplace_back(dist_fast(vec(i), vec(row[j]), dim, from->metric), row[j]);
+        std::sort(c.begin(), c.end());
+        std::vector<int32_t> keep;
+        for (auto &p : c) {
+            if ((int)keep.size() >= R) break;
+            bool good = true;
+            for (int32_t s : keep)
+                if (dist_fast(vec(s), vec(p.second), dim, from->metric) < p.first) {
+                    good = false;
+                    break;
+                }
+            if (good) keep.push_back(p.second);
+        }
+        // fill up to R with the nearest pruned ones (fixed out-degree)
+        for (auto &p : c) {
+            if ((int)keep.size() >= R) break;
+            if (std::find(keep.begin(), keep.end(), p.second) == keep.end()) keep.push_back(p.second);
+        }
+        for (size_t j = 0; j < keep.size(); j++) g->adj0[(size_t)i * R + j] = keep[j];
+    }
+    // navigating node
+    std::vector<double> cen((size_t)dim, 0.0);
+    for (int64_t i = 0; i < n; i++)
+        for (int d = 0; d < dim; d++) cen[(size_t)d] += vec(i)[d];
+    std::vector<float> cf((size_t)dim);
+    for (int d = 0; d < dim; d++) cf[(size_t)d] = (float)(cen[(size_t)d] / (double)n);
+    float best = INFINITY;
+    int32_t bi = 0;
+    for (int64_t i = 0; i < n; i++) {
+        float d = dist_fast(cf.data(), vec(i), dim, GVS_METRIC_L2);
+        if (d < best) {
+            best = d;
+            bi = (int32_t)i;
+        }
+    }
+    g->max_level = 0;
+    g->entry = bi;
+    g->ru = 0;
+    g->levels.assign((size_t)n, 0);
+    g->labels = from->labels;
+    *out = g;
+    return GVS_OK;
+}

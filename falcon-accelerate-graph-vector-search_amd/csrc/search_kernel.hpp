@@ -1,0 +1,279 @@
+// Device code for graph search on gfx950 (CDNA4, wave64).
+//
+// One wavefront owns one query (inter-query parallelism, the reference's
+// per-channel pipelines INTER/per_channel_processing_wrapper.hpp:11-185).
+// Per wave, in LDS:
+//   pool      ef keys, ascending, double-buffered.  key = ord(dist) << 32 |
+//             id << 1 | expanded.  The pool is hnswlib's top_candidates; the
+//             unexpanded pool entries are its candidate_set restricted to
+//             entries that can still be expanded (searchBaseLayerST,
+//             HNSWLIB/hnswalg.h:240-326), so one sorted array replaces both
+//             heaps and the FPGA's register queues (INTER/priority_queue.hpp).
+//   bloom     blocked Bloom filter, one 32-bit word and 3 bits per id,
+//             MurmurHash2_KeyLen4 (INTER/bloom_filter.hpp:43-66, :174-188).
+//   list      ids to evaluate this step and their (dist, id) keys.
+// In HBM, per query slot: an open-addressing hash of visited ids (exact).
+// Every neighbour id is inserted with atomicCAS; the CAS winner is the only
+// lane that evaluates the id, so the visited set is exact (the FPGA Bloom
+// filter alone skips nodes on false positives).  Ids the Bloom filter has
+// never seen are gathered speculatively while their CAS is in flight, which
+// takes the hash round trip off the critical path.
+//
+// Distances: 8 lanes per row, 16 bytes per lane per load (one 128-byte line
+// per row per load), 8 rows per pass, 8 passes per 64 rows, partials
+// reduced by an xor butterfly across the 8 lanes (see orc_dist_wave_e in
+// the oracle for the exact summation order).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gvsk {
+
+constexpr int WAVE = 64;
+constexpr uint64_t EMPTY_KEY = ~0ull;
+constexpr int MAX_ROWS = 8;        // mc * ceil(r0/64) adjacency rows per step
+constexpr int STORE_F32 = 0, STORE_BF16 = 1, STORE_FP8 = 2;
+constexpr int METRIC_L2 = 0, METRIC_IP = 1;
+
+struct KParams {
+    const void *vec;           // traversal copy (f32 / bf16 / e4m3)
+    const float *vscale;       // e4m3 per-vector scale
+    const float *vec_f32;      // fp32 copy (rerank)
+    int64_t row_elems;         // stored row length of the traversal copy
+    int64_t row_elems_f32;     // stored row length of the fp32 copy
+    int dim;
+    int r0;
+    const int32_t *adj0;
+    const int32_t *adj_up;
+    const int64_t *upper_off;
+    int ru;
+    int max_level;
+    int entry;
+    const int64_t *labels;
+    const float *queries;
+    int64_t ldq;
+    int64_t nq;
+    int64_t q_base;            // first query of this launch (non-persistent chunks)
+    const int32_t *entries;
+    int k, ef, mc;
+    int rerank;
+    uint32_t *vtab;
+    int vlog2;
+    int blog2;
+    uint32_t bloom_seed;
+    int ef_cap;                // ef rounded up to 64
+    int lcap;                  // list capacity (mc * rows * 64)
+    int lds_per_wave;          // bytes
+    int waves_per_block;
+    int persistent;
+    int64_t *out_labels;
+    float *out_dists;
+    int32_t *out_stats;
+    int32_t *status;           // per query: 0 ok, 1 visited-table overflow
+    uint32_t *counters;        // [0] work queue head, [1] overflow count
+    // per-query k / ef (serving); NULL = use k / ef above
+    const int32_t *q_k;
+    const int32_t *q_ef;
+    const int64_t *q_index;    // serving: slot -> query row; NULL = identity
+};
+
+__device__ __forceinline__ uint32_t ord_f(float f)
+{
+    uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float unord_f(uint32_t o)
+{
+    uint32_t u = (o & 0x80000000u) ? (o & 0x7FFFFFFFu) : ~o;
+    return __uint_as_float(u);
+}
+__device__ __forceinline__ uint64_t mk_key(float d, uint32_t id)
+{
+    return ((uint64_t)ord_f(d) << 32) | ((uint64_t)id << 1);
+}
+__device__ __forceinline__ uint32_t key_id(uint64_t k) { return (uint32_t)(k >> 1) & 0x7FFFFFFFu; }
+__device__ __forceinline__ float key_dist(uint64_t k) { return unord_f((uint32_t)(k >> 32)); }
+
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ int mbcnt(uint64_t m)
+{
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int m)
+{
+    uint32_t lo = __shfl_xor((int)(uint32_t)v, m);
+    uint32_t hi = __shfl_xor((int)(uint32_t)(v >> 32), m);
+    return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ uint64_t wave_min_u64(uint64_t v)
+{
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+        uint64_t o = shfl_xor_u64(v, m);
+        v = o < v ? o : v;
+    }
+    return v;
+}
+__device__ __forceinline__ uint64_t readlane_u64(uint64_t v, int l)
+{
+    uint32_t lo = __builtin_amdgcn_readlane((uint32_t)v, l);
+    uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(v >> 32), l);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ uint32_t murmur2_k4(uint32_t key, uint32_t seed)
+{
+    const uint32_t m = 0x5bd1e995u;
+    uint32_t k = key * m;
+    k ^= k >> 24;
+    k *= m;
+    uint32_t h = (seed ^ 4u) * m;
+    h ^= k;
+    h ^= h >> 13;
+    h *= m;
+    h ^= h >> 15;
+    return h;
+}
+
+// e4m3 (OCP) -> f32, exact
+__device__ __forceinline__ float e4m3_to_f32(uint32_t v)
+{
+    uint32_t s = (v & 0x80u) << 24;
+    uint32_t e = (v >> 3) & 15u, m = v & 7u;
+    float sub = (float)m * 0x1p-9f;
+    float nor = __uint_as_float(s | ((e + 120u) << 23) | (m << 20));
+    float r = e ? nor : (s ? -sub : sub);
+    return r;
+}
+
+template <int STORE>
+struct StoreTraits;
+template <>
+struct StoreTraits<STORE_F32> {
+    static constexpr int E = 4;
+    static constexpr int BYTES = 4;
+};
+template <>
+struct StoreTraits<STORE_BF16> {
+    static constexpr int E = 8;
+    static constexpr int BYTES = 2;
+};
+template <>
+struct StoreTraits<STORE_FP8> {
+    static constexpr int E = 16;
+    static constexpr int BYTES = 1;
+};
+
+// decode one 16-byte chunk into E floats
+template <int STORE>
+__device__ __forceinline__ void decode_chunk(const uint4 raw, float scale, float *x)
+{
+    if constexpr (STORE == STORE_F32) {
+        x[0] = __uint_as_float(raw.x);
+        x[1] = __uint_as_float(raw.y);
+        x[2] = __uint_as_float(raw.z);
+        x[3] = __uint_as_float(raw.w);
+    } else if constexpr (STORE == STORE_BF16) {
+        const uint32_t w[4] = {raw.x, raw.y, raw.z, raw.w};
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            x[2 * i] = __uint_as_float(w[i] << 16);
+            x[2 * i + 1] = __uint_as_float(w[i] & 0xFFFF0000u);
+        }
+    } else {
+        const uint32_t w[4] = {raw.x, raw.y, raw.z, raw.w};
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int b = 0; b < 4; b++) x[4 * i + b] = e4m3_to_f32((w[i] >> (8 * b)) & 0xFFu) * scale;
+    }
+}
+
+// partial distance of this lane's chunks of row `id`
+template <int NCH, int STORE, int METRIC>
+__device__ __forceinline__ float row_partial(const KParams &P, const float *qf, uint32_t id, int p)
+{
+    constexpr int E = StoreTraits<STORE>::E;
+    constexpr int B = StoreTraits<STORE>::BYTES;
+    const unsigned char *row = (const unsigned char *)P.vec + (size_t)id * (size_t)P.row_elems * B;
+    float scale = 1.0f;
+    if constexpr (STORE == STORE_FP8) scale = P.vscale[id];
+    float acc = 0.0f;
+#pragma unroll
+    for (int i = 0; i < NCH; i++) {
+        uint4 raw = *(const uint4 *)(row + (size_t)(p + 8 * i) * 16);
+        float x[E];
+        decode_chunk<STORE>(raw, scale, x);
+#pragma unroll
+        for (int t = 0; t < E; t++) {
+            if constexpr (METRIC == METRIC_L2) {
+                float d = x[t] - qf[i * E + t];
+                acc = __builtin_fmaf(d, d, acc);
+            } else {
+                acc = __builtin_fmaf(x[t], qf[i * E + t], acc);
+            }
+        }
+    }
+    return acc;
+}
+
+// Transposed 8-lane reduction: in acc[t] = partial of pass t; out = full sum
+// of pass p (this lane's position in its 8-lane group).
+__device__ __forceinline__ float reduce8(const float acc[8], int p)
+{
+    const bool b2 = p & 4, b1 = p & 2, b0 = p & 1;
+    float v4[4], v2[2];
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        float send = b2 ? acc[t] : acc[t + 4];
+        float keep = b2 ? acc[t + 4] : acc[t];
+        v4[t] = keep + __shfl_xor(send, 4);
+    }
+#pragma unroll
+    for (int t = 0; t < 2; t++) {
+        float send = b1 ? v4[t] : v4[t + 2];
+        float keep = b1 ? v4[t + 2] : v4[t];
+        v2[t] = keep + __shfl_xor(send, 2);
+    }
+    float send = b0 ? v2[0] : v2[1];
+    float keep = b0 ? v2[1] : v2[0];
+    return keep + __shfl_xor(send, 1);
+}
+
+// Distances of rows lids[0..n) -> keys[0..n).  All 64 lanes take part.
+template <int NCH, int STORE, int METRIC>
+__device__ void eval_rows(const KParams &P, const float *qf, const uint32_t *lids, int n, uint64_t *keys)
+{
+    const int lane = lane_id();
+    const int p = lane & 7, g = lane >> 3;
+    for (int b0 = 0; b0 < n; b0 += WAVE) {
+        const int nb = min(WAVE, n - b0);
+        const int npass = (nb + 7) >> 3;
+        float acc[8];
+#pragma unroll
+        for (int t = 0; t < 8; t++) {
+            acc[t] = 0.0f;
+            if (t < npass) {
+                int r = b0 + t * 8 + g;
+                if (r < n) acc[t] = row_partial<NCH, STORE, METRIC>(P, qf, lids[r], p);
+            }
+        }
+        float v = reduce8(acc, p);
+        int r = b0 + p * 8 + g;
+        if (r < n) {
+            float d = (METRIC == METRIC_L2) ? v : 1.0f - v;
+            keys[r] = mk_key(d, lids[r]);
+        }
+    }
+}
+
+}  // namespace gvsk

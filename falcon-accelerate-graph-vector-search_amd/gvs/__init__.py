@@ -1,0 +1,286 @@
+"""ctypes binding of libgvs.so (the C ABI in include/gvs.h).
+
+Mirrors the Python surface of hnswlib (``Index(space, dim)``,
+``init_index``/``add_items`` (build), ``set_ef``, ``knn_query``,
+``load_index``/``save_index``) that the reference's scripts drive
+(vector_search_baselines/scripts_hnsw/construct_and_search_hnsw.py:126-148,
+vector_search_baselines/scripts_hnsw/hnsw.py:379-517), with the search running
+on the GPU through the HIP library.  There is no CPU search fallback: if the
+library is missing, importing the module raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(_PKG, "lib", "libgvs.so")
+
+GVS_OK = 0
+ERRORS = {-1: "ARG", -2: "HIP", -3: "NOMEM", -4: "UNSUPPORTED", -5: "OVERFLOW", -6: "IO", -7: "STATE"}
+L2, IP = 0, 1
+F32, BF16, FP8 = 0, 1, 2
+STATS_PER_QUERY = 4
+
+
+class GvsError(RuntimeError):
+    def __init__(self, rc, msg):
+        super().__init__(f"gvs error {rc} ({ERRORS.get(rc, '?')}): {msg}")
+        self.rc = rc
+
+
+class Graph(C.Structure):
+    _fields_ = [
+        ("n", C.c_int64), ("dim", C.c_int32), ("metric", C.c_int32),
+        ("vectors", C.c_void_p), ("ld_vectors", C.c_int64),
+        ("r0", C.c_int32), ("adj0", C.c_void_p),
+        ("max_level", C.c_int32), ("entry", C.c_int32),
+        ("ru", C.c_int32), ("up_rows", C.c_int64),
+        ("upper_off", C.c_void_p), ("adj_up", C.c_void_p), ("labels", C.c_void_p),
+    ]
+
+
+class Params(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in (
+        "k", "ef", "mc", "mg", "waves_per_query", "storage", "rerank", "persistent",
+        "visited_log2", "bloom_log2", "grid_waves", "reserved")]
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in (
+        "max_inflight", "max_k", "max_ef", "queue_capacity", "grid_waves", "mc", "storage", "reserved")]
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+    lib = C.CDLL(LIB_PATH)
+    P, I32, I64, U32, U64 = C.c_void_p, C.c_int32, C.c_int64, C.c_uint32, C.c_uint64
+    sig = {
+        "gvs_last_error": (C.c_char_p, []),
+        "gvs_version": (C.c_char_p, []),
+        "gvs_hnsw_build": (C.c_int, [P, I64, I32, I32, I32, I32, U64, I32, I32, P]),
+        "gvs_nsg_from_hnsw": (C.c_int, [P, I32, P]),
+        "gvs_hnswlib_load": (C.c_int, [C.c_char_p, I32, I32, P]),
+        "gvs_hnswlib_save": (C.c_int, [P, C.c_char_p]),
+        "gvs_nsg_load": (C.c_int, [C.c_char_p, P, I64, I32, I32, P]),
+        "gvs_host_graph_view": (C.c_int, [P, P]),
+        "gvs_host_graph_free": (None, [P]),
+        "gvs_gen_mixture": (C.c_int, [U64, U64, I64, I32, I32, C.c_float, C.c_float, I32, P, I32]),
+        "gvs_index_create": (C.c_int, [P, U32, P]),
+        "gvs_index_destroy": (C.c_int, [P]),
+        "gvs_index_device_bytes": (C.c_int, [P, P, P]),
+        "gvs_search_device": (C.c_int, [P, P, I64, I64, P, P, P, P, P, P]),
+        "gvs_search_host": (C.c_int, [P, P, I64, I64, P, P, P, P, P]),
+        "gvs_bruteforce_f64_device": (C.c_int, [P, P, I64, I64, I32, P, P, P]),
+        "gvs_server_open": (C.c_int, [P, P, P]),
+        "gvs_server_submit": (I64, [P, P, I32, I32]),
+        "gvs_server_poll": (C.c_int, [P, I32, P, P, P, P]),
+        "gvs_server_cancel": (C.c_int, [P, I64]),
+        "gvs_server_stats": (C.c_int, [P, P]),
+        "gvs_server_close": (C.c_int, [P]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name, None)
+        if fn is None:
+            continue
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+def exported_symbols():
+    """Names of the C ABI functions this binding knows, present in the library."""
+    names = ["gvs_last_error", "gvs_version", "gvs_hnsw_build", "gvs_nsg_from_hnsw", "gvs_hnswlib_load",
+             "gvs_hnswlib_save", "gvs_nsg_load", "gvs_host_graph_view", "gvs_host_graph_free",
+             "gvs_gen_mixture", "gvs_index_create", "gvs_index_destroy", "gvs_index_device_bytes",
+             "gvs_search_device", "gvs_search_host", "gvs_bruteforce_f64_device", "gvs_server_open",
+             "gvs_server_submit", "gvs_server_poll", "gvs_server_cancel", "gvs_server_stats",
+             "gvs_server_close"]
+    return [n for n in names if hasattr(lib, n)]
+
+
+def _check(rc):
+    if rc != GVS_OK:
+        raise GvsError(rc, lib.gvs_last_error().decode())
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def gen_mixture(n, dim, ncenters, seed_centers, seed_points, center_std=4.0, noise_std=1.0,
+                normalize=False, nthreads=8):
+    """Seeded Gaussian mixture (float32, n x dim)."""
+    out = np.empty((n, dim), dtype=np.float32)
+    _check(lib.gvs_gen_mixture(seed_centers, seed_points, n, dim, ncenters, center_std, noise_std,
+                               int(bool(normalize)), _ptr(out), nthreads))
+    return out
+
+
+class HostGraph:
+    """Owned host graph (HNSW levels + base layer, or flat)."""
+
+    def __init__(self, handle):
+        self._h = C.c_void_p(handle) if isinstance(handle, int) else handle
+        self._view = Graph()
+        _check(lib.gvs_host_graph_view(self._h, C.byref(self._view)))
+
+    @classmethod
+    def build_hnsw(cls, x, M=16, ef_construction=200, seed=100, metric=L2, nthreads=8, exact_order=False):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        h = C.c_void_p()
+        _check(lib.gvs_hnsw_build(_ptr(x), x.shape[0], x.shape[1], metric, M, ef_construction, seed,
+                                  nthreads, int(bool(exact_order)), C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def load_hnswlib(cls, path, dim, metric=L2):
+        h = C.c_void_p()
+        _check(lib.gvs_hnswlib_load(path.encode(), dim, metric, C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def load_nsg(cls, path, x, metric=L2):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        h = C.c_void_p()
+        _check(lib.gvs_nsg_load(path.encode(), _ptr(x), x.shape[0], x.shape[1], metric, C.byref(h)))
+        return cls(h)
+
+    def to_nsg(self, R):
+        h = C.c_void_p()
+        _check(lib.gvs_nsg_from_hnsw(self._h, R, C.byref(h)))
+        return HostGraph(h)
+
+    def save_hnswlib(self, path):
+        _check(lib.gvs_hnswlib_save(self._h, path.encode()))
+
+    @property
+    def view(self):
+        return self._view
+
+    def arrays(self):
+        """numpy copies of the flat arrays (for the oracle and tests)."""
+        v = self._view
+
+        def arr(ptr, count, dt):
+            if not ptr or count == 0:
+                return np.zeros(0, dtype=dt)
+            buf = (C.c_char * (count * np.dtype(dt).itemsize)).from_address(ptr)
+            return np.frombuffer(buf, dtype=dt).copy()
+
+        out = {
+            "n": v.n, "dim": v.dim, "metric": v.metric, "r0": v.r0, "max_level": v.max_level,
+            "entry": v.entry, "ru": v.ru,
+            "vectors": arr(v.vectors, v.n * v.dim, np.float32).reshape(v.n, v.dim),
+            "adj0": arr(v.adj0, v.n * v.r0, np.int32).reshape(v.n, v.r0),
+            "upper_off": arr(v.upper_off, v.n if v.max_level > 0 else 0, np.int64),
+            "adj_up": arr(v.adj_up, v.up_rows * v.ru, np.int32).reshape(-1, max(v.ru, 1)),
+            "labels": arr(v.labels, v.n, np.int64),
+        }
+        return out
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and lib is not None:
+            lib.gvs_host_graph_free(h)
+            self._h = None
+
+
+def make_params(k=10, ef=64, mc=1, mg=1, waves_per_query=1, storage=F32, rerank=False, persistent=False,
+                visited_log2=0, bloom_log2=0, grid_waves=0):
+    return Params(k, ef, mc, mg, waves_per_query, storage, int(bool(rerank)), int(bool(persistent)),
+                  visited_log2, bloom_log2, grid_waves, 0)
+
+
+class DeviceIndex:
+    """Graph + vectors resident on the current HIP device."""
+
+    def __init__(self, graph, storages=(F32,)):
+        mask = 0
+        for s in storages:
+            mask |= 1 << s
+        self._graph = graph  # keep host arrays alive for the view
+        self._h = C.c_void_p()
+        _check(lib.gvs_index_create(C.byref(graph.view), mask, C.byref(self._h)))
+        self.dim = graph.view.dim
+        self.n = graph.view.n
+
+    @property
+    def handle(self):
+        return self._h
+
+    def search(self, queries, k=10, ef=64, entries=None, with_stats=False, **kw):
+        """searchKnn over a batch of host queries -> (labels, dists[, stats])."""
+        q = np.ascontiguousarray(queries, dtype=np.float32)
+        if q.ndim == 1:
+            q = q[None, :]
+        nq = q.shape[0]
+        lab = np.empty((nq, k), dtype=np.int64)
+        dist = np.empty((nq, k), dtype=np.float32)
+        st = np.empty((nq, STATS_PER_QUERY), dtype=np.int32) if with_stats else None
+        ent = None if entries is None else np.ascontiguousarray(entries, dtype=np.int32)
+        p = make_params(k=k, ef=ef, **kw)
+        _check(lib.gvs_search_host(self._h, _ptr(q), nq, q.shape[1], _ptr(ent), C.byref(p), _ptr(lab),
+                                   _ptr(dist), _ptr(st)))
+        return (lab, dist, st) if with_stats else (lab, dist)
+
+    def search_device(self, q_ptr, nq, ldq, labels_ptr, dists_ptr, k=10, ef=64, entries_ptr=None,
+                      stats_ptr=None, stream=None, **kw):
+        """Search with device pointers (e.g. torch tensors' data_ptr())."""
+        p = make_params(k=k, ef=ef, **kw)
+        _check(lib.gvs_search_device(self._h, C.c_void_p(q_ptr), nq, ldq,
+                                     C.c_void_p(entries_ptr) if entries_ptr else None, C.byref(p),
+                                     C.c_void_p(labels_ptr), C.c_void_p(dists_ptr),
+                                     C.c_void_p(stats_ptr) if stats_ptr else None,
+                                     C.c_void_p(stream) if stream else None))
+
+    def bruteforce_f64_device(self, q_ptr, nq, ldq, k, ids_ptr, dists_ptr, stream=None):
+        _check(lib.gvs_bruteforce_f64_device(self._h, C.c_void_p(q_ptr), nq, ldq, k, C.c_void_p(ids_ptr),
+                                             C.c_void_p(dists_ptr), C.c_void_p(stream) if stream else None))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and lib is not None:
+            lib.gvs_index_destroy(h)
+            self._h = None
+
+
+class Index:
+    """hnswlib-style front end: Index(space, dim) -> init_index / add_items /
+    set_ef / knn_query, with the search on the GPU."""
+
+    def __init__(self, space, dim):
+        self.space = space
+        self.dim = dim
+        self.metric = L2 if space == "l2" else IP
+        self.ef = 10
+        self._graph = None
+        self._dev = None
+        self._params = {}
+
+    def init_index(self, max_elements, M=16, ef_construction=200, random_seed=100):
+        self._params = dict(M=M, ef_construction=ef_construction, seed=random_seed)
+
+    def add_items(self, data, num_threads=8, exact_order=False):
+        self._graph = HostGraph.build_hnsw(data, metric=self.metric, nthreads=num_threads,
+                                           exact_order=exact_order, **self._params)
+        self._dev = None
+
+    def load_index(self, path):
+        self._graph = HostGraph.load_hnswlib(path, self.dim, self.metric)
+        self._dev = None
+
+    def save_index(self, path):
+        self._graph.save_hnswlib(path)
+
+    def set_ef(self, ef):
+        self.ef = ef
+
+    def knn_query(self, data, k=1, **kw):
+        if self._dev is None:
+            self._dev = DeviceIndex(self._graph)
+        return self._dev.search(data, k=k, ef=self.ef, **kw)

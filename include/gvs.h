@@ -1,0 +1,197 @@
+/*
+ * gvs.h - C ABI of the MI355X graph vector search library (libgvs.so).
+ *
+ * The library searches an HNSW (or flat NSG-style) graph whose adjacency
+ * lists and vectors are resident in HBM, one wavefront per query by default.
+ * All entry points are plain C: pointers, sizes and int status codes; no
+ * entry point exits the process.  Errors return a negative gvs_status and
+ * set a thread-local message readable with gvs_last_error().
+ *
+ * Reference interfaces replaced (paths relative to the reference root):
+ *   INTER = FPGA_multi_DDR/FPGA_inter_query_v1.3_longer_FIFO_alt_PR/src
+ *   HNSWLIB = networked_FPGA/CPU_programs_oldest/hnswlib
+ */
+#ifndef GVS_H
+#define GVS_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GVS_API __attribute__((visibility("default")))
+
+typedef enum {
+    GVS_OK = 0,
+    GVS_ERR_ARG = -1,          /* invalid argument */
+    GVS_ERR_HIP = -2,          /* HIP runtime error (message has the HIP error string) */
+    GVS_ERR_NOMEM = -3,        /* host or device allocation failed */
+    GVS_ERR_UNSUPPORTED = -4,  /* shape/mode not compiled in (e.g. dim > 512) */
+    GVS_ERR_OVERFLOW = -5,     /* a per-query visited table overflowed after retries */
+    GVS_ERR_IO = -6,           /* file could not be read or has a bad format */
+    GVS_ERR_STATE = -7         /* call not valid in the object's current state */
+} gvs_status;
+
+typedef enum { GVS_METRIC_L2 = 0, GVS_METRIC_IP = 1 } gvs_metric;
+typedef enum { GVS_STORE_F32 = 0, GVS_STORE_BF16 = 1, GVS_STORE_FP8 = 2 } gvs_storage;
+
+/* Flat host description of a graph.  Layer 0: n rows of r0 int32 neighbour
+ * ids, unused slots = -1.  Upper layers: node v with level L >= 1 owns rows
+ * upper_off[v] .. upper_off[v]+L-1 of adj_up (row l-1 = level l), each ru
+ * int32 ids, unused = -1; upper_off[v] = -1 for level-0 nodes.  This is the
+ * information of hnswlib's level-0 block and linkLists_
+ * (HNSWLIB/hnswalg.h:44-66, :371-381) and of the FPGA layout's
+ * ground_links / upper_links files (vector_search_baselines/scripts_hnsw/hnsw.py:264-377). */
+typedef struct gvs_graph {
+    int64_t n;
+    int32_t dim;
+    int32_t metric;            /* gvs_metric */
+    const float *vectors;      /* n rows */
+    int64_t ld_vectors;        /* row stride in floats, >= dim */
+    int32_t r0;                /* layer-0 row width (maxM0) */
+    const int32_t *adj0;       /* n * r0 */
+    int32_t max_level;         /* 0 for a flat graph */
+    int32_t entry;             /* enterpoint_node_ (or NSG navigating node) */
+    int32_t ru;                /* upper row width (maxM) */
+    int64_t up_rows;           /* rows in adj_up */
+    const int64_t *upper_off;  /* n, may be NULL when max_level == 0 */
+    const int32_t *adj_up;     /* up_rows * ru, may be NULL when max_level == 0 */
+    const int64_t *labels;     /* n external labels, NULL = identity */
+} gvs_graph;
+
+/* Search parameters.  k, ef as in hnswlib (ef_eff = max(ef, k),
+ * HNSWLIB/hnswalg.h:1150-1156).  mc/mg as the reference's DST scheduler
+ * (max_cand_batch_size / max_async_stage_num, INTER/scheduler.hpp:100-157);
+ * mc = mg = 1 is plain best-first search. */
+typedef struct gvs_params {
+    int32_t k;
+    int32_t ef;
+    int32_t mc;                /* candidates expanded per step (1..8) */
+    int32_t mg;                /* steps in flight per query (1..2) */
+    int32_t waves_per_query;   /* 1 = inter-query; 2,4,8 = intra-query (one workgroup per query) */
+    int32_t storage;           /* gvs_storage traversed; the copy must exist */
+    int32_t rerank;            /* 1: exact fp32 rerank of the final ef candidates */
+    int32_t persistent;        /* 1: persistent kernel draining a device work queue */
+    int32_t visited_log2;      /* log2 HBM visited-hash slots per query; 0 = auto */
+    int32_t bloom_log2;        /* log2 LDS Bloom words per query; 0 = auto */
+    int32_t grid_waves;        /* persistent mode: resident waves; 0 = auto */
+    int32_t reserved;
+} gvs_params;
+
+/* Per-query debug counters written when stats != NULL (4 int32 per query):
+ * layer-0 hops, layer-0 distance evaluations, upper-layer list scans,
+ * upper-layer distance evaluations.  The reference's mem_debug carries
+ * {hops, visited} per query (INTER/vadd.cpp:88-94, INTER/scheduler.hpp:98-166). */
+#define GVS_STATS_PER_QUERY 4
+
+typedef struct gvs_index gvs_index;
+typedef struct gvs_host_graph gvs_host_graph;
+
+GVS_API const char *gvs_last_error(void);
+GVS_API const char *gvs_version(void);
+
+/* ---- host-side graph construction and formats ---------------------- */
+
+/* HNSW build with hnswlib's addPoint semantics (HNSWLIB/hnswalg.h:982-1112:
+ * level draw, greedy descent, searchBaseLayer with efConstruction,
+ * getNeighborsByHeuristic2, mutuallyConnectNewElement).  nthreads = 1 gives
+ * hnswlib's single-thread graph; exact_order = 1 sums distances in hnswlib's
+ * scalar order (bit-identical graph), 0 uses a faster order. */
+GVS_API int gvs_hnsw_build(const float *x, int64_t n, int32_t dim, int32_t metric,
+                           int32_t M, int32_t ef_construction, uint64_t seed,
+                           int32_t nthreads, int32_t exact_order, gvs_host_graph **out);
+/* NSG-style flat graph: prune the HNSW layer-0 lists of `from` to out-degree
+ * <= R with the same occlusion rule and set the navigating node (the node
+ * nearest to the dataset centroid) as entry; vector_search_baselines/scripts_nsg/nsg_to_FPGA.py:56-137
+ * defines the format this mirrors (fixed-width rows, single entry point). */
+GVS_API int gvs_nsg_from_hnsw(const gvs_host_graph *from, int32_t R, gvs_host_graph **out);
+/* hnswlib .bin reader / writer (HNSWLIB/hnswalg.h:584-700). */
+GVS_API int gvs_hnswlib_load(const char *path, int32_t dim, int32_t metric, gvs_host_graph **out);
+GVS_API int gvs_hnswlib_save(const gvs_host_graph *g, const char *path);
+/* NSG .nsg reader (width, ep, then per node k, ids[k];
+ * vector_search_baselines/scripts_nsg/nsg_to_FPGA.py:31-47); vectors are
+ * attached from x (n x dim, copied). */
+GVS_API int gvs_nsg_load(const char *path, const float *x, int64_t n, int32_t dim,
+                         int32_t metric, gvs_host_graph **out);
+GVS_API int gvs_host_graph_view(const gvs_host_graph *g, gvs_graph *view);
+GVS_API void gvs_host_graph_free(gvs_host_graph *g);
+
+/* Seeded Gaussian mixture: ncenters centres ~ N(0, center_std^2 I) drawn from
+ * seed_centers, point i = centre[u_i] + N(0, noise_std^2 I) from seed_points;
+ * normalize = 1 scales rows to unit L2 norm.  Deterministic for any nthreads. */
+GVS_API int gvs_gen_mixture(uint64_t seed_centers, uint64_t seed_points, int64_t n, int32_t dim,
+                            int32_t ncenters, float center_std, float noise_std,
+                            int32_t normalize, float *out, int32_t nthreads);
+
+/* ---- device index --------------------------------------------------- */
+
+/* Copy the graph to the current HIP device and keep it resident.
+ * storage_mask: bit GVS_STORE_* set = build that vector copy on the device
+ * (F32 is always kept; BF16 / FP8-e4m3 with per-vector scale are optional). */
+GVS_API int gvs_index_create(const gvs_graph *g, uint32_t storage_mask, gvs_index **out);
+GVS_API int gvs_index_destroy(gvs_index *ix);
+/* device pointers of the resident arrays (for tests / introspection) */
+GVS_API int gvs_index_device_bytes(const gvs_index *ix, int64_t *graph_bytes, int64_t *vector_bytes);
+
+/* Batch searchKnn over nq queries held in DEVICE memory (row stride ldq
+ * floats), asynchronous on `stream` (hipStream_t, NULL = default stream).
+ * Replaces the FPGA kernel `vadd` (INTER/vadd.cpp:11-95: query_num, ef, mc,
+ * mg, entry_point_ids, query_vectors, out_id/out_dist, mem_debug) and
+ * hnswlib searchKnn (HNSWLIB/hnswalg.h:1114-1168).  entries (device, nq) =
+ * per-query layer-0 entry point (skips the upper-layer descent, as
+ * INTER/host.cpp:711-713 does) or NULL.  Outputs (device): labels[nq*k],
+ * dists[nq*k] ascending (closer first, HNSWLIB/hnswlib.h:171-185); missing
+ * results are label -1 / dist +inf.  stats: see GVS_STATS_PER_QUERY. */
+GVS_API int gvs_search_device(gvs_index *ix, const float *queries, int64_t nq, int64_t ldq,
+                              const int32_t *entries, const gvs_params *p,
+                              int64_t *labels, float *dists, int32_t *stats, void *stream);
+/* Same with HOST buffers; synchronous (copies in and out). */
+GVS_API int gvs_search_host(gvs_index *ix, const float *queries, int64_t nq, int64_t ldq,
+                            const int32_t *entries, const gvs_params *p,
+                            int64_t *labels, float *dists, int32_t *stats);
+
+/* Exact FP64 brute-force kNN on the device (MFMA f64 GEMM + fused top-k):
+ * device queries [nq x dim] (stride ldq), out ids (int32) and FP64 dists,
+ * ascending, ties by id.  The reference loads ground truth from files
+ * (INTER/host.cpp:148-192); this computes it. */
+GVS_API int gvs_bruteforce_f64_device(const gvs_index *ix, const float *queries, int64_t nq,
+                                      int64_t ldq, int32_t k, int32_t *ids, double *dists,
+                                      void *stream);
+
+/* ---- serving loop ------------------------------------------------------ */
+/* Continuous batching: requests with their own k and ef are admitted to the
+ * device as query slots and LDS/queue memory free up; a persistent kernel
+ * serves them.  Semantics of in-flight windows after the reference's
+ * networked client (networked_FPGA/CPU_programs/CPU_client.cpp:68-130,
+ * :490-620). */
+typedef struct gvs_server gvs_server;
+typedef struct gvs_server_config {
+    int32_t max_inflight;      /* device query slots */
+    int32_t max_k;             /* largest k accepted */
+    int32_t max_ef;            /* largest ef accepted */
+    int32_t queue_capacity;    /* host submission queue length */
+    int32_t grid_waves;        /* persistent workers, 0 = auto */
+    int32_t mc;                /* DST candidates per step used for every request */
+    int32_t storage;           /* vector copy traversed */
+    int32_t reserved;
+} gvs_server_config;
+
+GVS_API int gvs_server_open(gvs_index *ix, const gvs_server_config *cfg, gvs_server **out);
+/* Enqueue one request (host query of dim floats).  Returns a request id >= 0,
+ * or GVS_ERR_STATE when the submission queue is full (caller retries). */
+GVS_API int64_t gvs_server_submit(gvs_server *s, const float *query, int32_t k, int32_t ef);
+/* Copy out up to max_done finished requests: ids[i], labels[i*max_k..],
+ * dists[i*max_k..], latency_us[i] (submit -> completion observed on host).
+ * Returns the number copied (>= 0). */
+GVS_API int gvs_server_poll(gvs_server *s, int32_t max_done, int64_t *req_ids,
+                            int64_t *labels, float *dists, double *latency_us);
+/* Cancel a request that has not completed; returns GVS_OK or GVS_ERR_STATE. */
+GVS_API int gvs_server_cancel(gvs_server *s, int64_t req_id);
+/* counters: submitted, admitted, completed, cancelled, rejected, device_loops */
+GVS_API int gvs_server_stats(gvs_server *s, int64_t counters[6]);
+GVS_API int gvs_server_close(gvs_server *s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

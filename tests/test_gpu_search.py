@@ -1,0 +1,152 @@
+"""GPU parity tests: the HIP search (through the C ABI) against the oracle.
+
+Bar: ids, distances and the per-query counters (hops, evaluations, upper
+hops, upper evaluations) equal to the oracle's in the kernel's summation
+order (bit-exact), on the same seeded inputs.  Against hnswlib's scalar order
+the distances agree within 1e-5 relative (tests/test_oracle.py)."""
+import numpy as np
+import pytest
+
+import gvs
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+_CACHE = {}
+
+
+def graph(n, dim, M, efc, centres=32, metric=gvs.L2, seed=1, normalize=False):
+    key = (n, dim, M, efc, centres, metric, seed, normalize)
+    if key not in _CACHE:
+        x = gvs.gen_mixture(n, dim, centres, seed, seed + 1, normalize=normalize)
+        g = gvs.HostGraph.build_hnsw(x, M=M, ef_construction=efc, metric=metric, nthreads=16)
+        _CACHE[key] = (x, g, gvs.DeviceIndex(g))
+    return _CACHE[key]
+
+
+def queries(nq, dim, centres=32, seed=1, normalize=False):
+    return gvs.gen_mixture(nq, dim, centres, seed, seed + 99, normalize=normalize)
+
+
+def check_equal(dev, og, q, k, ef, mc=1, mg=1, entries=None, **kw):
+    lab, dist, st = dev.search(q, k=k, ef=ef, mc=mc, mg=mg, entries=entries, with_stats=True, **kw)
+    ids, ds, ost = og.search(q, k, ef, mc=mc, mg=mg, order=orc.WAVE, entries=entries)
+    np.testing.assert_array_equal(lab, ids.astype(np.int64))
+    np.testing.assert_array_equal(dist, ds)
+    np.testing.assert_array_equal(st.astype(np.int64), ost)
+    return lab, dist, st
+
+
+@pytest.mark.parametrize("ef", [10, 64, 200])
+def test_plain_l2_d128_matches_oracle(ef):
+    x, g, dev = graph(20000, 128, 16, 100)
+    og = orc.OracleGraph(g.arrays())
+    q = queries(300, 128)
+    print(f"case plain L2 d128 ef={ef}", flush=True)
+    check_equal(dev, og, q, 10, ef)
+
+
+@pytest.mark.parametrize("mc", [2, 4, 8])
+def test_dst_multi_candidate_matches_oracle(mc):
+    x, g, dev = graph(20000, 128, 16, 100)
+    og = orc.OracleGraph(g.arrays())
+    q = queries(300, 128, seed=3)
+    print(f"case dst mc={mc}", flush=True)
+    check_equal(dev, og, q, 10, 64, mc=mc)
+
+
+def test_persistent_equals_plain():
+    x, g, dev = graph(20000, 128, 16, 100)
+    q = queries(2000, 128, seed=4)
+    a = dev.search(q, k=10, ef=64, with_stats=True)
+    b = dev.search(q, k=10, ef=64, with_stats=True, persistent=True)
+    c = dev.search(q, k=10, ef=64, with_stats=True, persistent=True, grid_waves=64)
+    for u, v in zip(a, b):
+        np.testing.assert_array_equal(u, v)
+    for u, v in zip(a, c):
+        np.testing.assert_array_equal(u, v)
+
+
+@pytest.mark.parametrize("dim,M", [(96, 16), (384, 12)])
+def test_ip_unit_norm_matches_oracle(dim, M):
+    x, g, dev = graph(8000, dim, M, 80, metric=gvs.IP, normalize=True)
+    og = orc.OracleGraph(g.arrays())
+    q = queries(200, dim, normalize=True)
+    print(f"case IP d{dim}", flush=True)
+    check_equal(dev, og, q, 10, 64)
+    check_equal(dev, og, q, 100, 128)
+
+
+def test_entries_skip_descent():
+    x, g, dev = graph(20000, 128, 16, 100)
+    og = orc.OracleGraph(g.arrays())
+    q = queries(200, 128, seed=5)
+    ent = np.random.default_rng(0).integers(0, 20000, size=200).astype(np.int32)
+    lab, dist, st = check_equal(dev, og, q, 10, 64, entries=ent)
+    assert np.all(st[:, 2] == 0) and np.all(st[:, 3] == 0)
+
+
+def test_edge_cases():
+    x, g, dev = graph(20000, 128, 16, 100)
+    og = orc.OracleGraph(g.arrays())
+    q = queries(64, 128, seed=6)
+    check_equal(dev, og, q, 1, 1)            # ef = k = 1
+    check_equal(dev, og, q, 50, 10)          # k > ef -> ef_eff = k
+    check_equal(dev, og, q, 100, 256)
+    lab, dist = dev.search(q[:0], k=10, ef=64)
+    assert lab.shape == (0, 10)
+
+
+def test_duplicate_vectors_tie_break_by_id():
+    base = gvs.gen_mixture(3000, 32, 8, 7, 8)
+    x = np.concatenate([base, base[:1000]])  # exact duplicates -> equal distances
+    g = gvs.HostGraph.build_hnsw(x, M=8, ef_construction=64, nthreads=16)
+    dev = gvs.DeviceIndex(g)
+    og = orc.OracleGraph(g.arrays())
+    q = np.concatenate([x[:50], queries(50, 32, centres=8, seed=7)])
+    check_equal(dev, og, q, 10, 40)
+
+
+def test_tiny_graphs():
+    for n in (1, 2, 5):
+        x = gvs.gen_mixture(n, 32, 2, 9, 10)
+        g = gvs.HostGraph.build_hnsw(x, M=4, ef_construction=16, nthreads=1)
+        dev = gvs.DeviceIndex(g)
+        og = orc.OracleGraph(g.arrays())
+        q = queries(8, 32, centres=2, seed=9)
+        check_equal(dev, og, q, 3, 10)
+
+
+def test_flat_nsg_style_graph():
+    x, g, dev = graph(20000, 128, 16, 100)
+    flat = g.to_nsg(32)
+    fdev = gvs.DeviceIndex(flat)
+    og = orc.OracleGraph(flat.arrays())
+    q = queries(200, 128, seed=11)
+    check_equal(fdev, og, q, 10, 64)
+
+
+def test_visited_table_overflow_retry_is_exact():
+    x, g, dev = graph(20000, 128, 16, 100)
+    og = orc.OracleGraph(g.arrays())
+    q = queries(100, 128, seed=12)
+    # smallest table: ef=400 overflows it, the library retries with larger tables
+    check_equal(dev, og, q, 10, 400, visited_log2=11)
+
+
+def test_recall_c1_scaled():
+    """100k x 128, M=16, efC=128, k=10, ef=64 (the reference's CPU case):
+    GPU recall@10 equals the oracle's (same ids) and is reported."""
+    x, g, dev = graph(100000, 128, 16, 128, centres=64, seed=31)
+    q = queries(1000, 128, centres=64, seed=31)
+    og = orc.OracleGraph(g.arrays())
+    lab, dist = dev.search(q, k=10, ef=64)
+    ids, ds, _ = og.search(q, 10, 64, order=orc.WAVE)
+    np.testing.assert_array_equal(lab, ids)
+    gt, _ = orc.bruteforce_f64(x, q, 10)
+    r_gpu = orc.recall_at(lab, gt, 10)
+    ids_seq, _, _ = og.search(q, 10, 64, order=orc.SEQ)
+    r_cpu = orc.recall_at(ids_seq, gt, 10)
+    print(f"recall@10 gpu={r_gpu:.4f} cpu(hnswlib order)={r_cpu:.4f}", flush=True)
+    assert abs(r_gpu - r_cpu) <= 0.001
+    assert r_gpu > 0.9

@@ -1,0 +1,265 @@
+"""CPU tests: the oracle pinned against the reference hnswlib (golden vectors
+and oracle/_ref when present), closed forms and invariants; the host-side
+builder, formats and generator; the C ABI exports."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import gvs
+from oracle import oracle as orc
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = os.path.join(HERE, "golden")
+CASES = ["l2_n1000_d16_M8", "ip_n1000_d24_M6", "l2_n400_d8_M4"]
+
+
+def load_case(name):
+    z = np.load(os.path.join(GOLD, name + ".npz"))
+    g = gvs.HostGraph.load_hnswlib(os.path.join(GOLD, name + ".hnsw.bin"), int(z["x"].shape[1]), int(z["metric"]))
+    return z, g
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_oracle_matches_reference_golden(name):
+    """hnswlib-order oracle == reference searchKnnCloserFirst, bit for bit."""
+    z, g = load_case(name)
+    og = orc.OracleGraph(g.arrays())
+    k = int(z["k"])
+    for ef in z["efs"]:
+        ids, ds, _ = og.search(z["q"], k, int(ef), order=orc.SEQ)
+        np.testing.assert_array_equal(ids, z[f"ids_ef{ef}"])
+        np.testing.assert_array_equal(ds, z[f"dists_ef{ef}"])
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_builder_reproduces_reference_graph(name):
+    """Single-thread builder in hnswlib's scalar order == the reference graph."""
+    z, g = load_case(name)
+    ref = g.arrays()
+    mine = gvs.HostGraph.build_hnsw(z["x"], M=int(z["M"]), ef_construction=int(z["efc"]), seed=100,
+                                    metric=int(z["metric"]), nthreads=1, exact_order=True).arrays()
+    assert mine["entry"] == ref["entry"] and mine["max_level"] == ref["max_level"]
+    np.testing.assert_array_equal(mine["adj0"], ref["adj0"])
+    np.testing.assert_array_equal(mine["upper_off"], ref["upper_off"])
+    np.testing.assert_array_equal(mine["adj_up"], ref["adj_up"])
+    np.testing.assert_array_equal(mine["vectors"], z["x"])
+
+
+def test_hnswlib_save_roundtrip(tmp_path):
+    z, g = load_case(CASES[0])
+    p = str(tmp_path / "rt.bin")
+    g.save_hnswlib(p)
+    g2 = gvs.HostGraph.load_hnswlib(p, int(z["x"].shape[1]))
+    a, b = g.arrays(), g2.arrays()
+    for key in ("adj0", "upper_off", "adj_up", "vectors", "labels"):
+        np.testing.assert_array_equal(a[key], b[key])
+
+
+@pytest.mark.skipif(not os.path.exists(orc.REF_BIN), reason="oracle/_ref not built (no reference tree)")
+def test_oracle_matches_reference_live(tmp_path):
+    """Larger seeded case against the reference binary (when it is built)."""
+    x = gvs.gen_mixture(4000, 64, 32, 5, 6)
+    q = gvs.gen_mixture(200, 64, 32, 5, 7)
+    x.tofile(tmp_path / "x.f32")
+    q.tofile(tmp_path / "q.f32")
+    idx = str(tmp_path / "i.bin")
+    subprocess.run([orc.REF_BIN, "build", str(tmp_path / "x.f32"), "4000", "64", "l2", "12", "100", "100", idx],
+                   check=True)
+    subprocess.run([orc.REF_BIN, "search", idx, "64", "l2", str(tmp_path / "q.f32"), "200", "10", "50",
+                    str(tmp_path / "i"), str(tmp_path / "d")], check=True)
+    rid = np.fromfile(tmp_path / "i", dtype=np.int32).reshape(200, 10)
+    rd = np.fromfile(tmp_path / "d", dtype=np.float32).reshape(200, 10)
+    og = orc.OracleGraph(gvs.HostGraph.load_hnswlib(idx, 64).arrays())
+    ids, ds, _ = og.search(q, 10, 50, order=orc.SEQ)
+    np.testing.assert_array_equal(ids, rid)
+    np.testing.assert_array_equal(ds, rd)
+
+
+def test_wave_order_distance_close_to_f64():
+    rng = np.random.default_rng(0)
+    for dim in (32, 96, 128, 384):
+        a = rng.normal(0, 3, size=(50, dim)).astype(np.float32)
+        b = rng.normal(0, 3, size=(50, dim)).astype(np.float32)
+        for metric in (orc.L2, orc.IP):
+            for i in range(50):
+                w = orc.lib.orc_dist_wave(orc._p(a[i]), orc._p(b[i]), dim, metric)
+                s = orc.lib.orc_dist_seq(orc._p(b[i]), orc._p(a[i]), dim, metric)
+                ref = np.sum((a[i].astype(np.float64) - b[i]) ** 2) if metric == orc.L2 else \
+                    1.0 - np.dot(a[i].astype(np.float64), b[i].astype(np.float64))
+                scale = max(abs(ref), np.sum(np.abs(a[i].astype(np.float64) * b[i])) if metric else abs(ref))
+                assert abs(w - ref) <= 1e-5 * scale + 1e-6
+                assert abs(s - ref) <= 1e-5 * scale + 1e-6
+
+
+def test_wave_order_is_the_stated_butterfly():
+    """Closed form of the summation order on a vector where order matters."""
+    dim = 32
+    a = np.zeros(dim, dtype=np.float32)
+    # chunk c = lanes p = c % 8; give lane partials very different magnitudes
+    for c in range(8):
+        a[4 * c] = np.float32(2.0 ** (12 - 3 * c))
+    b = np.zeros(dim, dtype=np.float32)
+    s = [np.float32(a[4 * p]) ** 2 for p in range(8)]
+    f = np.float32
+    expect = (f(s[0] + s[4]) + f(s[2] + s[6])) + (f(s[1] + s[5]) + f(s[3] + s[7]))
+    got = orc.lib.orc_dist_wave(orc._p(a), orc._p(b), dim, orc.L2)
+    assert np.float32(got) == np.float32(expect)
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_wave_order_search_matches_seq_except_ties(name):
+    z, g = load_case(name)
+    og = orc.OracleGraph(g.arrays())
+    k = int(z["k"])
+    ef = int(z["efs"][1])
+    i1, d1, _ = og.search(z["q"], k, ef, order=orc.SEQ)
+    i2, d2, _ = og.search(z["q"], k, ef, order=orc.WAVE)
+    np.testing.assert_allclose(d2, d1, rtol=1e-5, atol=1e-6)
+    diff = i1 != i2
+    # any id difference must sit on a distance tie (within 1e-5 relative)
+    for r, c in zip(*np.nonzero(diff)):
+        assert abs(d1[r, c] - d2[r, c]) <= 1e-5 * max(abs(d1[r, c]), 1e-6)
+
+
+def test_exhaustive_ef_gives_exact_knn():
+    z, g = load_case("l2_n400_d8_M4")
+    og = orc.OracleGraph(g.arrays())
+    ids, ds, _ = og.search(z["q"], 5, 400, order=orc.SEQ)
+    bi, bd = orc.bruteforce_f64(z["x"], z["q"], 5)
+    np.testing.assert_array_equal(ids, bi)
+    np.testing.assert_allclose(ds, bd, rtol=1e-5)
+
+
+def test_bruteforce_matches_numpy():
+    rng = np.random.default_rng(1)
+    x = rng.normal(size=(700, 20)).astype(np.float32)
+    q = rng.normal(size=(30, 20)).astype(np.float32)
+    for metric in (orc.L2, orc.IP):
+        ids, ds = orc.bruteforce_f64(x, q, 7, metric=metric)
+        x64, q64 = x.astype(np.float64), q.astype(np.float64)
+        if metric == orc.L2:
+            d = ((q64[:, None, :] - x64[None]) ** 2).sum(-1)
+        else:
+            d = 1.0 - q64 @ x64.T
+        order = np.lexsort((np.broadcast_to(np.arange(700), d.shape), d), axis=1)[:, :7]
+        np.testing.assert_array_equal(ids, order)
+        np.testing.assert_allclose(ds, np.take_along_axis(d, order, 1), rtol=1e-12, atol=1e-12)
+
+
+def test_dst_mc1_mg1_equals_plain_and_invariants():
+    x = gvs.gen_mixture(6000, 32, 32, 21, 22)
+    q = gvs.gen_mixture(300, 32, 32, 21, 23)
+    g = gvs.HostGraph.build_hnsw(x, M=12, ef_construction=80, nthreads=8)
+    og = orc.OracleGraph(g.arrays())
+    ids0, d0, st0 = og.search(q, 10, 48)
+    ids1, d1, st1 = og.search(q, 10, 48, mc=1, mg=1)
+    np.testing.assert_array_equal(ids0, ids1)
+    np.testing.assert_array_equal(st0, st1)
+    gt, _ = orc.bruteforce_f64(x, q, 10)
+    r0 = orc.recall_at(ids0, gt, 10)
+    for mc, mg in ((2, 1), (4, 1), (1, 2), (2, 2), (8, 1)):
+        i, d, st = og.search(q, 10, 48, mc=mc, mg=mg)
+        assert orc.recall_at(i, gt, 10) >= r0 - 0.01
+        # delayed synchronisation evaluates at least as many nodes on average
+        assert st[:, 1].mean() >= st0[:, 1].mean() * 0.99
+        assert np.all(np.diff(d, axis=1) >= 0)
+
+
+def test_murmur2_keylen4_matches_generic():
+    rng = np.random.default_rng(2)
+    for key, seed in zip(rng.integers(0, 2**32, 500, dtype=np.uint64), rng.integers(0, 2**32, 500, dtype=np.uint64)):
+        b = np.array([key], dtype=np.uint32)
+        assert orc.lib.orc_murmur2(orc._p(b), 4, int(seed)) == orc.lib.orc_murmur2_keylen4(int(key), int(seed))
+    # empty input with seed 0 hashes to 0 (h = seed ^ len = 0 through the final mix)
+    e = np.zeros(1, dtype=np.uint8)
+    assert orc.lib.orc_murmur2(orc._p(e), 0, 0) == 0
+
+
+def test_bloom_no_false_negatives_and_fp_rate():
+    nwords = 1 << 10
+    words = np.zeros(nwords, dtype=np.uint32)
+    ins = 2500
+    for i in range(ins):
+        assert orc.lib.orc_bloom_test_and_set(orc._p(words), nwords, 7, i * 7919) == 0 or True
+    for i in range(ins):
+        assert orc.lib.orc_bloom_test_and_set(orc._p(words), nwords, 7, i * 7919) == 1
+    fp = 0
+    trials = 20000
+    w2 = words.copy()
+    for i in range(trials):
+        fp += orc.lib.orc_bloom_test_and_set(orc._p(w2), nwords, 7, 10**8 + i)
+        w2[:] = words
+    rate = fp / trials
+    # blocked filter estimate: per-word Poisson load lambda, 3 bits of 32
+    lam = ins / nwords
+    j = np.arange(0, 40)
+    from math import factorial, exp
+    pois = np.array([exp(-lam) * lam ** int(t) / factorial(int(t)) for t in j])
+    est = float(np.sum(pois * (1 - (1 - 1 / 32) ** (3 * j)) ** 3))
+    assert 0.5 * est <= rate <= 1.5 * est + 1e-3
+
+
+def test_low_precision_encoders_match_torch():
+    torch = pytest.importorskip("torch")
+    rng = np.random.default_rng(3)
+    x = np.concatenate([rng.normal(0, 50, 4000), rng.normal(0, 0.01, 2000), [0.0, -0.0, 447.9, -448.0]]).astype(np.float32)
+    tb = torch.from_numpy(x).to(torch.bfloat16).view(torch.int16).numpy().astype(np.uint16)
+    ob = np.array([orc.lib.orc_f32_to_bf16(float(v)) for v in x], dtype=np.uint16)
+    np.testing.assert_array_equal(ob, tb)
+    xs = np.clip(x, -448, 448)
+    t8 = torch.from_numpy(xs).to(torch.float8_e4m3fn).view(torch.uint8).numpy()
+    o8 = np.array([orc.lib.orc_f32_to_e4m3(float(v)) for v in xs], dtype=np.uint8)
+    np.testing.assert_array_equal(o8, t8)
+    back = torch.from_numpy(t8).view(torch.float8_e4m3fn).to(torch.float32).numpy()
+    ob8 = np.array([orc.lib.orc_e4m3_to_f32(int(v)) for v in t8], dtype=np.float32)
+    np.testing.assert_array_equal(ob8, back)
+
+
+def test_gen_mixture_deterministic_and_shaped():
+    a = gvs.gen_mixture(5000, 16, 8, 1, 2, nthreads=1)
+    b = gvs.gen_mixture(5000, 16, 8, 1, 2, nthreads=7)
+    np.testing.assert_array_equal(a, b)
+    c = gvs.gen_mixture(5000, 16, 8, 1, 3)
+    assert not np.array_equal(a, c)
+    # per-point noise ~ N(0,1): within-cluster spread, centres ~ N(0, 16)
+    assert 14.0 < np.var(a, axis=0).mean() < 20.0
+    u = gvs.gen_mixture(100, 16, 8, 1, 2, normalize=True)
+    np.testing.assert_allclose(np.linalg.norm(u, axis=1), 1.0, rtol=1e-5)
+
+
+def test_cabi_exports_every_declared_symbol():
+    hdr = open(os.path.join(os.path.dirname(HERE), "include", "gvs.h")).read()
+    declared = set(re.findall(r"GVS_API\s+[\w\s\*]+?\b(gvs_\w+)\s*\(", hdr))
+    assert len(declared) >= 20
+    out = subprocess.run(["nm", "-D", "--defined-only", gvs.LIB_PATH], capture_output=True, text=True).stdout
+    exported = set(re.findall(r"\bT\s+(gvs_\w+)", out))
+    missing = declared - exported
+    assert not missing, f"declared but not exported: {sorted(missing)}"
+
+
+def test_nsg_format_reader(tmp_path):
+    x = gvs.gen_mixture(300, 8, 4, 1, 2)
+    rng = np.random.default_rng(4)
+    width, ep = 6, 17
+    rows = [rng.choice(300, size=rng.integers(1, width + 1), replace=False) for _ in range(300)]
+    with open(tmp_path / "g.nsg", "wb") as f:
+        np.array([width, ep], dtype=np.uint32).tofile(f)
+        for r in rows:
+            np.array([len(r)], dtype=np.uint32).tofile(f)
+            r.astype(np.uint32).tofile(f)
+    g = gvs.HostGraph.load_nsg(str(tmp_path / "g.nsg"), x)
+    a = g.arrays()
+    assert a["entry"] == ep and a["max_level"] == 0 and a["r0"] == width
+    for i, r in enumerate(rows):
+        np.testing.assert_array_equal(a["adj0"][i, : len(r)], r)
+        assert np.all(a["adj0"][i, len(r):] == -1)
+
+
+def test_invalid_arguments_return_errors():
+    with pytest.raises(gvs.GvsError):
+        gvs.HostGraph.load_hnswlib("/nonexistent/file.bin", 16)
+    with pytest.raises(gvs.GvsError):
+        gvs.HostGraph.build_hnsw(np.zeros((10, 4), np.float32), M=1)
