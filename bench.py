@@ -1,0 +1,232 @@
+#!/usr/bin/env python3
+"""Benchmark: QPS at recall@10 >= 0.95 on a 1M x 128 HNSW index, one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+Workload (synthetic; the reference's SIFT/Deep datasets are not available):
+1,000,000 base vectors, D=128, from a seeded 256-centre Gaussian mixture
+(centres ~ N(0, 16 I), noise N(0, I)); HNSW M=32 (layer-0 degree 64),
+efConstruction=200; 10,000 queries from the same mixture; k=10, L2.
+Ground truth: exact FP64 brute force on the GPU (MFMA kernel).
+
+A step is one search of the whole 10k-query batch with queries, graph and
+vectors resident in HBM.  ef is the smallest of a fixed sweep that reaches
+recall@10 >= 0.95 for the plain one-wavefront-per-query search; the tuned
+path (persistent work-queue kernel, DST candidates per step) is chosen by the
+autotuner among configurations that keep recall@10 >= 0.95.  rank 0 prints
+one JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "falcon-accelerate-graph-vector-search_amd")
+for _p in (ROOT, PKG):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+import numpy as np  # noqa: E402
+
+METRIC = "QPS @ recall@10≥0.95, 1M×128d HNSW; p50/p99 latency under load vs plain GPU"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def recall_at(found, truth, k):
+    f = np.asarray(found)[:, :k]
+    t = np.asarray(truth)[:, :k]
+    return float((f[:, :, None] == t[:, None, :]).any(-1).sum()) / (t.shape[0] * k)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--dim", type=int, default=128)
+    ap.add_argument("--M", type=int, default=32)
+    ap.add_argument("--efc", type=int, default=200)
+    ap.add_argument("--centres", type=int, default=256)
+    ap.add_argument("--nq", type=int, default=10_000)
+    ap.add_argument("--k", type=int, default=10)
+    ap.add_argument("--target", type=float, default=0.95)
+    ap.add_argument("--efs", default="16,24,32,48,64,96,128,192,256")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--build-threads", type=int, default=16)
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+
+    if a.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {a.gpus}); "
+              "the graph search is replicated per GPU with queries split round-robin (see DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import gvs
+    from gvs import hip as H
+
+    t0 = time.time()
+    x = gvs.gen_mixture(a.n, a.dim, a.centres, 1234, 1235, nthreads=a.build_threads)
+    q = gvs.gen_mixture(a.nq, a.dim, a.centres, 1234, 98765, nthreads=a.build_threads)
+    log(f"[bench] data {a.n}x{a.dim} + {a.nq} queries in {time.time() - t0:.1f}s; building HNSW "
+        f"M={a.M} efC={a.efc} on {a.build_threads} threads")
+    t0 = time.time()
+    g = gvs.HostGraph.build_hnsw(x, M=a.M, ef_construction=a.efc, nthreads=a.build_threads)
+    build_s = time.time() - t0
+    log(f"[bench] build {build_s:.1f}s")
+    dev = gvs.DeviceIndex(g)
+    r0 = g.view.r0
+    ru = g.view.ru
+
+    stream = H.Stream()
+    dq = H.DeviceArray.from_host(q)
+    dlab = H.DeviceArray((a.nq, a.k), np.int64)
+    ddist = H.DeviceArray((a.nq, a.k), np.float32)
+    dstat = H.DeviceArray((a.nq, 4), np.int32)
+    gti = H.DeviceArray((a.nq, a.k), np.int32)
+    gtd = H.DeviceArray((a.nq, a.k), np.float64)
+    t0 = time.time()
+    dev.bruteforce_f64_device(dq.address, a.nq, a.dim, a.k, gti.address, gtd.address, stream.handle)
+    stream.synchronize()
+    gt = gti.to_host()
+    log(f"[bench] FP64 brute-force ground truth (MFMA) {time.time() - t0:.2f}s")
+
+    def run(cfg, stats=False, asynchronous=True):
+        dev.search_device(dq.address, a.nq, a.dim, dlab.address, ddist.address, k=a.k,
+                          stats_ptr=dstat.address if stats else None, stream=stream.handle,
+                          asynchronous=asynchronous, **cfg)
+
+    def timed(cfg, reps):
+        e0, e1 = H.Event(), H.Event()
+        stream.synchronize()
+        e0.record(stream)
+        for _ in range(reps):
+            run(cfg)
+        e1.record(stream)
+        e1.synchronize()
+        return e0.elapsed_ms(e1) / reps
+
+    def measure_recall(cfg):
+        run(cfg, asynchronous=False)
+        stream.synchronize()
+        return recall_at(dlab.to_host(), gt, a.k)
+
+    # ef sweep for the plain one-wavefront-per-query search
+    efs = [int(e) for e in a.efs.split(",")]
+    plain_ef, plain_rec = None, None
+    sweep = []
+    for ef in efs:
+        cfg = dict(ef=ef, mc=1, persistent=False)
+        rec = measure_recall(cfg)
+        ms = timed(cfg, 2)
+        sweep.append({"ef": ef, "recall": round(rec, 4), "qps_plain": round(a.nq / (ms / 1e3))})
+        log(f"[bench] plain ef={ef}: recall@10={rec:.4f} {ms:.2f} ms/batch")
+        if rec >= a.target and plain_ef is None:
+            plain_ef, plain_rec = ef, rec
+    if plain_ef is None:
+        plain_ef, plain_rec = efs[-1], rec
+    plain_cfg = dict(ef=plain_ef, mc=1, persistent=False)
+
+    # autotune the batched path: persistent work queue x DST candidates per step x ef
+    best = None
+    for ef in [e for e in efs if e <= plain_ef]:
+        for mc in (1, 2, 4):
+            for persistent in (False, True):
+                cfg = dict(ef=ef, mc=mc, persistent=persistent)
+                rec = measure_recall(cfg)
+                if rec < a.target:
+                    continue
+                ms = timed(cfg, 2)
+                log(f"[bench] tune ef={ef} mc={mc} persistent={persistent}: recall={rec:.4f} {ms:.2f} ms")
+                if best is None or ms < best[0]:
+                    best = (ms, cfg, rec)
+    if best is None:
+        best = (timed(plain_cfg, 2), plain_cfg, plain_rec)
+    tuned_cfg, tuned_rec = best[1], best[2]
+    log(f"[bench] chosen {tuned_cfg} recall={tuned_rec:.4f}")
+
+    # timed region: W warmup + K steps, HIP events on the search stream
+    for _ in range(a.warmup):
+        run(tuned_cfg)
+    stream.synchronize()
+    H.synchronize()
+    e0, e1 = H.Event(), H.Event()
+    w0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(a.steps):
+        run(tuned_cfg)
+    e1.record(stream)
+    e1.synchronize()
+    H.synchronize()
+    wall = time.perf_counter() - w0
+    ev_ms = e0.elapsed_ms(e1)
+    if dev.overflow_count() != 0:
+        log("[bench] warning: visited-table overflow in the timed region")
+    ms_step = max(ev_ms, wall * 1e3) / a.steps
+    qps = a.nq / (ms_step / 1e3)
+    plain_ms = timed(plain_cfg, a.steps)
+    plain_qps = a.nq / (plain_ms / 1e3)
+
+    # algorithmic bytes of the tuned search, from the kernel's exact counters
+    run(tuned_cfg, stats=True, asynchronous=False)
+    stream.synchronize()
+    st = dstat.to_host().astype(np.int64)
+    row_b = 4 * a.dim
+    traffic = int((st[:, 0] * 4 * r0 + st[:, 1] * row_b + st[:, 2] * 4 * ru + st[:, 3] * row_b).sum())
+    kernel_ms = timed(tuned_cfg, 3)
+    achieved = traffic / (kernel_ms / 1e3) / 1e9
+    avg = st.mean(axis=0)
+    log(f"[bench] tuned {qps:.0f} QPS, plain {plain_qps:.0f} QPS; hops {avg[0]:.1f} evals {avg[1]:.1f} "
+        f"upper {avg[2]:.1f}/{avg[3]:.1f}; {achieved:.0f} GB/s algorithmic")
+
+    cpu = None
+    if not a.no_cpu:
+        from oracle import oracle as orc
+        og = orc.OracleGraph(g.arrays())
+        threads = min(16, os.cpu_count() or 1)
+        probe = 200
+        t0 = time.perf_counter()
+        og.search(q[:probe], a.k, plain_ef, order=orc.SEQ, nthreads=threads)
+        per_q = (time.perf_counter() - t0) / probe
+        ns = int(min(a.nq, max(probe, a.cpu_seconds / max(per_q, 1e-7))))
+        t0 = time.perf_counter()
+        ids, _, _ = og.search(q[:ns], a.k, plain_ef, order=orc.SEQ, nthreads=threads)
+        cs = time.perf_counter() - t0
+        cpu = {"value": round(ns / cs, 1), "unit": "QPS", "cores": threads, "kind": "port",
+               "sample": f"first {ns} of the {a.nq} queries, ef={plain_ef}, k={a.k}, hnswlib-order oracle "
+                         f"(oracle/oracle.c searchKnn), OpenMP over queries on {threads} host threads",
+               "recall_at_10": round(recall_at(ids, gt[:ns], a.k), 4)}
+        log(f"[bench] CPU oracle {cpu['value']} QPS on {threads} threads ({ns} queries, {cs:.1f}s)")
+
+    out = {
+        "metric": METRIC, "value": round(qps, 1), "unit": "QPS", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": round(ms_step, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32",
+        "data": f"synthetic: seeded {a.centres}-centre Gaussian mixture (centres N(0,16I), noise N(0,I)); "
+                "queries from the same mixture; FP64 brute-force ground truth",
+        "config": {
+            "workload": f"HNSW search, {a.n}x{a.dim} fp32 L2, M={a.M} (layer-0 degree {r0}), efC={a.efc}, "
+                        f"{a.nq} queries per step, k={a.k}, recall@10>={a.target}",
+            "ef": tuned_cfg["ef"], "mc": tuned_cfg["mc"], "persistent": tuned_cfg["persistent"],
+            "recall_at_10": round(tuned_rec, 4), "plain_ef": plain_ef, "plain_recall_at_10": round(plain_rec, 4),
+            "plain_qps": round(plain_qps, 1), "speedup_vs_plain": round(qps / plain_qps, 3),
+            "avg_hops": round(float(avg[0]), 2), "avg_dist_evals": round(float(avg[1]), 2),
+            "build_seconds": round(build_s, 1), "ef_sweep": sweep,
+        },
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "algorithmic_bytes_per_launch": traffic, "kernel_ms": round(kernel_ms, 4)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -38,6 +38,13 @@ __global__ __launch_bounds__(256) void k_search(KParams P)
     }
 }
 
+// work-queue head and overflow counter reset (a kernel, not a memset, so the
+// launch sequence can be captured in a hipGraph)
+__global__ void k_reset_counters(uint32_t *c)
+{
+    if (threadIdx.x < 2) c[threadIdx.x] = 0u;
+}
+
 typedef void (*kfn_t)(KParams);
 
 template <int STORE, int METRIC>
@@ -134,6 +141,7 @@ static void free_index(gvs_index *ix)
     (void)hipFree(ix->d_vtab);
     (void)hipFree(ix->d_counters);
     (void)hipFree(ix->d_status);
+    (void)hipFree(ix->d_xnorm64);
     delete ix;
 }
 
@@ -378,6 +386,7 @@ int ensure_scratch(gvs_index *ix, int64_t slots, int vlog2, int64_t nstatus)
     }
     if (nstatus > ix->status_cap) {
         (void)hipFree(ix->d_status);
+    (void)hipFree(ix->d_xnorm64);
         ix->d_status = nullptr;
         ix->status_cap = 0;
         if (!hip_ok(hipMalloc(&ix->d_status, sizeof(int32_t) * nstatus), "hipMalloc status")) return GVS_ERR_NOMEM;
@@ -434,7 +443,7 @@ int launch_search(gvs_index *ix, const gvs_params *p, const LaunchPlan &lp, KPar
         P.persistent = 1;
         P.nq = nq;
         P.q_base = 0;
-        if (!hip_ok(hipMemsetAsync(ix->d_counters, 0, 8, stream), "hipMemsetAsync counters")) return GVS_ERR_HIP;
+        hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, stream, ix->d_counters);
         hipLaunchKernelGGL(fn, dim3((unsigned)(gw / lp.wpb)), dim3(lp.wpb * WAVE), shm, stream, P);
         if (!hip_ok(hipGetLastError(), "launch k_search (persistent)")) return GVS_ERR_HIP;
         return GVS_OK;
@@ -448,7 +457,7 @@ int launch_search(gvs_index *ix, const gvs_params *p, const LaunchPlan &lp, KPar
     P.vtab = ix->d_vtab;
     P.status = ix->d_status;
     P.persistent = 0;
-    if (!hip_ok(hipMemsetAsync(ix->d_counters, 0, 8, stream), "hipMemsetAsync counters")) return GVS_ERR_HIP;
+    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, stream, ix->d_counters);
     for (int64_t b = 0; b < nq; b += slots) {
         int64_t cnt = std::min(slots, nq - b);
         P.q_base = b;
@@ -488,6 +497,7 @@ static int search_impl(gvs_index *ix, const float *queries, int64_t nq, int64_t 
     P.out_stats = stats;
     rc = gvs::launch_search(ix, p, lp, P, nq, stream);
     if (rc) return rc;
+    if (p->flags & GVS_FLAG_ASYNC) return GVS_OK;
     // visited-table overflow: rerun the affected queries with 4x larger tables
     uint32_t novf = 0;
     if (!hip_ok(hipMemcpyAsync(&novf, ix->d_counters + 1, 4, hipMemcpyDeviceToHost, stream), "read overflow count") ||
@@ -540,8 +550,7 @@ static int search_impl(gvs_index *ix, const float *queries, int64_t nq, int64_t 
             P2.out_labels = t_lab;
             P2.out_dists = t_d;
             P2.out_stats = t_st;
-            ok = hip_ok(hipMemset(ix->d_counters, 0, 8), "memset") &&
-                 gvs::launch_search(ix, &p2, lp2, P2, m, stream) == GVS_OK &&
+            ok = gvs::launch_search(ix, &p2, lp2, P2, m, stream) == GVS_OK &&
                  hip_ok(hipStreamSynchronize(stream), "sync retry");
             for (int64_t i = 0; ok && i < m; i++) {
                 int64_t q = idx[(size_t)i];
@@ -572,6 +581,20 @@ extern "C" int gvs_search_device(gvs_index *ix, const float *queries, int64_t nq
                                  int32_t *stats, void *stream)
 {
     return search_impl(ix, queries, nq, ldq, entries, p, labels, dists, stats, (hipStream_t)stream);
+}
+
+extern "C" int gvs_search_overflow_count(gvs_index *ix, int64_t *count)
+{
+    if (!ix || !count) {
+        gvs::set_error("gvs_search_overflow_count: invalid argument");
+        return GVS_ERR_ARG;
+    }
+    uint32_t c = 0;
+    if (!hip_ok(hipDeviceSynchronize(), "hipDeviceSynchronize") ||
+        !hip_ok(hipMemcpy(&c, ix->d_counters + 1, 4, hipMemcpyDeviceToHost), "read overflow count"))
+        return GVS_ERR_HIP;
+    *count = c;
+    return GVS_OK;
 }
 
 extern "C" int gvs_search_host(gvs_index *ix, const float *queries, int64_t nq, int64_t ldq,

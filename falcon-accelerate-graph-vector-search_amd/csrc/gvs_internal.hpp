@@ -28,6 +28,7 @@ struct gvs_index {
     uint32_t *d_counters = nullptr;
     int32_t *d_status = nullptr;
     int64_t status_cap = 0;
+    double *d_xnorm64 = nullptr;   // |x|^2 in fp64 for the brute-force GT kernel
 };
 
 namespace gvs {

@@ -17,6 +17,7 @@ _PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(_PKG, "lib", "libgvs.so")
 
 GVS_OK = 0
+FLAG_ASYNC = 1
 ERRORS = {-1: "ARG", -2: "HIP", -3: "NOMEM", -4: "UNSUPPORTED", -5: "OVERFLOW", -6: "IO", -7: "STATE"}
 L2, IP = 0, 1
 F32, BF16, FP8 = 0, 1, 2
@@ -43,7 +44,7 @@ class Graph(C.Structure):
 class Params(C.Structure):
     _fields_ = [(n, C.c_int32) for n in (
         "k", "ef", "mc", "mg", "waves_per_query", "storage", "rerank", "persistent",
-        "visited_log2", "bloom_log2", "grid_waves", "reserved")]
+        "visited_log2", "bloom_log2", "grid_waves", "flags")]
 
 
 class ServerConfig(C.Structure):
@@ -72,6 +73,7 @@ def _load():
         "gvs_index_device_bytes": (C.c_int, [P, P, P]),
         "gvs_search_device": (C.c_int, [P, P, I64, I64, P, P, P, P, P, P]),
         "gvs_search_host": (C.c_int, [P, P, I64, I64, P, P, P, P, P]),
+        "gvs_search_overflow_count": (C.c_int, [P, P]),
         "gvs_bruteforce_f64_device": (C.c_int, [P, P, I64, I64, I32, P, P, P]),
         "gvs_server_open": (C.c_int, [P, P, P]),
         "gvs_server_submit": (I64, [P, P, I32, I32]),
@@ -97,7 +99,7 @@ def exported_symbols():
     names = ["gvs_last_error", "gvs_version", "gvs_hnsw_build", "gvs_nsg_from_hnsw", "gvs_hnswlib_load",
              "gvs_hnswlib_save", "gvs_nsg_load", "gvs_host_graph_view", "gvs_host_graph_free",
              "gvs_gen_mixture", "gvs_index_create", "gvs_index_destroy", "gvs_index_device_bytes",
-             "gvs_search_device", "gvs_search_host", "gvs_bruteforce_f64_device", "gvs_server_open",
+             "gvs_search_device", "gvs_search_host", "gvs_search_overflow_count", "gvs_bruteforce_f64_device", "gvs_server_open",
              "gvs_server_submit", "gvs_server_poll", "gvs_server_cancel", "gvs_server_stats",
              "gvs_server_close"]
     return [n for n in names if hasattr(lib, n)]
@@ -191,9 +193,9 @@ class HostGraph:
 
 
 def make_params(k=10, ef=64, mc=1, mg=1, waves_per_query=1, storage=F32, rerank=False, persistent=False,
-                visited_log2=0, bloom_log2=0, grid_waves=0):
+                visited_log2=0, bloom_log2=0, grid_waves=0, asynchronous=False):
     return Params(k, ef, mc, mg, waves_per_query, storage, int(bool(rerank)), int(bool(persistent)),
-                  visited_log2, bloom_log2, grid_waves, 0)
+                  visited_log2, bloom_log2, grid_waves, FLAG_ASYNC if asynchronous else 0)
 
 
 class DeviceIndex:
@@ -237,6 +239,11 @@ class DeviceIndex:
                                      C.c_void_p(labels_ptr), C.c_void_p(dists_ptr),
                                      C.c_void_p(stats_ptr) if stats_ptr else None,
                                      C.c_void_p(stream) if stream else None))
+
+    def overflow_count(self):
+        c = C.c_int64(0)
+        _check(lib.gvs_search_overflow_count(self._h, C.byref(c)))
+        return c.value
 
     def bruteforce_f64_device(self, q_ptr, nq, ldq, k, ids_ptr, dists_ptr, stream=None):
         _check(lib.gvs_bruteforce_f64_device(self._h, C.c_void_p(q_ptr), nq, ldq, k, C.c_void_p(ids_ptr),

@@ -75,8 +75,14 @@ typedef struct gvs_params {
     int32_t visited_log2;      /* log2 HBM visited-hash slots per query; 0 = auto */
     int32_t bloom_log2;        /* log2 LDS Bloom words per query; 0 = auto */
     int32_t grid_waves;        /* persistent mode: resident waves; 0 = auto */
-    int32_t reserved;
+    int32_t flags;             /* GVS_FLAG_* */
 } gvs_params;
+
+/* gvs_params.flags: GVS_FLAG_ASYNC = enqueue only, no host synchronisation.
+ * Queries whose visited table overflowed then return label -1; the caller
+ * reads the count with gvs_search_overflow_count() after synchronising and
+ * re-issues them (the synchronous form retries them itself). */
+#define GVS_FLAG_ASYNC 1
 
 /* Per-query debug counters written when stats != NULL (4 int32 per query):
  * layer-0 hops, layer-0 distance evaluations, upper-layer list scans,
@@ -145,6 +151,9 @@ GVS_API int gvs_index_device_bytes(const gvs_index *ix, int64_t *graph_bytes, in
 GVS_API int gvs_search_device(gvs_index *ix, const float *queries, int64_t nq, int64_t ldq,
                               const int32_t *entries, const gvs_params *p,
                               int64_t *labels, float *dists, int32_t *stats, void *stream);
+/* Number of queries of the last search on this index whose visited table
+ * overflowed (only non-zero after a GVS_FLAG_ASYNC search; synchronises). */
+GVS_API int gvs_search_overflow_count(gvs_index *ix, int64_t *count);
 /* Same with HOST buffers; synchronous (copies in and out). */
 GVS_API int gvs_search_host(gvs_index *ix, const float *queries, int64_t nq, int64_t ldq,
                             const int32_t *entries, const gvs_params *p,

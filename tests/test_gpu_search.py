@@ -150,3 +150,24 @@ def test_recall_c1_scaled():
     print(f"recall@10 gpu={r_gpu:.4f} cpu(hnswlib order)={r_cpu:.4f}", flush=True)
     assert abs(r_gpu - r_cpu) <= 0.001
     assert r_gpu > 0.9
+
+
+@pytest.mark.parametrize("metric,k", [(gvs.L2, 10), (gvs.L2, 100), (gvs.IP, 10)])
+def test_bruteforce_f64_mfma_matches_oracle(metric, k):
+    from gvs import hip as H
+    norm = metric == gvs.IP
+    x, g, dev = graph(20000, 128, 16, 100, metric=metric, normalize=norm)
+    q = queries(150, 128, seed=13, normalize=norm)
+    qt = H.DeviceArray.from_host(q)
+    ids = H.DeviceArray((150, k), np.int32)
+    ds = H.DeviceArray((150, k), np.float64)
+    print(f"case bruteforce metric={metric} k={k}", flush=True)
+    dev.bruteforce_f64_device(qt.address, 150, 128, k, ids.address, ds.address)
+    H.synchronize()
+    gi, gd = ids.to_host(), ds.to_host()
+    oi, od = orc.bruteforce_f64(x, q, k, metric=metric)
+    np.testing.assert_allclose(gd, od, rtol=1e-9, atol=1e-9)
+    mism = gi != oi
+    for r, c in zip(*np.nonzero(mism)):
+        assert abs(gd[r, c] - od[r, c]) <= 1e-9 * max(1.0, abs(od[r, c]))
+    assert mism.mean() < 0.01
