@@ -61,6 +61,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--build-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--config", default="",
+                    help="fixed search config, e.g. ef=64,mc=2,persistent=1 (skips the sweep and the autotuner)")
+    ap.add_argument("--copy-calib", type=int, default=0,
+                    help="also run the library's streaming copy kernel over this many MiB (counter calibration)")
     a = ap.parse_args()
 
     if a.gpus != 1:
@@ -118,8 +122,22 @@ def main():
         stream.synchronize()
         return recall_at(dlab.to_host(), gt, a.k)
 
+    if a.copy_calib > 0:
+        nb = a.copy_calib << 20
+        src, dst = H.DeviceArray(nb, np.uint8), H.DeviceArray(nb, np.uint8)
+        for _ in range(3):
+            gvs.device_copy(dst.address, src.address, nb, stream.handle)
+        stream.synchronize()
+        log(f"[bench] copy calibration: 3 x {a.copy_calib} MiB through k_copy16")
+        del src, dst
+
     # ef sweep for the plain one-wavefront-per-query search
     efs = [int(e) for e in a.efs.split(",")]
+    if a.config:
+        kv = dict(item.split("=") for item in a.config.split(","))
+        fixed = dict(ef=int(kv.get("ef", 64)), mc=int(kv.get("mc", 1)), mg=int(kv.get("mg", 1)),
+                     persistent=bool(int(kv.get("persistent", 0))))
+        efs = [fixed["ef"]]
     plain_ef, plain_rec = None, None
     sweep = []
     for ef in efs:
@@ -136,17 +154,20 @@ def main():
 
     # autotune the batched path: persistent work queue x DST candidates per step x ef
     best = None
-    for ef in [e for e in efs if e <= plain_ef]:
+    if a.config:
+        best = (0.0, fixed, measure_recall(fixed))
+    for ef in ([] if a.config else [e for e in efs if e <= plain_ef]):
         for mc in (1, 2, 4):
-            for persistent in (False, True):
-                cfg = dict(ef=ef, mc=mc, persistent=persistent)
-                rec = measure_recall(cfg)
-                if rec < a.target:
-                    continue
-                ms = timed(cfg, 2)
-                log(f"[bench] tune ef={ef} mc={mc} persistent={persistent}: recall={rec:.4f} {ms:.2f} ms")
-                if best is None or ms < best[0]:
-                    best = (ms, cfg, rec)
+            for mg in (1, 2):
+                for persistent in (False, True):
+                    cfg = dict(ef=ef, mc=mc, mg=mg, persistent=persistent)
+                    rec = measure_recall(cfg)
+                    if rec < a.target:
+                        continue
+                    ms = timed(cfg, 2)
+                    log(f"[bench] tune ef={ef} mc={mc} mg={mg} persistent={persistent}: recall={rec:.4f} {ms:.2f} ms")
+                    if best is None or ms < best[0]:
+                        best = (ms, cfg, rec)
     if best is None:
         best = (timed(plain_cfg, 2), plain_cfg, plain_rec)
     tuned_cfg, tuned_rec = best[1], best[2]
@@ -195,13 +216,17 @@ def main():
         t0 = time.perf_counter()
         og.search(q[:probe], a.k, plain_ef, order=orc.SEQ, nthreads=threads)
         per_q = (time.perf_counter() - t0) / probe
+        # a bounded sample of about cpu_seconds: the query batch, repeated as needed
+        reps = max(1, int(a.cpu_seconds / max(per_q * a.nq, 1e-6)))
         ns = int(min(a.nq, max(probe, a.cpu_seconds / max(per_q, 1e-7))))
         t0 = time.perf_counter()
-        ids, _, _ = og.search(q[:ns], a.k, plain_ef, order=orc.SEQ, nthreads=threads)
+        for _ in range(reps):
+            ids, _, _ = og.search(q[:ns], a.k, plain_ef, order=orc.SEQ, nthreads=threads)
         cs = time.perf_counter() - t0
-        cpu = {"value": round(ns / cs, 1), "unit": "QPS", "cores": threads, "kind": "port",
-               "sample": f"first {ns} of the {a.nq} queries, ef={plain_ef}, k={a.k}, hnswlib-order oracle "
-                         f"(oracle/oracle.c searchKnn), OpenMP over queries on {threads} host threads",
+        cpu = {"value": round(reps * ns / cs, 1), "unit": "QPS", "cores": threads, "kind": "port",
+               "sample": f"{reps} pass(es) over the first {ns} of the {a.nq} queries ({cs:.1f} s), ef={plain_ef}, "
+                         f"k={a.k}, hnswlib-order oracle (oracle/oracle.c searchKnn), OpenMP over queries on "
+                         f"{threads} host threads",
                "recall_at_10": round(recall_at(ids, gt[:ns], a.k), 4)}
         log(f"[bench] CPU oracle {cpu['value']} QPS on {threads} threads ({ns} queries, {cs:.1f}s)")
 
@@ -214,7 +239,8 @@ def main():
         "config": {
             "workload": f"HNSW search, {a.n}x{a.dim} fp32 L2, M={a.M} (layer-0 degree {r0}), efC={a.efc}, "
                         f"{a.nq} queries per step, k={a.k}, recall@10>={a.target}",
-            "ef": tuned_cfg["ef"], "mc": tuned_cfg["mc"], "persistent": tuned_cfg["persistent"],
+            "ef": tuned_cfg["ef"], "mc": tuned_cfg["mc"], "mg": tuned_cfg.get("mg", 1),
+            "persistent": tuned_cfg["persistent"],
             "recall_at_10": round(tuned_rec, 4), "plain_ef": plain_ef, "plain_recall_at_10": round(plain_rec, 4),
             "plain_qps": round(plain_qps, 1), "speedup_vs_plain": round(qps / plain_qps, 3),
             "avg_hops": round(float(avg[0]), 2), "avg_dist_evals": round(float(avg[1]), 2),

@@ -21,7 +21,8 @@ LIBDIR = os.path.join(PKG, "lib")
 LIB = os.path.join(LIBDIR, "libgvs.so")
 
 HOST_SRCS = ["hnsw_build.cpp", "formats.cpp", "datagen.cpp"]
-HIP_SRCS = ["gvs_device.hip", "gvs_bruteforce.hip", "gvs_server.hip"]
+HIP_SRCS = ["gvs_device.hip", "gvs_bruteforce.hip", "gvs_server.hip",
+            "k_f32_l2.hip", "k_f32_ip.hip", "k_bf16_l2.hip", "k_bf16_ip.hip", "k_fp8_l2.hip", "k_fp8_ip.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
@@ -43,34 +44,41 @@ def _run(cmd):
     return r
 
 
-def build(jobs=8, verbose=False):
-    os.makedirs(BUILD, exist_ok=True)
-    os.makedirs(LIBDIR, exist_ok=True)
+def build(jobs=8, verbose=False, extra_hip_flags=(), out=None, build_dir=None):
+    """Compile everything; extra_hip_flags (e.g. -DGVS_MIN_WAVES_PER_EU=5)
+    and out/build_dir allow side-by-side variants of the library."""
+    bdir = build_dir or BUILD
+    lib = out or LIB
+    os.makedirs(bdir, exist_ok=True)
+    os.makedirs(os.path.dirname(lib), exist_ok=True)
     cmds, objs = [], []
     for s in HOST_SRCS:
-        o = os.path.join(BUILD, s + ".o")
+        o = os.path.join(bdir, s + ".o")
         cmds.append(["g++", *HOST_FLAGS, "-c", os.path.join(CSRC, s), "-o", o])
         objs.append(o)
     for s in HIP_SRCS:
         src = os.path.join(CSRC, s)
         if not os.path.exists(src):
             continue
-        o = os.path.join(BUILD, s + ".o")
-        cmds.append([HIPCC, *HIP_FLAGS, "-c", src, "-o", o])
+        o = os.path.join(bdir, s + ".o")
+        cmds.append([HIPCC, *HIP_FLAGS, *extra_hip_flags, "-c", src, "-o", o])
         objs.append(o)
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         for r in ex.map(_run, cmds):
             if verbose and (r.stdout or r.stderr):
                 print(r.stdout, r.stderr, file=sys.stderr)
-    tmp = LIB + ".tmp"
+    tmp = lib + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-lpthread"])
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, lib)
+    return lib
 
 
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--jobs", type=int, default=8)
     ap.add_argument("-v", action="store_true")
+    ap.add_argument("--hipflag", action="append", default=[])
+    ap.add_argument("--out", default=None)
+    ap.add_argument("--build-dir", default=None)
     a = ap.parse_args()
-    print(build(a.jobs, a.v))
+    print(build(a.jobs, a.v, a.hipflag, a.out, a.build_dir))

@@ -27,9 +27,9 @@ __device__ __forceinline__ bool dlt(double a, int ia, double b, int ib) { return
 
 __device__ __forceinline__ void lds_wave_sync()
 {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 __global__ void k_xnorm64(const float *X, int64_t n, int dp, int dim, double *xn)

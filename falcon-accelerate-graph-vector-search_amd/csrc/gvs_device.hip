@@ -8,36 +8,13 @@
 #include <vector>
 
 #include "gvs_internal.hpp"
-#include "search_query.hpp"
+#include "search_kernels.hpp"
 
 using namespace gvsk;
 
 // ---------------------------------------------------------------------------
 // kernels
 // ---------------------------------------------------------------------------
-template <int NCH, int STORE, int METRIC>
-__global__ __launch_bounds__(256) void k_search(KParams P)
-{
-    extern __shared__ __align__(16) unsigned char smem[];
-    const int wib = threadIdx.x >> 6;
-    unsigned char *lds = smem + (size_t)wib * P.lds_per_wave;
-    const int64_t gw = (int64_t)blockIdx.x * P.waves_per_block + wib;
-    if (P.persistent) {
-        // device work queue: each wave takes the next query index
-        for (;;) {
-            uint32_t q = 0;
-            if (lane_id() == 0) q = atomicAdd(&P.counters[0], 1u);
-            q = __builtin_amdgcn_readfirstlane(q);
-            if ((int64_t)q >= P.nq) break;
-            search_query<NCH, STORE, METRIC>(P, (int64_t)q, gw, lds);
-        }
-    } else {
-        const int64_t qi = P.q_base + gw;
-        if (qi >= P.nq) return;
-        search_query<NCH, STORE, METRIC>(P, qi, gw, lds);
-    }
-}
-
 // work-queue head and overflow counter reset (a kernel, not a memset, so the
 // launch sequence can be captured in a hipGraph)
 __global__ void k_reset_counters(uint32_t *c)
@@ -45,32 +22,19 @@ __global__ void k_reset_counters(uint32_t *c)
     if (threadIdx.x < 2) c[threadIdx.x] = 0u;
 }
 
-typedef void (*kfn_t)(KParams);
-
-template <int STORE, int METRIC>
-static kfn_t pick_nch(int nch)
+// plain streaming copy (16 bytes per lane per iteration)
+__global__ __launch_bounds__(256) void k_copy16(const uint4 *__restrict__ src, uint4 *__restrict__ dst, int64_t n16)
 {
-    switch (nch) {
-    case 1: return k_search<1, STORE, METRIC>;
-    case 2: return k_search<2, STORE, METRIC>;
-    case 3: return k_search<3, STORE, METRIC>;
-    case 4: return k_search<4, STORE, METRIC>;
-    case 6: if constexpr (STORE != STORE_FP8) return k_search<6, STORE, METRIC>; else return nullptr;
-    case 8: if constexpr (STORE != STORE_FP8) return k_search<8, STORE, METRIC>; else return nullptr;
-    case 12: if constexpr (STORE == STORE_F32) return k_search<12, STORE, METRIC>; else return nullptr;
-    case 16: if constexpr (STORE == STORE_F32) return k_search<16, STORE, METRIC>; else return nullptr;
-    default: return nullptr;
-    }
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
 }
 
-static kfn_t pick_kernel(int storage, int metric, int nch)
+static kfn_t pick_kernel(int storage, int metric, int nch, int rows)
 {
-    if (storage == GVS_STORE_F32)
-        return metric == GVS_METRIC_L2 ? pick_nch<STORE_F32, METRIC_L2>(nch) : pick_nch<STORE_F32, METRIC_IP>(nch);
-    if (storage == GVS_STORE_BF16)
-        return metric == GVS_METRIC_L2 ? pick_nch<STORE_BF16, METRIC_L2>(nch) : pick_nch<STORE_BF16, METRIC_IP>(nch);
-    if (storage == GVS_STORE_FP8)
-        return metric == GVS_METRIC_L2 ? pick_nch<STORE_FP8, METRIC_L2>(nch) : pick_nch<STORE_FP8, METRIC_IP>(nch);
+    const bool l2 = metric == GVS_METRIC_L2;
+    if (storage == GVS_STORE_F32) return l2 ? gvs_kernel_f32_l2(nch, rows) : gvs_kernel_f32_ip(nch, rows);
+    if (storage == GVS_STORE_BF16) return l2 ? gvs_kernel_bf16_l2(nch, rows) : gvs_kernel_bf16_ip(nch, rows);
+    if (storage == GVS_STORE_FP8) return l2 ? gvs_kernel_fp8_l2(nch, rows) : gvs_kernel_fp8_ip(nch, rows);
     return nullptr;
 }
 
@@ -323,7 +287,7 @@ int plan_search(const gvs_index *ix, const gvs_params *p, int max_ef, int max_k,
         return GVS_ERR_ARG;
     }
     if (max_k < 1 || max_k > 4096 || max_ef < 1 || max_ef > 4096 || p->mc < 1 || p->mc > 8 || p->mg < 1 ||
-        p->mg > 1 || p->waves_per_query != 1) {
+        p->mg > 2 || p->waves_per_query != 1) {
         set_error("gvs_search: unsupported k=%d ef=%d mc=%d mg=%d waves_per_query=%d", max_k, max_ef, p->mc,
                   p->mg, p->waves_per_query);
         return GVS_ERR_UNSUPPORTED;
@@ -339,7 +303,7 @@ int plan_search(const gvs_index *ix, const gvs_params *p, int max_ef, int max_k,
     }
     const int E = p->storage == GVS_STORE_F32 ? 4 : p->storage == GVS_STORE_BF16 ? 8 : 16;
     const int nch = ix->dim_pad / (8 * E);
-    kfn_t fn = (ix->dim_pad % (8 * E) == 0) ? pick_kernel(p->storage, ix->metric, nch) : nullptr;
+    kfn_t fn = (ix->dim_pad % (8 * E) == 0) ? pick_kernel(p->storage, ix->metric, nch, p->mc * rpc) : nullptr;
     if (!fn) {
         set_error("gvs_search: dim %d (padded %d) not compiled for storage %d", ix->dim, ix->dim_pad, p->storage);
         return GVS_ERR_UNSUPPORTED;
@@ -348,17 +312,37 @@ int plan_search(const gvs_index *ix, const gvs_params *p, int max_ef, int max_k,
     lp->fn = (void *)fn;
     lp->ef_cap = (efe + 63) / 64 * 64;
     lp->lcap = p->mc * rpc * WAVE;
-    lp->blog2 = p->bloom_log2 > 0 ? p->bloom_log2 : std::min(12, std::max(8, ilog2_ceil((int64_t)efe * 16)));
+    lp->blog2 = p->bloom_log2 > 0 ? p->bloom_log2 : std::min(12, std::max(8, ilog2_ceil((int64_t)efe * 8)));
     int vmin = ilog2_ceil(4 * (int64_t)lp->lcap + 1024);
     lp->vlog2 = p->visited_log2 > 0 ? std::max(p->visited_log2, vmin)
                                     : std::max(vmin, std::min(20, ilog2_ceil((int64_t)efe * 128)));
-    if (lp->blog2 > 14 || lp->vlog2 > 26) {
-        set_error("gvs_search: bloom_log2 <= 14 and visited_log2 <= 26 required");
+    // exact visited set in LDS for small ef (no HBM table), unless disabled
+    lp->hlog2 = 0;
+    if (!(p->flags & GVS_FLAG_HBM_VISITED) && ((p->flags & GVS_FLAG_LDS_VISITED) || efe <= 96)) {
+        int want = ilog2_ceil(((int64_t)efe * 40 + lp->lcap) * 4 / 3);
+        lp->hlog2 = (p->flags & GVS_FLAG_LDS_VISITED) && p->visited_log2 >= 10 && p->visited_log2 <= 14
+                        ? p->visited_log2 : std::max(11, std::min(13, want));
+        lp->blog2 = 0;
+    }
+    if (lp->blog2 > 13 || lp->vlog2 > 26) {
+        set_error("gvs_search: bloom_log2 <= 13 and visited_log2 <= 26 required");
         return GVS_ERR_ARG;
     }
-    int bytes = 16 * lp->ef_cap + 8 * lp->lcap + 4 * (1 << lp->blog2) + 4 * lp->lcap + 256;
+    int bytes = 16 * lp->ef_cap + 8 * lp->lcap + 8 * (1 << lp->blog2) + 4 * lp->lcap + 64 +
+                (lp->hlog2 > 0 ? 4 * (1 << lp->hlog2) : 0);
     lp->lds_per_wave = (bytes + 15) / 16 * 16;
-    lp->wpb = std::max(1, std::min(4, 65536 / lp->lds_per_wave));
+    // waves per block: the most resident waves per CU (160 KiB LDS, at most
+    // 16 workgroups per CU, at most 64 KiB per block)
+    lp->wpb = 1;
+    int best = 0;
+    for (int w = 1; w <= 4; w *= 2) {
+        if (w * lp->lds_per_wave > 65536) break;
+        int per_cu = std::min(16, (160 * 1024) / (w * lp->lds_per_wave)) * w;
+        if (per_cu >= best) {
+            best = per_cu;
+            lp->wpb = w;
+        }
+    }
     if (lp->lds_per_wave > 65536) {
         set_error("gvs_search: per-query LDS %d bytes exceeds 64 KiB (ef too large)", lp->lds_per_wave);
         return GVS_ERR_UNSUPPORTED;
@@ -414,10 +398,12 @@ void fill_params(const gvs_index *ix, const gvs_params *p, const LaunchPlan &lp,
     P.entry = ix->entry;
     P.labels = ix->d_labels;
     P.mc = p->mc;
+    P.mg = p->mg;
     P.rerank = p->rerank;
     P.vtab = ix->d_vtab;
     P.vlog2 = lp.vlog2;
     P.blog2 = lp.blog2;
+    P.hlog2 = lp.hlog2;
     P.bloom_seed = 0x2545F491u;
     P.ef_cap = lp.ef_cap;
     P.lcap = lp.lcap;
@@ -436,7 +422,7 @@ int launch_search(gvs_index *ix, const gvs_params *p, const LaunchPlan &lp, KPar
         int64_t gw = p->grid_waves > 0 ? p->grid_waves : lp.resident_waves;
         gw = std::max<int64_t>(lp.wpb, std::min<int64_t>(gw, (nq + lp.wpb - 1) / lp.wpb * lp.wpb));
         gw = (gw + lp.wpb - 1) / lp.wpb * lp.wpb;
-        int rc = ensure_scratch(ix, gw, lp.vlog2, nq);
+        int rc = ensure_scratch(ix, lp.hlog2 > 0 ? 1 : gw, lp.vlog2, nq);
         if (rc) return rc;
         P.vtab = ix->d_vtab;
         P.status = ix->d_status;
@@ -452,7 +438,7 @@ int launch_search(gvs_index *ix, const gvs_params *p, const LaunchPlan &lp, KPar
     int64_t slots = std::min<int64_t>(nq, std::max<int64_t>(lp.wpb, ((int64_t)1 << 30) >> lp.vlog2));
     slots = std::min<int64_t>(slots, 65536);
     slots = (slots + lp.wpb - 1) / lp.wpb * lp.wpb;
-    int rc = ensure_scratch(ix, slots, lp.vlog2, nq);
+    int rc = ensure_scratch(ix, lp.hlog2 > 0 ? 1 : slots, lp.vlog2, nq);
     if (rc) return rc;
     P.vtab = ix->d_vtab;
     P.status = ix->d_status;
@@ -513,6 +499,7 @@ static int search_impl(gvs_index *ix, const float *queries, int64_t nq, int64_t 
         gvs_params p2 = *p;
         p2.visited_log2 = lp.vlog2 + 2 * (attempt + 1);
         p2.persistent = 0;
+        p2.flags = (p->flags & ~GVS_FLAG_LDS_VISITED) | GVS_FLAG_HBM_VISITED;
         gvs::LaunchPlan lp2;
         rc = gvs::plan_search(ix, &p2, p2.ef, p2.k, &lp2);
         if (rc) return rc;
@@ -581,6 +568,19 @@ extern "C" int gvs_search_device(gvs_index *ix, const float *queries, int64_t nq
                                  int32_t *stats, void *stream)
 {
     return search_impl(ix, queries, nq, ldq, entries, p, labels, dists, stats, (hipStream_t)stream);
+}
+
+extern "C" int gvs_device_copy(void *dst, const void *src, int64_t bytes, void *stream)
+{
+    if (!dst || !src || bytes < 0 || (bytes & 15) || (((uintptr_t)dst | (uintptr_t)src) & 15)) {
+        gvs::set_error("gvs_device_copy: 16-byte aligned pointers and sizes required");
+        return GVS_ERR_ARG;
+    }
+    if (bytes == 0) return GVS_OK;
+    int64_t n16 = bytes / 16;
+    unsigned blocks = (unsigned)std::min<int64_t>((n16 + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_copy16, dim3(blocks), dim3(256), 0, (hipStream_t)stream, (const uint4 *)src, (uint4 *)dst, n16);
+    return hip_ok(hipGetLastError(), "launch k_copy16") ? GVS_OK : GVS_ERR_HIP;
 }
 
 extern "C" int gvs_search_overflow_count(gvs_index *ix, int64_t *count)

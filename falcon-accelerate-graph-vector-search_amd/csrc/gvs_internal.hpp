@@ -36,7 +36,7 @@ bool hip_ok(hipError_t e, const char *what);
 
 struct LaunchPlan {
     void *fn = nullptr;
-    int ef_cap = 0, lcap = 0, blog2 = 0, vlog2 = 0, lds_per_wave = 0, wpb = 1;
+    int ef_cap = 0, lcap = 0, blog2 = 0, vlog2 = 0, hlog2 = 0, lds_per_wave = 0, wpb = 1;
     int64_t resident_waves = 0;
 };
 }  // namespace gvs

@@ -31,7 +31,7 @@ namespace gvsk {
 
 constexpr int WAVE = 64;
 constexpr uint64_t EMPTY_KEY = ~0ull;
-constexpr int MAX_ROWS = 8;        // mc * ceil(r0/64) adjacency rows per step
+constexpr int MAX_ROWS = 8;        // largest mc * ceil(r0/64) adjacency rows per step
 constexpr int STORE_F32 = 0, STORE_BF16 = 1, STORE_FP8 = 2;
 constexpr int METRIC_L2 = 0, METRIC_IP = 1;
 
@@ -55,11 +55,12 @@ struct KParams {
     int64_t nq;
     int64_t q_base;            // first query of this launch (non-persistent chunks)
     const int32_t *entries;
-    int k, ef, mc;
+    int k, ef, mc, mg;
     int rerank;
     uint32_t *vtab;
     int vlog2;
     int blog2;
+    int hlog2;                 // > 0: exact visited hash in LDS (2^hlog2 slots), no HBM table
     uint32_t bloom_seed;
     int ef_cap;                // ef rounded up to 64
     int lcap;                  // list capacity (mc * rows * 64)
@@ -94,11 +95,15 @@ __device__ __forceinline__ uint64_t mk_key(float d, uint32_t id)
 __device__ __forceinline__ uint32_t key_id(uint64_t k) { return (uint32_t)(k >> 1) & 0x7FFFFFFFu; }
 __device__ __forceinline__ float key_dist(uint64_t k) { return unord_f((uint32_t)(k >> 32)); }
 
+// Orders this wave's LDS accesses across lanes.  A wave's LDS instructions
+// execute in order, so only the compiler must be kept from moving them: a
+// wavefront-scope fence does that without waiting for global memory (a
+// workgroup-scope fence would wait for every outstanding load and atomic).
 __device__ __forceinline__ void wave_sync()
 {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
@@ -226,18 +231,17 @@ __device__ __forceinline__ float row_partial(const KParams &P, const float *qf, 
     return acc;
 }
 
-// Transposed 8-lane reduction: in acc[t] = partial of pass t; out = full sum
-// of pass p (this lane's position in its 8-lane group).
-__device__ __forceinline__ float reduce8(const float acc[8], int p)
+// 8-lane reduction of 4 passes: in acc[t] = this lane's partial of pass t.
+// xor 4 adds the partner's partials (no halving), xor 2 and xor 1 halve, so
+// lane p ends with the full sum of pass (p & 3) (lanes p and p^4 agree).
+// Per row the order is ((s0+s4)+(s2+s6)) + ((s1+s5)+(s3+s7)) up to operand
+// swaps, which IEEE addition does not see.
+__device__ __forceinline__ float reduce8x4(const float acc[4], int p)
 {
-    const bool b2 = p & 4, b1 = p & 2, b0 = p & 1;
+    const bool b1 = p & 2, b0 = p & 1;
     float v4[4], v2[2];
 #pragma unroll
-    for (int t = 0; t < 4; t++) {
-        float send = b2 ? acc[t] : acc[t + 4];
-        float keep = b2 ? acc[t + 4] : acc[t];
-        v4[t] = keep + __shfl_xor(send, 4);
-    }
+    for (int t = 0; t < 4; t++) v4[t] = acc[t] + __shfl_xor(acc[t], 4);
 #pragma unroll
     for (int t = 0; t < 2; t++) {
         float send = b1 ? v4[t] : v4[t + 2];
@@ -249,30 +253,68 @@ __device__ __forceinline__ float reduce8(const float acc[8], int p)
     return keep + __shfl_xor(send, 1);
 }
 
-// Distances of rows lids[0..n) -> keys[0..n).  All 64 lanes take part.
-template <int NCH, int STORE, int METRIC>
-__device__ void eval_rows(const KParams &P, const float *qf, const uint32_t *lids, int n, uint64_t *keys)
+// One block of NP passes (8*NP rows from b0).  Every lane loads its chunks
+// unconditionally - rows past n repeat row n-1 - so all NP*NCH loads are in
+// flight before the first use; rows past n are dropped afterwards.
+template <int NCH, int STORE, int METRIC, int NP>
+__device__ __forceinline__ void eval_block(const KParams &P, const float *qf, const uint32_t *lids, int b0, int n,
+                                           uint64_t *keys)
 {
+    constexpr int E = StoreTraits<STORE>::E;
+    constexpr int B = StoreTraits<STORE>::BYTES;
     const int lane = lane_id();
     const int p = lane & 7, g = lane >> 3;
-    for (int b0 = 0; b0 < n; b0 += WAVE) {
-        const int nb = min(WAVE, n - b0);
-        const int npass = (nb + 7) >> 3;
-        float acc[8];
+    uint4 raw[NP][NCH];
+    float scale[NP];
 #pragma unroll
-        for (int t = 0; t < 8; t++) {
-            acc[t] = 0.0f;
-            if (t < npass) {
-                int r = b0 + t * 8 + g;
-                if (r < n) acc[t] = row_partial<NCH, STORE, METRIC>(P, qf, lids[r], p);
+    for (int t = 0; t < NP; t++) {
+        const int r = min(b0 + t * 8 + g, n - 1);
+        const uint32_t id = lids[r];
+        const unsigned char *row = (const unsigned char *)P.vec + (size_t)id * (size_t)P.row_elems * B;
+#pragma unroll
+        for (int i = 0; i < NCH; i++) raw[t][i] = *(const uint4 *)(row + (size_t)(p + 8 * i) * 16);
+        scale[t] = 1.0f;
+        if constexpr (STORE == STORE_FP8) scale[t] = P.vscale[id];
+    }
+    float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int t = 0; t < NP; t++) {
+        float a = 0.0f;
+#pragma unroll
+        for (int i = 0; i < NCH; i++) {
+            float x[E];
+            decode_chunk<STORE>(raw[t][i], scale[t], x);
+#pragma unroll
+            for (int e = 0; e < E; e++) {
+                if constexpr (METRIC == METRIC_L2) {
+                    const float d = x[e] - qf[i * E + e];
+                    a = __builtin_fmaf(d, d, a);
+                } else {
+                    a = __builtin_fmaf(x[e], qf[i * E + e], a);
+                }
             }
         }
-        float v = reduce8(acc, p);
-        int r = b0 + p * 8 + g;
-        if (r < n) {
-            float d = (METRIC == METRIC_L2) ? v : 1.0f - v;
-            keys[r] = mk_key(d, lids[r]);
-        }
+        acc[t] = (b0 + t * 8 + g < n) ? a : 0.0f;
+    }
+    const float v = reduce8x4(acc, p);
+    const int r = b0 + (p & 3) * 8 + g;
+    if (p < 4 && (p & 3) < NP && r < n) {
+        const float d = (METRIC == METRIC_L2) ? v : 1.0f - v;
+        keys[r] = mk_key(d, lids[r]);
+    }
+}
+
+// Distances of rows lids[0..n) -> keys[0..n).  All 64 lanes take part;
+// rows go in blocks of up to 32 (passes of 8 rows, 8 lanes per row).
+template <int NCH, int STORE, int METRIC>
+__device__ __forceinline__ void eval_rows(const KParams &P, const float *qf, const uint32_t *lids, int n,
+                                          uint64_t *keys)
+{
+    for (int b0 = 0; b0 < n; b0 += 32) {
+        const int nb = n - b0;
+        if (nb > 16) eval_block<NCH, STORE, METRIC, 4>(P, qf, lids, b0, n, keys);
+        else if (nb > 8) eval_block<NCH, STORE, METRIC, 2>(P, qf, lids, b0, n, keys);
+        else eval_block<NCH, STORE, METRIC, 1>(P, qf, lids, b0, n, keys);
     }
 }
 

@@ -1,14 +1,25 @@
 // One query on one wavefront: upper-layer greedy descent, then layer-0
-// best-first search with DST groups of up to mc candidates per step.
+// best-first search with DST groups (mc candidates per group, mg groups in
+// flight).
 #pragma once
 #include "search_kernel.hpp"
 
 namespace gvsk {
 
+// Optional phase timing (built only with -DGVS_PROFILE): per query, the
+// stats row holds 16 int32: the 4 counters, then phase cycles / 64.
+#ifdef GVS_PROFILE
+#define PROF_DECL uint64_t prof_acc[8] = {0, 0, 0, 0, 0, 0, 0, 0}; uint64_t prof_t = clock64();
+#define PROF_MARK(ph) do { uint64_t _n = clock64(); prof_acc[ph] += _n - prof_t; prof_t = _n; } while (0)
+#else
+#define PROF_DECL
+#define PROF_MARK(ph) do { } while (0)
+#endif
+
 // LDS carve-up of one wave's partition (sizes in P).
 struct WaveLds {
-    uint64_t *poolA, *poolB, *lkeys;
-    uint32_t *bloom, *lids, *sel;
+    uint64_t *poolA, *poolB, *lkeys, *bloom;
+    uint32_t *lids, *selA, *selB, *vh;
 };
 
 __device__ __forceinline__ WaveLds carve(const KParams &P, unsigned char *base)
@@ -17,64 +28,263 @@ __device__ __forceinline__ WaveLds carve(const KParams &P, unsigned char *base)
     L.poolA = (uint64_t *)base;
     L.poolB = L.poolA + P.ef_cap;
     L.lkeys = L.poolB + P.ef_cap;
-    L.bloom = (uint32_t *)(L.lkeys + P.lcap);
-    L.lids = L.bloom + (1u << P.blog2);
-    L.sel = L.lids + P.lcap;
+    L.bloom = L.lkeys + P.lcap;
+    L.lids = (uint32_t *)(L.bloom + (1u << P.blog2));
+    L.selA = L.lids + P.lcap;
+    L.selB = L.selA + 8;
+    L.vh = L.selB + 8;
     return L;
 }
 
-__device__ __forceinline__ uint32_t bloom_mask(uint32_t h1)
+// Blocked Bloom filter: one 64-bit word per id, six bit positions.
+// word = h0 mod nwords; bits from h1 (five 6-bit fields) and the top of h0.
+__device__ __forceinline__ void bloom_pos(uint32_t id, uint32_t seed, uint32_t nwords, uint32_t &w, uint64_t &m)
 {
-    return (1u << (h1 & 31u)) | (1u << ((h1 >> 5) & 31u)) | (1u << ((h1 >> 10) & 31u));
+    uint32_t h0 = murmur2_k4(id, seed), h1 = murmur2_k4(id, seed + 1u);
+    w = h0 & (nwords - 1u);
+    m = (1ull << (h1 & 63u)) | (1ull << ((h1 >> 6) & 63u)) | (1ull << ((h1 >> 12) & 63u)) |
+        (1ull << ((h1 >> 18) & 63u)) | (1ull << ((h1 >> 24) & 63u)) | (1ull << (h0 >> 26));
+}
+
+__device__ __forceinline__ uint32_t vhash(uint32_t id, int vlog2) { return (id * 0x9E3779B1u) >> (32 - vlog2); }
+
+// Pick up to mc unexpanded pool entries, closest first, from index `hint`
+// on (entries before hint are expanded); marks them expanded.
+__device__ __forceinline__ int select_group(uint64_t *pool, int np, int mc, int &hint, uint32_t *sel)
+{
+    const int lane = lane_id();
+    int nsel = 0;
+    for (int base = hint & ~(WAVE - 1); base < np && nsel < mc; base += WAVE) {
+        const int i = base + lane;
+        const uint64_t key = i < np ? pool[i] : EMPTY_KEY;
+        const bool unexp = i < np && i >= hint && !(key & 1ull);
+        uint64_t bal = __ballot(unexp);
+        while (bal && nsel < mc) {
+            const int l = __ffsll((unsigned long long)bal) - 1;
+            bal &= bal - 1;
+            if (lane == l) {
+                pool[i] = key | 1ull;
+                sel[nsel] = key_id(key);
+            }
+            nsel++;
+        }
+        hint = bal ? base + __ffsll((unsigned long long)bal) - 1 : min(np, base + WAVE);
+    }
+    wave_sync();
+    return nsel;
 }
 
 // Merge keys[0..n) (EMPTY entries allowed) into the sorted pool, keeping the
-// ef smallest; returns the new pool size.  pool/pool2 are swapped.
-__device__ __forceinline__ int merge_into_pool(uint64_t *&pool, uint64_t *&pool2, int np, int ef,
-                                               uint64_t *keys, int n)
+// ef smallest; returns the new size; pool/pool2 are swapped; `hint` becomes
+// min(hint, first position a new entry landed at).
+__device__ __forceinline__ int merge_into_pool(uint64_t *&pool, uint64_t *&pool2, int np, int ef, uint64_t *keys,
+                                               int n, int &hint)
 {
     const int lane = lane_id();
     const uint64_t thr = (np >= ef) ? pool[ef - 1] : EMPTY_KEY;
     int ns = 0;
     for (int b = 0; b < n; b += WAVE) {
-        int i = b + lane;
-        uint64_t key = i < n ? keys[i] : EMPTY_KEY;
-        bool s = key < thr;
-        uint64_t bal = __ballot(s);
+        const int i = b + lane;
+        const uint64_t key = i < n ? keys[i] : EMPTY_KEY;
+        const bool s = key < thr;
+        const uint64_t bal = __ballot(s);
         wave_sync();
         if (s) keys[ns + mbcnt(bal)] = key;
         ns += __popcll(bal);
     }
     wave_sync();
     if (ns == 0) return np;
-    for (int i = lane; i < ns; i += WAVE) {
-        uint64_t s = keys[i];
-        int r = 0;
-        for (int j = 0; j < ns; j++) r += (keys[j] < s) ? 1 : 0;
-        int lo = 0, hi = np;
-        while (lo < hi) {
-            int mid = (lo + hi) >> 1;
-            if (pool[mid] < s) lo = mid + 1;
-            else hi = mid;
+    // Merge survivors in batches of up to 64 (merging batch by batch gives the
+    // same ef best).  Survivor s_k is broadcast by readlane; one compare per
+    // lane gives both the pool entries' shift counts and, through the ballot
+    // mask, s_k's rank in the pool.
+    int mind = ef;
+    for (int b = 0; b < ns; b += WAVE) {
+        const int cnt = min(WAVE, ns - b);
+        const uint64_t sv = lane < cnt ? keys[b + lane] : EMPTY_KEY;
+        int lo = 0, rs = 0;
+        for (int t = 0; t * WAVE < np; t++) {
+            const int i = t * WAVE + lane;
+            const uint64_t pv = i < np ? pool[i] : EMPTY_KEY;
+            int pc = 0;
+            for (int kk = 0; kk < cnt; kk++) {
+                const uint64_t s = readlane_u64(sv, kk);
+                const int c = __popcll(__ballot(pv < s));
+                lo += (lane == kk) ? c : 0;
+                pc += (s < pv) ? 1 : 0;
+                if (t == 0) rs += (s < sv) ? 1 : 0;
+            }
+            const int dest = i + pc;
+            if (i < np && dest < ef) pool2[dest] = pv;
         }
-        int dest = r + lo;
-        if (dest < ef) pool2[dest] = s;
+        const int dest = lo + rs;
+        if (lane < cnt && dest < ef) {
+            pool2[dest] = sv;
+            mind = min(mind, dest);
+        }
+        wave_sync();
+        uint64_t *t = pool;
+        pool = pool2;
+        pool2 = t;
+        np = min(ef, np + cnt);
     }
-    for (int i = lane; i < np; i += WAVE) {
-        uint64_t v = pool[i];
-        int c = 0;
-        for (int j = 0; j < ns; j++) c += (keys[j] < v) ? 1 : 0;
-        int dest = i + c;
-        if (dest < ef) pool2[dest] = v;
-    }
-    wave_sync();
-    uint64_t *t = pool;
-    pool = pool2;
-    pool2 = t;
-    return min(ef, np + ns);
+    for (int m = 32; m >= 1; m >>= 1) mind = min(mind, __shfl_xor(mind, m));
+    hint = min(hint, mind);
+    return np;
 }
 
-template <int NCH, int STORE, int METRIC>
+template <int MROWS>
+__device__ __forceinline__ void load_adj(const KParams &P, const uint32_t *sel, int nsel, int rpc, int32_t (&ids)[MROWS])
+{
+    const int lane = lane_id();
+    const int nrows = nsel * rpc;
+#pragma unroll
+    for (int r = 0; r < MROWS; r++) {
+        ids[r] = -1;
+        if (r < nrows) {
+            const int ci = r / rpc, part = r - ci * rpc;
+            const int j = part * WAVE + lane;
+            if (j < P.r0) ids[r] = P.adj0[(size_t)sel[ci] * P.r0 + j];
+        }
+    }
+}
+
+// Visit the neighbour ids of one group, evaluate the new ones; results in
+// L.lkeys[0..return).  Exactly one lane evaluates each id (the CAS winner).
+#ifdef GVS_PROFILE
+#define EXP_PROF , uint64_t (&prof_acc)[8], uint64_t &prof_t
+#define EXP_PROF_ARGS , prof_acc, prof_t
+#else
+#define EXP_PROF
+#define EXP_PROF_ARGS
+#endif
+
+template <int NCH, int STORE, int METRIC, int MROWS>
+__device__ __forceinline__ int expand_group(const KParams &P, const WaveLds &L, const float *qf, uint32_t *T,
+                                            const int32_t (&ids)[MROWS], int nrows, int &evals EXP_PROF)
+{
+    const int lane = lane_id();
+    const uint32_t nbw = 1u << P.blog2;
+    const uint32_t tmask = (1u << P.vlog2) - 1u;
+    uint32_t prev[MROWS], pos[MROWS];
+    int spos[MROWS];
+    uint32_t spec_bits = 0u;
+    int nl = 0;
+#pragma unroll
+    for (int r = 0; r < MROWS; r++) {
+        prev[r] = 0u;
+        pos[r] = 0u;
+        spos[r] = 0;
+        if (r < nrows) {
+            const int32_t id = ids[r];
+            const bool valid = id >= 0;
+            bool maybe = true;
+            if (valid) {
+                uint32_t w;
+                uint64_t m;
+                bloom_pos((uint32_t)id, P.bloom_seed, nbw, w, m);
+                const uint64_t old = atomicOr((unsigned long long *)&L.bloom[w], (unsigned long long)m);
+                maybe = (old & m) == m;
+                pos[r] = vhash((uint32_t)id, P.vlog2);
+                prev[r] = atomicCAS(&T[pos[r]], 0u, (uint32_t)id + 1u);
+            }
+            const bool spec = valid && !maybe;
+            const uint64_t bal = __ballot(spec);
+            if (spec) {
+                spos[r] = nl + mbcnt(bal);
+                L.lids[spos[r]] = (uint32_t)id;
+            }
+            nl += __popcll(bal);
+            spec_bits |= (spec ? 1u : 0u) << r;
+        }
+    }
+    wave_sync();
+    PROF_MARK(2);
+    // ids the Bloom filter has never seen: gather while the CAS is in flight
+    eval_rows<NCH, STORE, METRIC>(P, qf, L.lids, nl, L.lkeys);
+    PROF_MARK(3);
+    int nl2 = nl;
+#pragma unroll
+    for (int r = 0; r < MROWS; r++) {
+        if (r < nrows) {
+            const int32_t id = ids[r];
+            const bool valid = id >= 0;
+            const uint32_t want = (uint32_t)id + 1u;
+            bool pending = valid && prev[r] != 0u && prev[r] != want;
+            while (__ballot(pending)) {
+                if (pending) {
+                    pos[r] = (pos[r] + 1u) & tmask;
+                    prev[r] = atomicCAS(&T[pos[r]], 0u, want);
+                    pending = prev[r] != 0u && prev[r] != want;
+                }
+            }
+            const bool won = valid && prev[r] == 0u;
+            const bool spec = (spec_bits >> r) & 1u;
+            if (spec && !won) L.lkeys[spos[r]] = EMPTY_KEY;  // same id twice in one group
+            const bool late = won && !spec;                   // Bloom false positive
+            const uint64_t bal = __ballot(late);
+            if (late) L.lids[nl2 + mbcnt(bal)] = (uint32_t)id;
+            nl2 += __popcll(bal);
+            evals += __popcll(__ballot(won));
+        }
+    }
+    wave_sync();
+    PROF_MARK(4);
+    if (nl2 > nl) {
+        eval_rows<NCH, STORE, METRIC>(P, qf, L.lids + nl, nl2 - nl, L.lkeys + nl);
+        wave_sync();
+    }
+    PROF_MARK(5);
+    return nl2;
+}
+
+// ---- exact visited set in LDS (small ef) -------------------------------
+// Open addressing over 2^hlog2 slots of this wave's LDS (key = id + 1), so a
+// lookup-and-insert is one LDS compare-and-swap and collisions cost LDS
+// latency only.  LDS CAS serialises lanes, so an id met twice in one group
+// is evaluated once.  Used while the visited count stays below 3/4 of the
+// slots; a query that outgrows it is rerun with the HBM table.
+template <int NCH, int STORE, int METRIC, int MROWS>
+__device__ __forceinline__ int expand_lds(const KParams &P, const WaveLds &L, const float *qf,
+                                          const int32_t (&ids)[MROWS], int nrows, int &evals EXP_PROF)
+{
+    const uint32_t hmask = (1u << P.hlog2) - 1u;
+    int nl = 0;
+#pragma unroll
+    for (int r = 0; r < MROWS; r++) {
+        if (r < nrows) {
+            const int32_t id = ids[r];
+            const uint32_t want = (uint32_t)id + 1u;
+            uint32_t pos = id >= 0 ? vhash((uint32_t)id, P.hlog2) : 0u;
+            bool pending = id >= 0, won = false;
+            while (__ballot(pending)) {
+                if (pending) {
+                    const uint32_t prev = atomicCAS(&L.vh[pos], 0u, want);
+                    if (prev == 0u) {
+                        won = true;
+                        pending = false;
+                    } else if (prev == want) {
+                        pending = false;
+                    } else {
+                        pos = (pos + 1u) & hmask;
+                    }
+                }
+            }
+            const uint64_t bal = __ballot(won);
+            if (won) L.lids[nl + mbcnt(bal)] = (uint32_t)id;
+            nl += __popcll(bal);
+        }
+    }
+    evals += nl;
+    wave_sync();
+    PROF_MARK(2);
+    eval_rows<NCH, STORE, METRIC>(P, qf, L.lids, nl, L.lkeys);
+    wave_sync();
+    PROF_MARK(3);
+    return nl;
+}
+
+template <int NCH, int STORE, int METRIC, int MROWS>
 __device__ void search_query(const KParams &P, int64_t qi, int64_t slot, unsigned char *lds)
 {
     constexpr int E = StoreTraits<STORE>::E;
@@ -83,6 +293,7 @@ __device__ void search_query(const KParams &P, int64_t qi, int64_t slot, unsigne
     WaveLds L = carve(P, lds);
     const uint32_t nbw = 1u << P.blog2;
 
+    PROF_DECL
     const int64_t qrow = P.q_index ? P.q_index[qi] : qi;
     const int k = P.q_k ? P.q_k[qi] : P.k;
     int ef = P.q_ef ? P.q_ef[qi] : P.ef;
@@ -94,11 +305,11 @@ __device__ void search_query(const KParams &P, int64_t qi, int64_t slot, unsigne
     for (int i = 0; i < NCH; i++)
 #pragma unroll
         for (int t = 0; t < E; t++) {
-            int idx = (p + 8 * i) * E + t;
+            const int idx = (p + 8 * i) * E + t;
             qf[i * E + t] = idx < P.dim ? Q[idx] : 0.0f;
         }
 
-    for (uint32_t w = lane; w < nbw; w += WAVE) L.bloom[w] = 0u;
+    for (uint32_t w = lane; w < nbw; w += WAVE) L.bloom[w] = 0ull;
 
     int up_hops = 0, up_evals = 0;
     uint32_t ep;
@@ -118,9 +329,9 @@ __device__ void search_query(const KParams &P, int64_t qi, int64_t slot, unsigne
         for (int level = P.max_level; level > 0; level--) {
             for (;;) {
                 const int64_t row = P.upper_off[cur] + (level - 1);
-                int32_t nid = lane < P.ru ? P.adj_up[row * P.ru + lane] : -1;
-                uint64_t bal = __ballot(nid >= 0);
-                int n = __popcll(bal);
+                const int32_t nid = lane < P.ru ? P.adj_up[row * P.ru + lane] : -1;
+                const uint64_t bal = __ballot(nid >= 0);
+                const int n = __popcll(bal);
                 wave_sync();
                 if (nid >= 0) L.lids[mbcnt(bal)] = (uint32_t)nid;
                 wave_sync();
@@ -132,8 +343,8 @@ __device__ void search_query(const KParams &P, int64_t qi, int64_t slot, unsigne
                 if (lane < n) best = ((uint64_t)ord_f(key_dist(L.lkeys[lane])) << 32) | (uint64_t)lane;
                 best = wave_min_u64(best);
                 if (n > 0) {
-                    float bd = unord_f((uint32_t)(best >> 32));
-                    if (bd < curd) {   // strictly closer
+                    const float bd = unord_f((uint32_t)(best >> 32));
+                    if (bd < curd) {  // strictly closer
                         curd = bd;
                         cur = L.lids[(int)(best & 63u)];
                         wave_sync();
@@ -147,7 +358,6 @@ __device__ void search_query(const KParams &P, int64_t qi, int64_t slot, unsigne
         epd = curd;
         have_epd = true;
     }
-
     if (!have_epd) {
         if (lane == 0) L.lids[0] = ep;
         wave_sync();
@@ -156,160 +366,124 @@ __device__ void search_query(const KParams &P, int64_t qi, int64_t slot, unsigne
         epd = key_dist(L.lkeys[0]);
     }
 
-    // layer-0 best-first search, HNSWLIB/hnswalg.h:240-326, DST groups as
-    // INTER/scheduler.hpp:100-157 with one group in flight
+    PROF_MARK(0);
+    // layer-0 search, HNSWLIB/hnswalg.h:240-326; groups of up to mc
+    // candidates, up to mg groups in flight (INTER/scheduler.hpp:100-157).
+    // A group is selected from the pool as it stands after the oldest
+    // in-flight group has been merged; the next group's adjacency loads are
+    // issued before the current group is gathered, so with mg = 2 they
+    // overlap.
     uint64_t *pool = L.poolA, *pool2 = L.poolB;
-    int np = 1;
-    uint32_t *T = P.vtab + ((size_t)slot << P.vlog2);
-    const uint32_t tmask = (1u << P.vlog2) - 1u;
-    const int tlimit = (int)((3u << P.vlog2) >> 2) - P.lcap;
+    int np = 1, hint = 0;
+    uint32_t *T = P.hlog2 > 0 ? nullptr : P.vtab + ((size_t)slot << P.vlog2);
+    const int tlimit = P.hlog2 > 0 ? (int)((3u << P.hlog2) >> 2) - P.lcap : (int)((3u << P.vlog2) >> 2) - P.lcap;
+    if (P.hlog2 > 0) {
+        uint4 *H4 = (uint4 *)L.vh;
+        const uint4 z4 = make_uint4(0u, 0u, 0u, 0u);
+        for (uint32_t i = lane; i < ((1u << P.hlog2) >> 2); i += WAVE) H4[i] = z4;
+    }
     wave_sync();
     if (lane == 0) {
         pool[0] = mk_key(epd, ep);
-        uint32_t h0 = murmur2_k4(ep, P.bloom_seed), h1 = murmur2_k4(ep, P.bloom_seed + 1u);
-        L.bloom[h0 & (nbw - 1u)] |= bloom_mask(h1);
-        uint32_t pos = (ep * 0x9E3779B1u) >> (32 - P.vlog2);
-        atomicCAS(&T[pos], 0u, ep + 1u);
+        if (P.hlog2 > 0) {
+            L.vh[vhash(ep, P.hlog2)] = ep + 1u;
+        } else {
+            uint32_t w;
+            uint64_t m;
+            bloom_pos(ep, P.bloom_seed, nbw, w, m);
+            L.bloom[w] |= m;
+            atomicCAS(&T[vhash(ep, P.vlog2)], 0u, ep + 1u);
+        }
     }
+    wave_sync();
     int hops = 0, evals = 1;
     bool overflow = false;
     const int rpc = (P.r0 + WAVE - 1) / WAVE;
-    wave_sync();
 
+    int nres = 0;
+    bool haveQ = false;
+    uint32_t *selP = L.selA, *selN = L.selB;
+    int32_t idsP[MROWS], idsN[MROWS];
+    int nselP = 0, nselN = 0;
     for (;;) {
-        // pick up to mc unexpanded entries, closest first
-        int nsel = 0;
-        for (int base = 0; base < np && nsel < P.mc; base += WAVE) {
-            int i = base + lane;
-            uint64_t key = i < np ? pool[i] : EMPTY_KEY;
-            bool unexp = (i < np) && !(key & 1ull);
-            uint64_t bal = __ballot(unexp);
-            while (bal && nsel < P.mc) {
-                int l = __ffsll((unsigned long long)bal) - 1;
-                bal &= bal - 1;
-                if (lane == l) {
-                    pool[i] = key | 1ull;
-                    L.sel[nsel] = key_id(key);
-                }
-                nsel++;
-            }
+        if (haveQ) {  // complete the oldest group
+            np = merge_into_pool(pool, pool2, np, ef, L.lkeys, nres, hint);
+            haveQ = false;
         }
-        if (nsel == 0) break;
-        wave_sync();
-        hops += nsel;
-        const int nrows = nsel * rpc;
-
-        int32_t ids[MAX_ROWS];
-        uint32_t prev[MAX_ROWS], pos[MAX_ROWS];
-        int spos[MAX_ROWS];
+        PROF_MARK(6);
+        // refill the pipeline up to mg groups
+        if (nselP == 0) {
+            nselP = select_group(pool, np, P.mc, hint, selP);
+            if (nselP) load_adj<MROWS>(P, selP, nselP, rpc, idsP);
+            hops += nselP;
+        }
+        if (P.mg > 1 && nselP > 0 && nselN == 0) {
+            nselN = select_group(pool, np, P.mc, hint, selN);
+            if (nselN) load_adj<MROWS>(P, selN, nselN, rpc, idsN);
+            hops += nselN;
+        }
+        if (nselP == 0) break;
+        PROF_MARK(1);
+        if (P.hlog2 > 0)
+            nres = expand_lds<NCH, STORE, METRIC, MROWS>(P, L, qf, idsP, nselP * rpc, evals EXP_PROF_ARGS);
+        else
+            nres = expand_group<NCH, STORE, METRIC, MROWS>(P, L, qf, T, idsP, nselP * rpc, evals EXP_PROF_ARGS);
+        haveQ = true;
+        // the next group moves up
+        uint32_t *ts = selP;
+        selP = selN;
+        selN = ts;
+        nselP = nselN;
+        nselN = 0;
 #pragma unroll
-        for (int r = 0; r < MAX_ROWS; r++) {
-            ids[r] = -1;
-            prev[r] = 0u;
-            pos[r] = 0u;
-            spos[r] = 0;
-            if (r < nrows) {
-                int ci = r / rpc, part = r - ci * rpc;
-                int j = part * WAVE + lane;
-                if (j < P.r0) ids[r] = P.adj0[(size_t)L.sel[ci] * P.r0 + j];
-            }
-        }
-        // Bloom probe (LDS) + exact visited insert (HBM CAS) + speculative list
-        uint32_t spec_bits = 0u;
-        int nl = 0;
-#pragma unroll
-        for (int r = 0; r < MAX_ROWS; r++) {
-            if (r < nrows) {
-                const int32_t id = ids[r];
-                const bool valid = id >= 0;
-                bool maybe = true;
-                if (valid) {
-                    uint32_t h0 = murmur2_k4((uint32_t)id, P.bloom_seed);
-                    uint32_t h1 = murmur2_k4((uint32_t)id, P.bloom_seed + 1u);
-                    uint32_t m = bloom_mask(h1);
-                    uint32_t old = atomicOr(&L.bloom[h0 & (nbw - 1u)], m);
-                    maybe = (old & m) == m;
-                    pos[r] = ((uint32_t)id * 0x9E3779B1u) >> (32 - P.vlog2);
-                    prev[r] = atomicCAS(&T[pos[r]], 0u, (uint32_t)id + 1u);
-                }
-                const bool spec = valid && !maybe;
-                uint64_t bal = __ballot(spec);
-                if (spec) {
-                    spos[r] = nl + mbcnt(bal);
-                    L.lids[spos[r]] = (uint32_t)id;
-                }
-                nl += __popcll(bal);
-                spec_bits |= (spec ? 1u : 0u) << r;
-            }
-        }
-        wave_sync();
-        eval_rows<NCH, STORE, METRIC>(P, qf, L.lids, nl, L.lkeys);
-        // resolve the CAS results
-        int nl2 = nl;
-#pragma unroll
-        for (int r = 0; r < MAX_ROWS; r++) {
-            if (r < nrows) {
-                const int32_t id = ids[r];
-                const bool valid = id >= 0;
-                const uint32_t want = (uint32_t)id + 1u;
-                bool pending = valid && prev[r] != 0u && prev[r] != want;
-                while (__ballot(pending)) {
-                    if (pending) {
-                        pos[r] = (pos[r] + 1u) & tmask;
-                        prev[r] = atomicCAS(&T[pos[r]], 0u, want);
-                        pending = prev[r] != 0u && prev[r] != want;
-                    }
-                }
-                const bool won = valid && prev[r] == 0u;
-                const bool spec = (spec_bits >> r) & 1u;
-                if (spec && !won) L.lkeys[spos[r]] = EMPTY_KEY;   // same id twice in one group
-                const bool late = won && !spec;                    // Bloom false positive
-                uint64_t bal = __ballot(late);
-                if (late) L.lids[nl2 + mbcnt(bal)] = (uint32_t)id;
-                nl2 += __popcll(bal);
-                evals += __popcll(__ballot(won));
-            }
-        }
-        wave_sync();
-        if (nl2 > nl) {
-            eval_rows<NCH, STORE, METRIC>(P, qf, L.lids + nl, nl2 - nl, L.lkeys + nl);
-            wave_sync();
-        }
+        for (int r = 0; r < MROWS; r++) idsP[r] = idsN[r];
         if (evals > tlimit) {
             overflow = true;
             break;
         }
-        np = merge_into_pool(pool, pool2, np, ef, L.lkeys, nl2);
     }
 
     // results, closer first (HNSWLIB/hnswlib.h:171-185)
     for (int i = lane; i < P.k; i += WAVE) {
-        uint64_t key = (!overflow && i < np && i < k) ? pool[i] : EMPTY_KEY;
+        const uint64_t key = (!overflow && i < np && i < k) ? pool[i] : EMPTY_KEY;
         int64_t lab = -1;
         float d = __builtin_inff();
         if (key != EMPTY_KEY) {
-            uint32_t id = key_id(key);
+            const uint32_t id = key_id(key);
             lab = P.labels ? P.labels[id] : (int64_t)id;
             d = key_dist(key);
         }
         P.out_labels[qi * P.k + i] = lab;
         P.out_dists[qi * P.k + i] = d;
     }
+    PROF_MARK(7);
     if (lane == 0) {
+#ifdef GVS_PROFILE
+        if (P.out_stats) {
+            P.out_stats[qi * 16 + 0] = hops;
+            P.out_stats[qi * 16 + 1] = evals;
+            P.out_stats[qi * 16 + 2] = up_hops;
+            P.out_stats[qi * 16 + 3] = up_evals;
+            for (int ph = 0; ph < 8; ph++) P.out_stats[qi * 16 + 4 + ph] = (int32_t)(prof_acc[ph] >> 6);
+        }
+#else
         if (P.out_stats) {
             P.out_stats[qi * 4 + 0] = hops;
             P.out_stats[qi * 4 + 1] = evals;
             P.out_stats[qi * 4 + 2] = up_hops;
             P.out_stats[qi * 4 + 3] = up_evals;
         }
+#endif
         if (P.status) P.status[qi] = overflow ? 1 : 0;
         if (overflow) atomicAdd(&P.counters[1], 1u);
     }
     // leave the slot's visited table empty for the next query
-    uint4 *T4 = (uint4 *)T;
-    const uint4 z = make_uint4(0u, 0u, 0u, 0u);
-    for (uint32_t i = lane; i < ((1u << P.vlog2) >> 2); i += WAVE) T4[i] = z;
-    __threadfence();
+    if (T) {
+        uint4 *T4 = (uint4 *)T;
+        const uint4 z = make_uint4(0u, 0u, 0u, 0u);
+        for (uint32_t i = lane; i < ((1u << P.vlog2) >> 2); i += WAVE) T4[i] = z;
+        __threadfence();
+    }
     wave_sync();
 }
 

@@ -14,10 +14,12 @@ import os
 import numpy as np
 
 _PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(_PKG, "lib", "libgvs.so")
+LIB_PATH = os.environ.get("GVS_LIBRARY") or os.path.join(_PKG, "lib", "libgvs.so")
 
 GVS_OK = 0
 FLAG_ASYNC = 1
+FLAG_HBM_VISITED = 2
+FLAG_LDS_VISITED = 4
 ERRORS = {-1: "ARG", -2: "HIP", -3: "NOMEM", -4: "UNSUPPORTED", -5: "OVERFLOW", -6: "IO", -7: "STATE"}
 L2, IP = 0, 1
 F32, BF16, FP8 = 0, 1, 2
@@ -74,6 +76,7 @@ def _load():
         "gvs_search_device": (C.c_int, [P, P, I64, I64, P, P, P, P, P, P]),
         "gvs_search_host": (C.c_int, [P, P, I64, I64, P, P, P, P, P]),
         "gvs_search_overflow_count": (C.c_int, [P, P]),
+        "gvs_device_copy": (C.c_int, [P, P, I64, P]),
         "gvs_bruteforce_f64_device": (C.c_int, [P, P, I64, I64, I32, P, P, P]),
         "gvs_server_open": (C.c_int, [P, P, P]),
         "gvs_server_submit": (I64, [P, P, I32, I32]),
@@ -99,7 +102,7 @@ def exported_symbols():
     names = ["gvs_last_error", "gvs_version", "gvs_hnsw_build", "gvs_nsg_from_hnsw", "gvs_hnswlib_load",
              "gvs_hnswlib_save", "gvs_nsg_load", "gvs_host_graph_view", "gvs_host_graph_free",
              "gvs_gen_mixture", "gvs_index_create", "gvs_index_destroy", "gvs_index_device_bytes",
-             "gvs_search_device", "gvs_search_host", "gvs_search_overflow_count", "gvs_bruteforce_f64_device", "gvs_server_open",
+             "gvs_search_device", "gvs_search_host", "gvs_search_overflow_count", "gvs_device_copy", "gvs_bruteforce_f64_device", "gvs_server_open",
              "gvs_server_submit", "gvs_server_poll", "gvs_server_cancel", "gvs_server_stats",
              "gvs_server_close"]
     return [n for n in names if hasattr(lib, n)]
@@ -112,6 +115,11 @@ def _check(rc):
 
 def _ptr(a):
     return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def device_copy(dst_ptr, src_ptr, nbytes, stream=None):
+    """Device-to-device copy through the library's streaming kernel."""
+    _check(lib.gvs_device_copy(C.c_void_p(dst_ptr), C.c_void_p(src_ptr), nbytes, C.c_void_p(stream) if stream else None))
 
 
 def gen_mixture(n, dim, ncenters, seed_centers, seed_points, center_std=4.0, noise_std=1.0,
@@ -193,9 +201,15 @@ class HostGraph:
 
 
 def make_params(k=10, ef=64, mc=1, mg=1, waves_per_query=1, storage=F32, rerank=False, persistent=False,
-                visited_log2=0, bloom_log2=0, grid_waves=0, asynchronous=False):
+                visited_log2=0, bloom_log2=0, grid_waves=0, asynchronous=False, visited=None):
+    """visited: None (auto), "hbm" (LDS Bloom + HBM hash) or "lds" (exact LDS hash)."""
+    flags = FLAG_ASYNC if asynchronous else 0
+    if visited == "hbm":
+        flags |= FLAG_HBM_VISITED
+    elif visited == "lds":
+        flags |= FLAG_LDS_VISITED
     return Params(k, ef, mc, mg, waves_per_query, storage, int(bool(rerank)), int(bool(persistent)),
-                  visited_log2, bloom_log2, grid_waves, FLAG_ASYNC if asynchronous else 0)
+                  visited_log2, bloom_log2, grid_waves, flags)
 
 
 class DeviceIndex:

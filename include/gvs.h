@@ -83,6 +83,12 @@ typedef struct gvs_params {
  * reads the count with gvs_search_overflow_count() after synchronising and
  * re-issues them (the synchronous form retries them itself). */
 #define GVS_FLAG_ASYNC 1
+/* Visited-set placement.  By default a query with ef_eff <= 96 keeps an exact
+ * visited hash in LDS; larger ef use an LDS Bloom filter in front of an exact
+ * per-query hash in HBM.  GVS_FLAG_HBM_VISITED forces the HBM form,
+ * GVS_FLAG_LDS_VISITED the LDS form (visited_log2 in [10, 14] sets its size). */
+#define GVS_FLAG_HBM_VISITED 2
+#define GVS_FLAG_LDS_VISITED 4
 
 /* Per-query debug counters written when stats != NULL (4 int32 per query):
  * layer-0 hops, layer-0 distance evaluations, upper-layer list scans,
@@ -154,6 +160,10 @@ GVS_API int gvs_search_device(gvs_index *ix, const float *queries, int64_t nq, i
 /* Number of queries of the last search on this index whose visited table
  * overflowed (only non-zero after a GVS_FLAG_ASYNC search; synchronises). */
 GVS_API int gvs_search_overflow_count(gvs_index *ix, int64_t *count);
+/* Device-to-device copy through a plain streaming kernel (16-byte aligned
+ * pointers and size).  No reference counterpart; used to calibrate HBM byte
+ * counters against a kernel of known traffic. */
+GVS_API int gvs_device_copy(void *dst, const void *src, int64_t bytes, void *stream);
 /* Same with HOST buffers; synchronous (copies in and out). */
 GVS_API int gvs_search_host(gvs_index *ix, const float *queries, int64_t nq, int64_t ldq,
                             const int32_t *entries, const gvs_params *p,

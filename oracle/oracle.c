@@ -562,22 +562,26 @@ uint32_t orc_murmur2_keylen4(uint32_t key, uint32_t seed)
     return h;
 }
 
-/* Blocked Bloom filter: the word is chosen by hash(id, seed), three bit
- * positions by hash(id, seed + 1); the reference uses three hash functions
- * with seeds hash_seed + pe (bloom_filter.hpp:274-288). */
-void orc_bloom_word_mask(uint32_t id, uint32_t seed, uint32_t nwords, uint32_t *word, uint32_t *mask)
+/* Blocked Bloom filter as the HIP kernels keep it in LDS: one 64-bit word
+ * per id chosen by hash(id, seed), six bit positions from hash(id, seed + 1)
+ * (five 6-bit fields) and the top six bits of hash(id, seed).  The reference
+ * uses three hash functions with seeds hash_seed + pe over a bit array
+ * (bloom_filter.hpp:174-188, :274-288). */
+void orc_bloom_word_mask(uint32_t id, uint32_t seed, uint32_t nwords, uint32_t *word, uint64_t *mask)
 {
     uint32_t h0 = orc_murmur2_keylen4(id, seed);
     uint32_t h1 = orc_murmur2_keylen4(id, seed + 1u);
     *word = h0 & (nwords - 1u);
-    *mask = (1u << (h1 & 31u)) | (1u << ((h1 >> 5) & 31u)) | (1u << ((h1 >> 10) & 31u));
+    *mask = (1ull << (h1 & 63u)) | (1ull << ((h1 >> 6) & 63u)) | (1ull << ((h1 >> 12) & 63u)) |
+            (1ull << ((h1 >> 18) & 63u)) | (1ull << ((h1 >> 24) & 63u)) | (1ull << (h0 >> 26));
 }
 
-int orc_bloom_test_and_set(uint32_t *words, uint32_t nwords, uint32_t seed, uint32_t id)
+int orc_bloom_test_and_set(uint64_t *words, uint32_t nwords, uint32_t seed, uint32_t id)
 {
-    uint32_t w, m;
+    uint32_t w;
+    uint64_t m;
     orc_bloom_word_mask(id, seed, nwords, &w, &m);
-    uint32_t old = words[w];
+    uint64_t old = words[w];
     words[w] = old | m;
     return (old & m) == m;
 }

@@ -135,10 +135,10 @@ int orc_rerank_f32(const float *base_f32, int dim_pad, int metric, const float *
 /* hashing / Bloom filter */
 uint32_t orc_murmur2(const void *key, int len, uint32_t seed);
 uint32_t orc_murmur2_keylen4(uint32_t key, uint32_t seed);
-/* blocked Bloom filter (one 32-bit word per key, 3 bits), word array of nwords (pow2) */
-void orc_bloom_word_mask(uint32_t id, uint32_t seed, uint32_t nwords, uint32_t *word, uint32_t *mask);
+/* blocked Bloom filter (one 64-bit word per key, 6 bits), word array of nwords (pow2) */
+void orc_bloom_word_mask(uint32_t id, uint32_t seed, uint32_t nwords, uint32_t *word, uint64_t *mask);
 /* returns 1 if the id was (maybe) present, 0 if definitely absent; inserts it */
-int orc_bloom_test_and_set(uint32_t *words, uint32_t nwords, uint32_t seed, uint32_t id);
+int orc_bloom_test_and_set(uint64_t *words, uint32_t nwords, uint32_t seed, uint32_t id);
 
 #ifdef __cplusplus
 }

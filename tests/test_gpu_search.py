@@ -55,6 +55,16 @@ def test_dst_multi_candidate_matches_oracle(mc):
     check_equal(dev, og, q, 10, 64, mc=mc)
 
 
+@pytest.mark.parametrize("mc,mg", [(1, 2), (2, 2), (4, 2)])
+def test_dst_pipelined_groups_match_oracle(mc, mg):
+    x, g, dev = graph(20000, 128, 16, 100)
+    og = orc.OracleGraph(g.arrays())
+    q = queries(300, 128, seed=14)
+    print(f"case dst mc={mc} mg={mg}", flush=True)
+    check_equal(dev, og, q, 10, 64, mc=mc, mg=mg)
+    check_equal(dev, og, q, 10, 200, mc=mc, mg=mg)
+
+
 def test_persistent_equals_plain():
     x, g, dev = graph(20000, 128, 16, 100)
     q = queries(2000, 128, seed=4)

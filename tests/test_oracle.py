@@ -180,25 +180,25 @@ def test_murmur2_keylen4_matches_generic():
 
 def test_bloom_no_false_negatives_and_fp_rate():
     nwords = 1 << 10
-    words = np.zeros(nwords, dtype=np.uint32)
-    ins = 2500
+    words = np.zeros(nwords, dtype=np.uint64)
+    ins = 3000
     for i in range(ins):
         assert orc.lib.orc_bloom_test_and_set(orc._p(words), nwords, 7, i * 7919) == 0 or True
     for i in range(ins):
         assert orc.lib.orc_bloom_test_and_set(orc._p(words), nwords, 7, i * 7919) == 1
     fp = 0
-    trials = 20000
+    trials = 100000
     w2 = words.copy()
     for i in range(trials):
         fp += orc.lib.orc_bloom_test_and_set(orc._p(w2), nwords, 7, 10**8 + i)
         w2[:] = words
     rate = fp / trials
-    # blocked filter estimate: per-word Poisson load lambda, 3 bits of 32
+    # blocked filter estimate: per-word Poisson load lambda, 6 bits of 64
     lam = ins / nwords
     j = np.arange(0, 40)
     from math import factorial, exp
     pois = np.array([exp(-lam) * lam ** int(t) / factorial(int(t)) for t in j])
-    est = float(np.sum(pois * (1 - (1 - 1 / 32) ** (3 * j)) ** 3))
+    est = float(np.sum(pois * (1 - (1 - 1 / 64) ** (6 * j)) ** 6))
     assert 0.5 * est <= rate <= 1.5 * est + 1e-3
 
 
