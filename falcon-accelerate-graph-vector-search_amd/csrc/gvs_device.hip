@@ -322,7 +322,7 @@ int plan_search(const gvs_index *ix, const gvs_params *p, int max_ef, int max_k,
         int want = ilog2_ceil(((int64_t)efe * 40 + lp->lcap) * 4 / 3);
         lp->hlog2 = (p->flags & GVS_FLAG_LDS_VISITED) && p->visited_log2 >= 10 && p->visited_log2 <= 14
                         ? p->visited_log2 : std::max(11, std::min(13, want));
-        lp->blog2 = 0;
+        lp->blog2 = 1;  // 16 bytes: keeps the LDS hash 16-byte aligned
     }
     if (lp->blog2 > 13 || lp->vlog2 > 26) {
         set_error("gvs_search: bloom_log2 <= 13 and visited_log2 <= 26 required");
@@ -422,7 +422,7 @@ int launch_search(gvs_index *ix, const gvs_params *p, const LaunchPlan &lp, KPar
         int64_t gw = p->grid_waves > 0 ? p->grid_waves : lp.resident_waves;
         gw = std::max<int64_t>(lp.wpb, std::min<int64_t>(gw, (nq + lp.wpb - 1) / lp.wpb * lp.wpb));
         gw = (gw + lp.wpb - 1) / lp.wpb * lp.wpb;
-        int rc = ensure_scratch(ix, lp.hlog2 > 0 ? 1 : gw, lp.vlog2, nq);
+        int rc = ensure_scratch(ix, gw, lp.vlog2, nq);
         if (rc) return rc;
         P.vtab = ix->d_vtab;
         P.status = ix->d_status;
@@ -438,7 +438,7 @@ int launch_search(gvs_index *ix, const gvs_params *p, const LaunchPlan &lp, KPar
     int64_t slots = std::min<int64_t>(nq, std::max<int64_t>(lp.wpb, ((int64_t)1 << 30) >> lp.vlog2));
     slots = std::min<int64_t>(slots, 65536);
     slots = (slots + lp.wpb - 1) / lp.wpb * lp.wpb;
-    int rc = ensure_scratch(ix, lp.hlog2 > 0 ? 1 : slots, lp.vlog2, nq);
+    int rc = ensure_scratch(ix, slots, lp.vlog2, nq);
     if (rc) return rc;
     P.vtab = ix->d_vtab;
     P.status = ix->d_status;

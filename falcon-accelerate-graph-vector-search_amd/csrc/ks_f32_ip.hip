@@ -1,0 +1,19 @@
+// Serving kernel instantiations: fp32 storage, IP metric.
+#include "serve_kernel.hpp"
+
+using namespace gvsk;
+
+sfn_t gvs_serve_kernel_f32_ip(int nch, int rows)
+{
+    switch (nch) {
+    case 1: return pick_serve_rows<1, STORE_F32, METRIC_IP>(rows);
+    case 2: return pick_serve_rows<2, STORE_F32, METRIC_IP>(rows);
+    case 3: return pick_serve_rows<3, STORE_F32, METRIC_IP>(rows);
+    case 4: return pick_serve_rows<4, STORE_F32, METRIC_IP>(rows);
+    case 6: return pick_serve_rows<6, STORE_F32, METRIC_IP>(rows);
+    case 8: return pick_serve_rows<8, STORE_F32, METRIC_IP>(rows);
+    case 12: return pick_serve_rows<12, STORE_F32, METRIC_IP>(rows);
+    case 16: return pick_serve_rows<16, STORE_F32, METRIC_IP>(rows);
+    default: return nullptr;
+    }
+}

@@ -199,7 +199,7 @@ __device__ __forceinline__ void decode_chunk(const uint4 raw, float scale, float
 #pragma unroll
         for (int i = 0; i < 4; i++)
 #pragma unroll
-            for (int b = 0; b < 4; b++) x[4 * i + b] = e4m3_to_f32((w[i] >> (8 * b)) & 0xFFu) * scale;
+            for (int b = 0; b < 4; b++) x[4 * i + b] = __fmul_rn(e4m3_to_f32((w[i] >> (8 * b)) & 0xFFu), scale);  // rounded before use (no fma contraction)
     }
 }
 

@@ -20,6 +20,7 @@ namespace gvsk {
 struct WaveLds {
     uint64_t *poolA, *poolB, *lkeys, *bloom;
     uint32_t *lids, *selA, *selB, *vh;
+    uint32_t nbw;  // Bloom words
 };
 
 __device__ __forceinline__ WaveLds carve(const KParams &P, unsigned char *base)
@@ -33,6 +34,7 @@ __device__ __forceinline__ WaveLds carve(const KParams &P, unsigned char *base)
     L.selA = L.lids + P.lcap;
     L.selB = L.selA + 8;
     L.vh = L.selB + 8;
+    L.nbw = 1u << P.blog2;
     return L;
 }
 
@@ -164,7 +166,7 @@ __device__ __forceinline__ int expand_group(const KParams &P, const WaveLds &L, 
                                             const int32_t (&ids)[MROWS], int nrows, int &evals EXP_PROF)
 {
     const int lane = lane_id();
-    const uint32_t nbw = 1u << P.blog2;
+    const uint32_t nbw = L.nbw;
     const uint32_t tmask = (1u << P.vlog2) - 1u;
     uint32_t prev[MROWS], pos[MROWS];
     int spos[MROWS];
@@ -285,31 +287,28 @@ __device__ __forceinline__ int expand_lds(const KParams &P, const WaveLds &L, co
 }
 
 template <int NCH, int STORE, int METRIC, int MROWS>
-__device__ void search_query(const KParams &P, int64_t qi, int64_t slot, unsigned char *lds)
+__device__ void search_query_req(const KParams &P, int64_t qi, int64_t slot, unsigned char *lds, int k, int ef,
+                                 const float *Q, bool sys_query)
 {
     constexpr int E = StoreTraits<STORE>::E;
     const int lane = lane_id();
     const int p = lane & 7;
     WaveLds L = carve(P, lds);
-    const uint32_t nbw = 1u << P.blog2;
 
     PROF_DECL
-    const int64_t qrow = P.q_index ? P.q_index[qi] : qi;
-    const int k = P.q_k ? P.q_k[qi] : P.k;
-    int ef = P.q_ef ? P.q_ef[qi] : P.ef;
     if (ef < k) ef = k;
 
     float qf[NCH * E];
-    const float *Q = P.queries + qrow * P.ldq;
 #pragma unroll
     for (int i = 0; i < NCH; i++)
 #pragma unroll
         for (int t = 0; t < E; t++) {
             const int idx = (p + 8 * i) * E + t;
-            qf[i * E + t] = idx < P.dim ? Q[idx] : 0.0f;
+            float v = 0.0f;
+            if (idx < P.dim)
+                v = sys_query ? __hip_atomic_load(&Q[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) : Q[idx];
+            qf[i * E + t] = v;
         }
-
-    for (uint32_t w = lane; w < nbw; w += WAVE) L.bloom[w] = 0ull;
 
     int up_hops = 0, up_evals = 0;
     uint32_t ep;
@@ -375,71 +374,135 @@ __device__ void search_query(const KParams &P, int64_t qi, int64_t slot, unsigne
     // overlap.
     uint64_t *pool = L.poolA, *pool2 = L.poolB;
     int np = 1, hint = 0;
-    uint32_t *T = P.hlog2 > 0 ? nullptr : P.vtab + ((size_t)slot << P.vlog2);
-    const int tlimit = P.hlog2 > 0 ? (int)((3u << P.hlog2) >> 2) - P.lcap : (int)((3u << P.vlog2) >> 2) - P.lcap;
-    if (P.hlog2 > 0) {
-        uint4 *H4 = (uint4 *)L.vh;
-        const uint4 z4 = make_uint4(0u, 0u, 0u, 0u);
-        for (uint32_t i = lane; i < ((1u << P.hlog2) >> 2); i += WAVE) H4[i] = z4;
-    }
-    wave_sync();
-    if (lane == 0) {
-        pool[0] = mk_key(epd, ep);
-        if (P.hlog2 > 0) {
-            L.vh[vhash(ep, P.hlog2)] = ep + 1u;
-        } else {
-            uint32_t w;
-            uint64_t m;
-            bloom_pos(ep, P.bloom_seed, nbw, w, m);
-            L.bloom[w] |= m;
-            atomicCAS(&T[vhash(ep, P.vlog2)], 0u, ep + 1u);
-        }
-    }
-    wave_sync();
     int hops = 0, evals = 1;
     bool overflow = false;
+    uint32_t *T = nullptr;
     const int rpc = (P.r0 + WAVE - 1) / WAVE;
+    // A query that outgrows the LDS visited hash starts layer 0 again with
+    // the HBM table, its Bloom filter taking the LDS hash's space.
+    bool lds_mode = P.hlog2 > 0;
+    for (int pass = 0; pass < 2; pass++) {
+        if (!lds_mode) {
+            T = P.vtab + ((size_t)slot << P.vlog2);
+            if (P.hlog2 > 0) {
+                L.bloom = (uint64_t *)L.vh;
+                L.nbw = 1u << (P.hlog2 - 1);
+            }
+        }
+        const int tlimit = lds_mode ? (int)((3u << P.hlog2) >> 2) - P.lcap : (int)((3u << P.vlog2) >> 2) - P.lcap;
+        pool = L.poolA;
+        pool2 = L.poolB;
+        np = 1;
+        hint = 0;
+        hops = 0;
+        evals = 1;
+        overflow = false;
+        if (lds_mode) {
+            uint4 *H4 = (uint4 *)L.vh;
+            const uint4 z4 = make_uint4(0u, 0u, 0u, 0u);
+            for (uint32_t i = lane; i < ((1u << P.hlog2) >> 2); i += WAVE) H4[i] = z4;
+        } else {
+            for (uint32_t w = lane; w < L.nbw; w += WAVE) L.bloom[w] = 0ull;
+        }
+        wave_sync();
+        if (lane == 0) {
+            pool[0] = mk_key(epd, ep);
+            if (lds_mode) {
+                L.vh[vhash(ep, P.hlog2)] = ep + 1u;
+            } else {
+                uint32_t w;
+                uint64_t m;
+                bloom_pos(ep, P.bloom_seed, L.nbw, w, m);
+                L.bloom[w] |= m;
+                atomicCAS(&T[vhash(ep, P.vlog2)], 0u, ep + 1u);
+            }
+        }
+        wave_sync();
 
-    int nres = 0;
-    bool haveQ = false;
-    uint32_t *selP = L.selA, *selN = L.selB;
-    int32_t idsP[MROWS], idsN[MROWS];
-    int nselP = 0, nselN = 0;
-    for (;;) {
-        if (haveQ) {  // complete the oldest group
-            np = merge_into_pool(pool, pool2, np, ef, L.lkeys, nres, hint);
-            haveQ = false;
-        }
-        PROF_MARK(6);
-        // refill the pipeline up to mg groups
-        if (nselP == 0) {
-            nselP = select_group(pool, np, P.mc, hint, selP);
-            if (nselP) load_adj<MROWS>(P, selP, nselP, rpc, idsP);
-            hops += nselP;
-        }
-        if (P.mg > 1 && nselP > 0 && nselN == 0) {
-            nselN = select_group(pool, np, P.mc, hint, selN);
-            if (nselN) load_adj<MROWS>(P, selN, nselN, rpc, idsN);
-            hops += nselN;
-        }
-        if (nselP == 0) break;
-        PROF_MARK(1);
-        if (P.hlog2 > 0)
-            nres = expand_lds<NCH, STORE, METRIC, MROWS>(P, L, qf, idsP, nselP * rpc, evals EXP_PROF_ARGS);
-        else
-            nres = expand_group<NCH, STORE, METRIC, MROWS>(P, L, qf, T, idsP, nselP * rpc, evals EXP_PROF_ARGS);
-        haveQ = true;
-        // the next group moves up
-        uint32_t *ts = selP;
-        selP = selN;
-        selN = ts;
-        nselP = nselN;
-        nselN = 0;
+        int nres = 0;
+        bool haveQ = false;
+        uint32_t *selP = L.selA, *selN = L.selB;
+        int32_t idsP[MROWS], idsN[MROWS];
+        int nselP = 0, nselN = 0;
+        for (;;) {
+            if (haveQ) {  // complete the oldest group
+                np = merge_into_pool(pool, pool2, np, ef, L.lkeys, nres, hint);
+                haveQ = false;
+            }
+            PROF_MARK(6);
+            // refill the pipeline up to mg groups
+            if (nselP == 0) {
+                nselP = select_group(pool, np, P.mc, hint, selP);
+                if (nselP) load_adj<MROWS>(P, selP, nselP, rpc, idsP);
+                hops += nselP;
+            }
+            if (P.mg > 1 && nselP > 0 && nselN == 0) {
+                nselN = select_group(pool, np, P.mc, hint, selN);
+                if (nselN) load_adj<MROWS>(P, selN, nselN, rpc, idsN);
+                hops += nselN;
+            }
+            if (nselP == 0) break;
+            PROF_MARK(1);
+            if (lds_mode)
+                nres = expand_lds<NCH, STORE, METRIC, MROWS>(P, L, qf, idsP, nselP * rpc, evals EXP_PROF_ARGS);
+            else
+                nres = expand_group<NCH, STORE, METRIC, MROWS>(P, L, qf, T, idsP, nselP * rpc, evals EXP_PROF_ARGS);
+            haveQ = true;
+            // the next group moves up
+            uint32_t *ts = selP;
+            selP = selN;
+            selN = ts;
+            nselP = nselN;
+            nselN = 0;
 #pragma unroll
-        for (int r = 0; r < MROWS; r++) idsP[r] = idsN[r];
-        if (evals > tlimit) {
-            overflow = true;
-            break;
+            for (int r = 0; r < MROWS; r++) idsP[r] = idsN[r];
+            if (evals > tlimit) {
+                overflow = true;
+                break;
+            }
+        }
+        if (!(overflow && lds_mode)) break;
+        lds_mode = false;
+        wave_sync();
+    }
+
+    // Low-precision traversal: rerank the final pool with exact fp32
+    // distances (fp32 copy, the kernels' fp32 summation order), then sort by
+    // (dist, id).
+    if constexpr (STORE != STORE_F32) {
+        if (P.rerank && !overflow) {
+            constexpr int NCH32 = NCH * E / 4;
+            float q32[NCH32 * 4];
+#pragma unroll
+            for (int i = 0; i < NCH32; i++)
+#pragma unroll
+                for (int t = 0; t < 4; t++) {
+                    const int idx = (p + 8 * i) * 4 + t;
+                    float v = 0.0f;
+                    if (idx < P.dim)
+                        v = sys_query ? __hip_atomic_load(&Q[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) : Q[idx];
+                    q32[i * 4 + t] = v;
+                }
+            KParams P32 = P;
+            P32.vec = P.vec_f32;
+            P32.row_elems = P.row_elems_f32;
+            for (int b = 0; b < np; b += P.lcap) {
+                const int cnt = min(P.lcap, np - b);
+                for (int i = lane; i < cnt; i += WAVE) L.lids[i] = key_id(pool[b + i]);
+                wave_sync();
+                eval_rows<NCH32, STORE_F32, METRIC>(P32, q32, L.lids, cnt, L.lkeys);
+                wave_sync();
+                for (int i = lane; i < cnt; i += WAVE) pool2[b + i] = L.lkeys[i];
+                wave_sync();
+            }
+            // rank sort of the reranked keys
+            for (int i = lane; i < np; i += WAVE) {
+                const uint64_t v = pool2[i];
+                int r = 0;
+                for (int j = 0; j < np; j++) r += (pool2[j] < v) ? 1 : 0;
+                pool[r] = v;
+            }
+            wave_sync();
         }
     }
 
@@ -482,9 +545,20 @@ __device__ void search_query(const KParams &P, int64_t qi, int64_t slot, unsigne
         uint4 *T4 = (uint4 *)T;
         const uint4 z = make_uint4(0u, 0u, 0u, 0u);
         for (uint32_t i = lane; i < ((1u << P.vlog2) >> 2); i += WAVE) T4[i] = z;
-        __threadfence();
+        // only this wave uses the slot's table, and only through L2 atomics:
+        // the cleared lines must have reached L2 before its next CAS
+        __builtin_amdgcn_s_waitcnt(0);
     }
     wave_sync();
+}
+
+template <int NCH, int STORE, int METRIC, int MROWS>
+__device__ __forceinline__ void search_query(const KParams &P, int64_t qi, int64_t slot, unsigned char *lds)
+{
+    const int64_t qrow = P.q_index ? P.q_index[qi] : qi;
+    const int k = P.q_k ? P.q_k[qi] : P.k;
+    const int ef = P.q_ef ? P.q_ef[qi] : P.ef;
+    search_query_req<NCH, STORE, METRIC, MROWS>(P, qi, slot, lds, k, ef, P.queries + qrow * P.ldq, false);
 }
 
 }  // namespace gvsk

@@ -82,6 +82,8 @@ def _load():
         "gvs_server_submit": (I64, [P, P, I32, I32]),
         "gvs_server_poll": (C.c_int, [P, I32, P, P, P, P]),
         "gvs_server_cancel": (C.c_int, [P, I64]),
+        "gvs_server_poll_stats": (C.c_int, [P, I32, P, P, P, P, P]),
+        "gvs_server_loadgen": (C.c_int, [P, P, I64, I64, C.c_double, U64, P, P, P, P, P, P]),
         "gvs_server_stats": (C.c_int, [P, P]),
         "gvs_server_close": (C.c_int, [P]),
     }
@@ -103,8 +105,8 @@ def exported_symbols():
              "gvs_hnswlib_save", "gvs_nsg_load", "gvs_host_graph_view", "gvs_host_graph_free",
              "gvs_gen_mixture", "gvs_index_create", "gvs_index_destroy", "gvs_index_device_bytes",
              "gvs_search_device", "gvs_search_host", "gvs_search_overflow_count", "gvs_device_copy", "gvs_bruteforce_f64_device", "gvs_server_open",
-             "gvs_server_submit", "gvs_server_poll", "gvs_server_cancel", "gvs_server_stats",
-             "gvs_server_close"]
+             "gvs_server_submit", "gvs_server_poll", "gvs_server_poll_stats", "gvs_server_cancel",
+             "gvs_server_stats", "gvs_server_close", "gvs_server_loadgen"]
     return [n for n in names if hasattr(lib, n)]
 
 
@@ -268,6 +270,82 @@ class DeviceIndex:
         if h is not None and h.value and lib is not None:
             lib.gvs_index_destroy(h)
             self._h = None
+
+
+class Server:
+    """Continuous-batching serving loop over a DeviceIndex (C ABI gvs_server_*)."""
+
+    def __init__(self, dev, max_inflight=1024, max_k=100, max_ef=512, queue_capacity=0, grid_waves=0, mc=1):
+        self.dev = dev
+        self.max_k = max_k
+        cfg = ServerConfig(max_inflight, max_k, max_ef, queue_capacity, grid_waves, mc, F32, 0)
+        self._h = C.c_void_p()
+        _check(lib.gvs_server_open(dev.handle, C.byref(cfg), C.byref(self._h)))
+
+    def submit(self, query, k, ef):
+        q = np.ascontiguousarray(query, dtype=np.float32)
+        r = lib.gvs_server_submit(self._h, _ptr(q), k, ef)
+        if r < 0:
+            raise GvsError(int(r), lib.gvs_last_error().decode())
+        return r
+
+    def try_submit(self, query, k, ef):
+        """Like submit, but returns None when all device slots are busy."""
+        q = np.ascontiguousarray(query, dtype=np.float32)
+        r = lib.gvs_server_submit(self._h, _ptr(q), k, ef)
+        if r == -7:
+            return None
+        if r < 0:
+            raise GvsError(int(r), lib.gvs_last_error().decode())
+        return r
+
+    def poll(self, max_done=256, with_stats=False):
+        rid = np.empty(max_done, dtype=np.int64)
+        lab = np.empty((max_done, self.max_k), dtype=np.int64)
+        dist = np.empty((max_done, self.max_k), dtype=np.float32)
+        lat = np.empty(max_done, dtype=np.float64)
+        if with_stats:
+            st = np.empty((max_done, STATS_PER_QUERY), dtype=np.int32)
+            n = lib.gvs_server_poll_stats(self._h, max_done, _ptr(rid), _ptr(lab), _ptr(dist), _ptr(lat), _ptr(st))
+        else:
+            st = None
+            n = lib.gvs_server_poll(self._h, max_done, _ptr(rid), _ptr(lab), _ptr(dist), _ptr(lat))
+        if n < 0:
+            raise GvsError(n, lib.gvs_last_error().decode())
+        out = (rid[:n], lab[:n], dist[:n], lat[:n])
+        return out + (st[:n],) if with_stats else out
+
+    def cancel(self, req):
+        return lib.gvs_server_cancel(self._h, req) == GVS_OK
+
+    def stats(self):
+        c = np.zeros(6, dtype=np.int64)
+        _check(lib.gvs_server_stats(self._h, _ptr(c)))
+        return dict(zip(("submitted", "admitted", "completed", "cancelled", "rejected", "device_loops"), c.tolist()))
+
+    def loadgen(self, queries, rate_qps, ks, efs, seed=1):
+        q = np.ascontiguousarray(queries, dtype=np.float32)
+        nq = q.shape[0]
+        ks = np.ascontiguousarray(ks, dtype=np.int32)
+        efs = np.ascontiguousarray(efs, dtype=np.int32)
+        lab = np.empty((nq, self.max_k), dtype=np.int64)
+        dist = np.empty((nq, self.max_k), dtype=np.float32)
+        lat = np.empty(nq, dtype=np.float64)
+        wall = C.c_double(0.0)
+        _check(lib.gvs_server_loadgen(self._h, _ptr(q), nq, q.shape[1], float(rate_qps), seed, _ptr(ks), _ptr(efs),
+                                      _ptr(lab), _ptr(dist), _ptr(lat), C.byref(wall)))
+        return lab, dist, lat, wall.value
+
+    def close(self):
+        if self._h is not None and self._h.value:
+            _check(lib.gvs_server_close(self._h))
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Index:

@@ -204,11 +204,24 @@ GVS_API int64_t gvs_server_submit(gvs_server *s, const float *query, int32_t k, 
  * Returns the number copied (>= 0). */
 GVS_API int gvs_server_poll(gvs_server *s, int32_t max_done, int64_t *req_ids,
                             int64_t *labels, float *dists, double *latency_us);
+/* As gvs_server_poll, plus the request's GVS_STATS_PER_QUERY counters. */
+GVS_API int gvs_server_poll_stats(gvs_server *s, int32_t max_done, int64_t *req_ids, int64_t *labels,
+                                  float *dists, double *latency_us, int32_t *stats);
 /* Cancel a request that has not completed; returns GVS_OK or GVS_ERR_STATE. */
 GVS_API int gvs_server_cancel(gvs_server *s, int64_t req_id);
 /* counters: submitted, admitted, completed, cancelled, rejected, device_loops */
 GVS_API int gvs_server_stats(gvs_server *s, int64_t counters[6]);
 GVS_API int gvs_server_close(gvs_server *s);
+/* Open-loop load generator over an open server: query i (host, stride ldq)
+ * arrives at a seeded Poisson time of rate rate_qps with its own k = ks[i]
+ * and ef = efs[i]; it is submitted once it has arrived and a slot is free.
+ * Outputs per query i: labels/dists rows of max_k, latency_us[i] from
+ * arrival to completion; wall_s = time to drain all nq.  The reference's
+ * client measures per-batch latency the same way around its windows
+ * (networked_FPGA/CPU_programs/CPU_client.cpp:742-790). */
+GVS_API int gvs_server_loadgen(gvs_server *s, const float *queries, int64_t nq, int64_t ldq, double rate_qps,
+                               uint64_t seed, const int32_t *ks, const int32_t *efs, int64_t *labels,
+                               float *dists, double *latency_us, double *wall_s);
 
 #ifdef __cplusplus
 }

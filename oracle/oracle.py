@@ -144,3 +144,15 @@ def recall_at(found, truth, k):
     truth = np.asarray(truth)[:, :k]
     hit = sum(len(set(f.tolist()) & set(t.tolist())) for f, t in zip(found, truth))
     return hit / float(truth.shape[0] * k)
+
+
+def rerank_f32(og, queries, cand, k, order=WAVE):
+    """Exact fp32 rerank of candidate ids per query (oracle.c orc_rerank_f32)."""
+    q = pad_rows(queries, og.dim_pad)
+    cand = np.ascontiguousarray(cand, dtype=np.int32)
+    ids = np.empty((q.shape[0], k), dtype=np.int32)
+    ds = np.empty((q.shape[0], k), dtype=np.float32)
+    for i in range(q.shape[0]):
+        lib.orc_rerank_f32(_p(og.vec), og.dim_pad, og.g.metric, _p(q[i]), _p(cand[i]), cand.shape[1], k, order,
+                           _p(ids[i]), _p(ds[i]))
+    return ids, ds

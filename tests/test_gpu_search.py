@@ -181,3 +181,65 @@ def test_bruteforce_f64_mfma_matches_oracle(metric, k):
     for r, c in zip(*np.nonzero(mism)):
         assert abs(gd[r, c] - od[r, c]) <= 1e-9 * max(1.0, abs(od[r, c]))
     assert mism.mean() < 0.01
+
+
+@pytest.mark.parametrize("mc", [1, 4])
+def test_lds_visited_overflow_restarts_exactly(mc):
+    """A query that outgrows a small LDS visited hash restarts layer 0 with the
+    HBM table inside the kernel: same results and counters as the oracle."""
+    x, g, dev = graph(20000, 128, 16, 100)
+    og = orc.OracleGraph(g.arrays())
+    q = queries(200, 128, seed=15)
+    print(f"case lds overflow mc={mc}", flush=True)
+    check_equal(dev, og, q, 10, 64, mc=mc, visited="lds", visited_log2=10)
+    check_equal(dev, og, q, 10, 64, mc=mc, visited="hbm")
+    check_equal(dev, og, q, 10, 64, mc=mc, visited="lds")
+
+
+def test_async_search_has_no_overflow_at_default_sizes():
+    from gvs import hip as H
+    x, g, dev = graph(20000, 128, 16, 100)
+    q = queries(2000, 128, seed=16)
+    dq = H.DeviceArray.from_host(q)
+    dl = H.DeviceArray((2000, 10), np.int64)
+    dd = H.DeviceArray((2000, 10), np.float32)
+    for mc in (1, 2, 4):
+        dev.search_device(dq.address, 2000, 128, dl.address, dd.address, k=10, ef=64, mc=mc, asynchronous=True)
+        assert dev.overflow_count() == 0
+        lab_sync, _ = dev.search(q, k=10, ef=64, mc=mc)
+        np.testing.assert_array_equal(dl.to_host(), lab_sync)
+
+
+@pytest.mark.parametrize("storage", [gvs.BF16, gvs.FP8])
+def test_low_precision_traversal_and_fp32_rerank(storage):
+    """BF16 / FP8-e4m3 (per-vector scale) storage, fp32 accumulation: the
+    traversal equals the oracle on the same quantised rows; the exact fp32
+    rerank of the final ef candidates equals the oracle's rerank; distance
+    error and recall against the FP64 ground truth are reported."""
+    import lowprec
+    x = gvs.gen_mixture(20000, 128, 32, 41, 42)
+    g = gvs.HostGraph.build_hnsw(x, M=16, ef_construction=100, nthreads=16)
+    dev = gvs.DeviceIndex(g, storages=(gvs.F32, gvs.BF16, gvs.FP8))
+    a = g.arrays()
+    if storage == gvs.BF16:
+        og = orc.OracleGraph(a, dim_pad=128, storage=orc.BF16, bf16=lowprec.to_bf16(a["vectors"]))
+    else:
+        codes, scale = lowprec.quantize_fp8_rows(a["vectors"])
+        og = orc.OracleGraph(a, dim_pad=128, storage=orc.FP8, fp8=codes, scale=scale)
+    q = gvs.gen_mixture(200, 128, 32, 41, 77)  # same mixture as the base set
+    print(f"case low precision storage={storage}", flush=True)
+    check_equal(dev, og, q, 10, 64, storage=storage)
+    lab, dist = dev.search(q, k=10, ef=64, storage=storage, rerank=True)
+    cand, _, _ = og.search(q, 64, 64, order=orc.WAVE)
+    ri, rd = orc.rerank_f32(og, q, cand, 10)
+    np.testing.assert_array_equal(lab, ri)
+    np.testing.assert_array_equal(dist, rd)
+    gt, gd = orc.bruteforce_f64(x, q, 10)
+    lab32, d32 = dev.search(q, k=10, ef=64)
+    r32, rlp = orc.recall_at(lab32, gt, 10), orc.recall_at(lab, gt, 10)
+    lab_nr, d_nr = dev.search(q, k=10, ef=64, storage=storage)
+    rel = np.abs(d_nr[:, 0] - gd[:, 0]) / gd[:, 0]
+    print(f"storage={storage}: recall@10 fp32 {r32:.4f} low-precision+rerank {rlp:.4f} "
+          f"(no rerank {orc.recall_at(lab_nr, gt, 10):.4f}); top-1 traversal distance rel. error "
+          f"mean {rel.mean():.2e} max {rel.max():.2e}", flush=True)
+    assert rlp >= r32 - 0.02

@@ -263,3 +263,15 @@ def test_invalid_arguments_return_errors():
         gvs.HostGraph.load_hnswlib("/nonexistent/file.bin", 16)
     with pytest.raises(gvs.GvsError):
         gvs.HostGraph.build_hnsw(np.zeros((10, 4), np.float32), M=1)
+
+
+def test_vectorised_test_encoders_match_oracle():
+    import lowprec
+    rng = np.random.default_rng(9)
+    x = np.concatenate([rng.normal(0, 40, 20000), rng.normal(0, 0.003, 5000),
+                        [0.0, -0.0, 447.0, -447.9, 448.0, 1e-7]]).astype(np.float32)
+    ob = np.array([orc.lib.orc_f32_to_bf16(float(v)) for v in x], dtype=np.uint16)
+    np.testing.assert_array_equal(lowprec.to_bf16(x), ob)
+    xs = np.clip(x, -448, 448)
+    o8 = np.array([orc.lib.orc_f32_to_e4m3(float(v)) for v in xs], dtype=np.uint8)
+    np.testing.assert_array_equal(lowprec.to_e4m3(xs), o8)
