@@ -44,6 +44,66 @@ def recall_at(found, truth, k):
     return float((f[:, :, None] == t[:, None, :]).any(-1).sum()) / (t.shape[0] * k)
 
 
+def read_traffic(path, cfg):
+    """HBM bytes per search launch from a committed rocprofv3 PMC summary:
+    the line 'traffic ef=E mc=C mg=G persistent=P bytes=B' for this config."""
+    if not os.path.exists(path):
+        return None
+    key = f"ef={cfg['ef']} mc={cfg['mc']} mg={cfg.get('mg', 1)} persistent={int(bool(cfg['persistent']))}"
+    for line in open(path):
+        line = line.strip()
+        if line.startswith("traffic ") and key in line:
+            for tok in line.split():
+                if tok.startswith("bytes="):
+                    return int(tok[6:])
+    return None
+
+
+def serving_leg(a, cache, ef):
+    """Serving latency in a child process (bounded by a timeout, so that the
+    batch figures above are reported whatever happens to the serving loop)."""
+    import subprocess
+    cmd = [sys.executable, "-u", os.path.abspath(__file__), "--serve-child", "--graph-cache", cache,
+           "--n", str(a.n), "--dim", str(a.dim), "--centres", str(a.centres), "--nq", str(a.nq),
+           "--k", str(a.k), "--serve-ef", str(ef), "--build-threads", str(a.build_threads)]
+    log(f"[bench] serving leg: {' '.join(cmd[2:])}")
+    try:
+        r = subprocess.run(cmd, stdout=subprocess.PIPE, timeout=300)
+    except subprocess.TimeoutExpired:
+        log("[bench] serving leg timed out")
+        return None
+    if r.returncode != 0:
+        log(f"[bench] serving leg failed with status {r.returncode}")
+        return None
+    lines = [ln for ln in r.stdout.decode().splitlines() if ln.startswith("{")]
+    return json.loads(lines[-1]) if lines else None
+
+
+def serve_child(a, gvs, g, q):
+    """Open-loop Poisson load at fractions of the serving loop's saturated
+    throughput; latency runs from a request's arrival to the poll that sees
+    its result."""
+    dev = gvs.DeviceIndex(g)
+    nreq = min(len(q), 20000)
+    qq = q[:nreq]
+    ks = np.full(nreq, a.k, dtype=np.int32)
+    efs = np.full(nreq, a.serve_ef, dtype=np.int32)
+    srv = gvs.Server(dev, max_inflight=4096, max_k=a.k, max_ef=max(a.serve_ef, a.k))
+    _, _, lat, wall = srv.loadgen(qq, 1e9, ks, efs, seed=1)
+    peak = nreq / wall
+    log(f"[serve] saturated {peak:.0f} QPS, p50 {np.percentile(lat, 50):.0f} us")
+    loads = []
+    for frac in (0.25, 0.5, 0.75, 0.9):
+        _, _, lat, wall = srv.loadgen(qq, peak * frac, ks, efs, seed=2)
+        loads.append({"load": frac, "offered_qps": round(peak * frac, 1), "achieved_qps": round(nreq / wall, 1),
+                      "p50_us": round(float(np.percentile(lat, 50)), 1),
+                      "p99_us": round(float(np.percentile(lat, 99)), 1)})
+        log(f"[serve] load {frac:.2f}: p50 {loads[-1]['p50_us']} us p99 {loads[-1]['p99_us']} us")
+    srv.close()
+    print(json.dumps({"requests": nreq, "k": a.k, "ef": a.serve_ef, "peak_qps": round(peak, 1),
+                      "max_inflight": 4096, "loads": loads}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -65,6 +125,13 @@ def main():
                     help="fixed search config, e.g. ef=64,mc=2,persistent=1 (skips the sweep and the autotuner)")
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="also run the library's streaming copy kernel over this many MiB (counter calibration)")
+    ap.add_argument("--graph-cache", default="",
+                    help="hnswlib .bin file: loaded if present, else written after the build")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving-latency leg")
+    ap.add_argument("--serve-child", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--serve-ef", type=int, default=64, help=argparse.SUPPRESS)
+    ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "k_search_1Mx128_hbm_traffic.txt"),
+                    help="rocprofv3 PMC summary holding HBM bytes per launch of the search kernel")
     a = ap.parse_args()
 
     if a.gpus != 1:
@@ -82,9 +149,19 @@ def main():
     log(f"[bench] data {a.n}x{a.dim} + {a.nq} queries in {time.time() - t0:.1f}s; building HNSW "
         f"M={a.M} efC={a.efc} on {a.build_threads} threads")
     t0 = time.time()
-    g = gvs.HostGraph.build_hnsw(x, M=a.M, ef_construction=a.efc, nthreads=a.build_threads)
+    if a.graph_cache and os.path.exists(a.graph_cache):
+        g = gvs.HostGraph.load_hnswlib(a.graph_cache, a.dim)
+        log(f"[bench] graph loaded from {a.graph_cache}")
+    else:
+        g = gvs.HostGraph.build_hnsw(x, M=a.M, ef_construction=a.efc, nthreads=a.build_threads)
     build_s = time.time() - t0
     log(f"[bench] build {build_s:.1f}s")
+    if a.serve_child:
+        serve_child(a, gvs, g, q)
+        return
+    cache = a.graph_cache or f"/tmp/gvs_bench_graph_{os.getpid()}.bin"
+    if not os.path.exists(cache):
+        g.save_hnswlib(cache)
     dev = gvs.DeviceIndex(g)
     r0 = g.view.r0
     ru = g.view.ru
@@ -207,6 +284,16 @@ def main():
     log(f"[bench] tuned {qps:.0f} QPS, plain {plain_qps:.0f} QPS; hops {avg[0]:.1f} evals {avg[1]:.1f} "
         f"upper {avg[2]:.1f}/{avg[3]:.1f}; {achieved:.0f} GB/s algorithmic")
 
+    serving = None
+    if not a.no_serving:
+        serving = serving_leg(a, cache, plain_ef)
+    if not a.graph_cache and os.path.exists(cache):
+        os.remove(cache)
+
+    traffic_hbm = read_traffic(a.traffic_file, tuned_cfg)
+    if traffic_hbm is None:
+        log(f"[bench] no PMC traffic summary for {tuned_cfg} in {a.traffic_file}")
+
     cpu = None
     if not a.no_cpu:
         from oracle import oracle as orc
@@ -247,9 +334,10 @@ def main():
             "build_seconds": round(build_s, 1), "ef_sweep": sweep,
         },
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_hbm,
                      "algorithmic_bytes_per_launch": traffic, "kernel_ms": round(kernel_ms, 4)},
         "cpu_baseline": cpu,
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
 

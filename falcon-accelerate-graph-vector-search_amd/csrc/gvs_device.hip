@@ -19,7 +19,7 @@ using namespace gvsk;
 // launch sequence can be captured in a hipGraph)
 __global__ void k_reset_counters(uint32_t *c)
 {
-    if (threadIdx.x < 2) c[threadIdx.x] = 0u;
+    if (threadIdx.x < 4) c[threadIdx.x] = 0u;
 }
 
 // plain streaming copy (16 bytes per lane per iteration)
@@ -314,15 +314,22 @@ int plan_search(const gvs_index *ix, const gvs_params *p, int max_ef, int max_k,
     lp->lcap = p->mc * rpc * WAVE;
     lp->blog2 = p->bloom_log2 > 0 ? p->bloom_log2 : std::min(12, std::max(8, ilog2_ceil((int64_t)efe * 8)));
     int vmin = ilog2_ceil(4 * (int64_t)lp->lcap + 1024);
-    lp->vlog2 = p->visited_log2 > 0 ? std::max(p->visited_log2, vmin)
-                                    : std::max(vmin, std::min(20, ilog2_ceil((int64_t)efe * 128)));
+    // with GVS_FLAG_LDS_VISITED, visited_log2 in [10, 14] sizes the LDS hash
+    // and the HBM table (used once that hash is frozen) keeps its default
+    const bool lds_sized = (p->flags & GVS_FLAG_LDS_VISITED) && p->visited_log2 >= 10 && p->visited_log2 <= 14;
+    lp->vlog2 = p->visited_log2 > 0 && !lds_sized ? std::max(p->visited_log2, vmin)
+                                                   : std::max(vmin, std::min(20, ilog2_ceil((int64_t)efe * 128)));
     // exact visited set in LDS for small ef (no HBM table), unless disabled
     lp->hlog2 = 0;
-    if (!(p->flags & GVS_FLAG_HBM_VISITED) && ((p->flags & GVS_FLAG_LDS_VISITED) || efe <= 96)) {
+    // (2^12 slots hold the p99 query at ef 64-96 on degree-64 graphs; 2^13
+    // costs more in occupancy than the spills it saves)
+    if (!(p->flags & GVS_FLAG_HBM_VISITED) && ((p->flags & GVS_FLAG_LDS_VISITED) || efe <= 128)) {
         int want = ilog2_ceil(((int64_t)efe * 40 + lp->lcap) * 4 / 3);
-        lp->hlog2 = (p->flags & GVS_FLAG_LDS_VISITED) && p->visited_log2 >= 10 && p->visited_log2 <= 14
-                        ? p->visited_log2 : std::max(11, std::min(13, want));
-        lp->blog2 = 1;  // 16 bytes: keeps the LDS hash 16-byte aligned
+        lp->hlog2 = lds_sized ? p->visited_log2 : std::max(11, std::min(12, want));
+        // Bloom filter for the ids met after the hash is frozen; by default
+        // none (2 words keep the hash 16-byte aligned): those ids are then
+        // all gathered speculatively
+        lp->blog2 = p->bloom_log2 > 0 ? p->bloom_log2 : 1;
     }
     if (lp->blog2 > 13 || lp->vlog2 > 26) {
         set_error("gvs_search: bloom_log2 <= 13 and visited_log2 <= 26 required");
@@ -594,6 +601,20 @@ extern "C" int gvs_search_overflow_count(gvs_index *ix, int64_t *count)
         !hip_ok(hipMemcpy(&c, ix->d_counters + 1, 4, hipMemcpyDeviceToHost), "read overflow count"))
         return GVS_ERR_HIP;
     *count = c;
+    return GVS_OK;
+}
+
+extern "C" int gvs_search_counters(gvs_index *ix, int64_t counters[3])
+{
+    if (!ix || !counters) {
+        gvs::set_error("gvs_search_counters: invalid argument");
+        return GVS_ERR_ARG;
+    }
+    uint32_t c[4] = {0u, 0u, 0u, 0u};
+    if (!hip_ok(hipDeviceSynchronize(), "hipDeviceSynchronize") ||
+        !hip_ok(hipMemcpy(c, ix->d_counters, sizeof c, hipMemcpyDeviceToHost), "read counters"))
+        return GVS_ERR_HIP;
+    for (int i = 0; i < 3; i++) counters[i] = c[i];
     return GVS_OK;
 }
 

@@ -161,14 +161,22 @@ __device__ __forceinline__ void load_adj(const KParams &P, const uint32_t *sel, 
 #define EXP_PROF_ARGS
 #endif
 
+// `frozen` (may be null) is the LDS hash of a query that outgrew it: its ids
+// are visited and are looked up there first (read-only); the HBM table holds
+// the ids visited since.  With fewer than 64 Bloom words every id the frozen
+// hash does not hold is gathered speculatively.
 template <int NCH, int STORE, int METRIC, int MROWS>
 __device__ __forceinline__ int expand_group(const KParams &P, const WaveLds &L, const float *qf, uint32_t *T,
-                                            const int32_t (&ids)[MROWS], int nrows, int &evals EXP_PROF)
+                                            const int32_t (&ids)[MROWS], int nrows, int &evals,
+                                            const uint32_t *frozen EXP_PROF)
 {
     const int lane = lane_id();
     const uint32_t nbw = L.nbw;
+    const bool use_bloom = nbw >= 64u;
     const uint32_t tmask = (1u << P.vlog2) - 1u;
+    const uint32_t hmask = (1u << P.hlog2) - 1u;
     uint32_t prev[MROWS], pos[MROWS];
+    int32_t idv[MROWS];  // the row's ids, -1 where the frozen hash holds them
     int spos[MROWS];
     uint32_t spec_bits = 0u;
     int nl = 0;
@@ -179,14 +187,35 @@ __device__ __forceinline__ int expand_group(const KParams &P, const WaveLds &L, 
         spos[r] = 0;
         if (r < nrows) {
             const int32_t id = ids[r];
-            const bool valid = id >= 0;
-            bool maybe = true;
+            bool valid = id >= 0;
+            if (frozen) {
+                const uint32_t want = (uint32_t)id + 1u;
+                uint32_t hp = valid ? vhash((uint32_t)id, P.hlog2) : 0u;
+                bool probing = valid;
+                while (__ballot(probing)) {
+                    if (probing) {
+                        const uint32_t v = frozen[hp];
+                        if (v == want) {
+                            valid = false;
+                            probing = false;
+                        } else if (v == 0u) {
+                            probing = false;
+                        } else {
+                            hp = (hp + 1u) & hmask;
+                        }
+                    }
+                }
+            }
+            idv[r] = valid ? id : -1;
+            bool maybe = false;
             if (valid) {
-                uint32_t w;
-                uint64_t m;
-                bloom_pos((uint32_t)id, P.bloom_seed, nbw, w, m);
-                const uint64_t old = atomicOr((unsigned long long *)&L.bloom[w], (unsigned long long)m);
-                maybe = (old & m) == m;
+                if (use_bloom) {
+                    uint32_t w;
+                    uint64_t m;
+                    bloom_pos((uint32_t)id, P.bloom_seed, nbw, w, m);
+                    const uint64_t old = atomicOr((unsigned long long *)&L.bloom[w], (unsigned long long)m);
+                    maybe = (old & m) == m;
+                }
                 pos[r] = vhash((uint32_t)id, P.vlog2);
                 prev[r] = atomicCAS(&T[pos[r]], 0u, (uint32_t)id + 1u);
             }
@@ -209,7 +238,7 @@ __device__ __forceinline__ int expand_group(const KParams &P, const WaveLds &L, 
 #pragma unroll
     for (int r = 0; r < MROWS; r++) {
         if (r < nrows) {
-            const int32_t id = ids[r];
+            const int32_t id = idv[r];
             const bool valid = id >= 0;
             const uint32_t want = (uint32_t)id + 1u;
             bool pending = valid && prev[r] != 0u && prev[r] != want;
@@ -378,92 +407,88 @@ __device__ void search_query_req(const KParams &P, int64_t qi, int64_t slot, uns
     bool overflow = false;
     uint32_t *T = nullptr;
     const int rpc = (P.r0 + WAVE - 1) / WAVE;
-    // A query that outgrows the LDS visited hash starts layer 0 again with
-    // the HBM table, its Bloom filter taking the LDS hash's space.
+    // Visited set: the LDS hash while it stays below 3/4 full (small ef);
+    // a query that outgrows it freezes it and goes on with the HBM table for
+    // the ids it meets from then on (frozen hash first, then the table).
+    // Large ef starts with the HBM table and the LDS Bloom filter.
     bool lds_mode = P.hlog2 > 0;
-    for (int pass = 0; pass < 2; pass++) {
-        if (!lds_mode) {
-            T = P.vtab + ((size_t)slot << P.vlog2);
-            if (P.hlog2 > 0) {
-                L.bloom = (uint64_t *)L.vh;
-                L.nbw = 1u << (P.hlog2 - 1);
-            }
-        }
-        const int tlimit = lds_mode ? (int)((3u << P.hlog2) >> 2) - P.lcap : (int)((3u << P.vlog2) >> 2) - P.lcap;
-        pool = L.poolA;
-        pool2 = L.poolB;
-        np = 1;
-        hint = 0;
-        hops = 0;
-        evals = 1;
-        overflow = false;
+    const uint32_t *frozen = nullptr;
+    if (!lds_mode) T = P.vtab + ((size_t)slot << P.vlog2);
+    int tlimit = lds_mode ? (int)((3u << P.hlog2) >> 2) - P.lcap : (int)((3u << P.vlog2) >> 2) - P.lcap;
+    if (lds_mode) {
+        uint4 *H4 = (uint4 *)L.vh;
+        const uint4 z4 = make_uint4(0u, 0u, 0u, 0u);
+        for (uint32_t i = lane; i < ((1u << P.hlog2) >> 2); i += WAVE) H4[i] = z4;
+    } else {
+        for (uint32_t w = lane; w < L.nbw; w += WAVE) L.bloom[w] = 0ull;
+    }
+    wave_sync();
+    if (lane == 0) {
+        pool[0] = mk_key(epd, ep);
         if (lds_mode) {
-            uint4 *H4 = (uint4 *)L.vh;
-            const uint4 z4 = make_uint4(0u, 0u, 0u, 0u);
-            for (uint32_t i = lane; i < ((1u << P.hlog2) >> 2); i += WAVE) H4[i] = z4;
+            L.vh[vhash(ep, P.hlog2)] = ep + 1u;
         } else {
-            for (uint32_t w = lane; w < L.nbw; w += WAVE) L.bloom[w] = 0ull;
+            uint32_t w;
+            uint64_t m;
+            bloom_pos(ep, P.bloom_seed, L.nbw, w, m);
+            L.bloom[w] |= m;
+            atomicCAS(&T[vhash(ep, P.vlog2)], 0u, ep + 1u);
         }
-        wave_sync();
-        if (lane == 0) {
-            pool[0] = mk_key(epd, ep);
-            if (lds_mode) {
-                L.vh[vhash(ep, P.hlog2)] = ep + 1u;
-            } else {
-                uint32_t w;
-                uint64_t m;
-                bloom_pos(ep, P.bloom_seed, L.nbw, w, m);
-                L.bloom[w] |= m;
-                atomicCAS(&T[vhash(ep, P.vlog2)], 0u, ep + 1u);
-            }
-        }
-        wave_sync();
+    }
+    wave_sync();
 
-        int nres = 0;
-        bool haveQ = false;
-        uint32_t *selP = L.selA, *selN = L.selB;
-        int32_t idsP[MROWS], idsN[MROWS];
-        int nselP = 0, nselN = 0;
-        for (;;) {
-            if (haveQ) {  // complete the oldest group
-                np = merge_into_pool(pool, pool2, np, ef, L.lkeys, nres, hint);
-                haveQ = false;
-            }
-            PROF_MARK(6);
-            // refill the pipeline up to mg groups
-            if (nselP == 0) {
-                nselP = select_group(pool, np, P.mc, hint, selP);
-                if (nselP) load_adj<MROWS>(P, selP, nselP, rpc, idsP);
-                hops += nselP;
-            }
-            if (P.mg > 1 && nselP > 0 && nselN == 0) {
-                nselN = select_group(pool, np, P.mc, hint, selN);
-                if (nselN) load_adj<MROWS>(P, selN, nselN, rpc, idsN);
-                hops += nselN;
-            }
-            if (nselP == 0) break;
-            PROF_MARK(1);
-            if (lds_mode)
-                nres = expand_lds<NCH, STORE, METRIC, MROWS>(P, L, qf, idsP, nselP * rpc, evals EXP_PROF_ARGS);
-            else
-                nres = expand_group<NCH, STORE, METRIC, MROWS>(P, L, qf, T, idsP, nselP * rpc, evals EXP_PROF_ARGS);
-            haveQ = true;
-            // the next group moves up
-            uint32_t *ts = selP;
-            selP = selN;
-            selN = ts;
-            nselP = nselN;
-            nselN = 0;
+    int nres = 0;
+    bool haveQ = false;
+    uint32_t *selP = L.selA, *selN = L.selB;
+    int32_t idsP[MROWS], idsN[MROWS];
+    int nselP = 0, nselN = 0;
+    for (;;) {
+        if (haveQ) {  // complete the oldest group
+            np = merge_into_pool(pool, pool2, np, ef, L.lkeys, nres, hint);
+            haveQ = false;
+        }
+        PROF_MARK(6);
+        // refill the pipeline up to mg groups
+        if (nselP == 0) {
+            nselP = select_group(pool, np, P.mc, hint, selP);
+            if (nselP) load_adj<MROWS>(P, selP, nselP, rpc, idsP);
+            hops += nselP;
+        }
+        if (P.mg > 1 && nselP > 0 && nselN == 0) {
+            nselN = select_group(pool, np, P.mc, hint, selN);
+            if (nselN) load_adj<MROWS>(P, selN, nselN, rpc, idsN);
+            hops += nselN;
+        }
+        if (nselP == 0) break;
+        PROF_MARK(1);
+        if (lds_mode)
+            nres = expand_lds<NCH, STORE, METRIC, MROWS>(P, L, qf, idsP, nselP * rpc, evals EXP_PROF_ARGS);
+        else
+            nres = expand_group<NCH, STORE, METRIC, MROWS>(P, L, qf, T, idsP, nselP * rpc, evals,
+                                                           frozen EXP_PROF_ARGS);
+        haveQ = true;
+        // the next group moves up
+        uint32_t *ts = selP;
+        selP = selN;
+        selN = ts;
+        nselP = nselN;
+        nselN = 0;
 #pragma unroll
-            for (int r = 0; r < MROWS; r++) idsP[r] = idsN[r];
-            if (evals > tlimit) {
+        for (int r = 0; r < MROWS; r++) idsP[r] = idsN[r];
+        if (evals > tlimit) {
+            if (lds_mode && P.vtab) {
+                lds_mode = false;
+                frozen = L.vh;
+                T = P.vtab + ((size_t)slot << P.vlog2);
+                tlimit = evals + (int)((3u << P.vlog2) >> 2) - P.lcap;
+                for (uint32_t w = lane; w < L.nbw; w += WAVE) L.bloom[w] = 0ull;
+                if (lane == 0) atomicAdd(&P.counters[2], 1u);
+                wave_sync();
+            } else {
                 overflow = true;
                 break;
             }
         }
-        if (!(overflow && lds_mode)) break;
-        lds_mode = false;
-        wave_sync();
     }
 
     // Low-precision traversal: rerank the final pool with exact fp32

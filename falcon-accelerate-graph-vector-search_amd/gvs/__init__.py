@@ -76,6 +76,7 @@ def _load():
         "gvs_search_device": (C.c_int, [P, P, I64, I64, P, P, P, P, P, P]),
         "gvs_search_host": (C.c_int, [P, P, I64, I64, P, P, P, P, P]),
         "gvs_search_overflow_count": (C.c_int, [P, P]),
+        "gvs_search_counters": (C.c_int, [P, P]),
         "gvs_device_copy": (C.c_int, [P, P, I64, P]),
         "gvs_bruteforce_f64_device": (C.c_int, [P, P, I64, I64, I32, P, P, P]),
         "gvs_server_open": (C.c_int, [P, P, P]),
@@ -104,7 +105,7 @@ def exported_symbols():
     names = ["gvs_last_error", "gvs_version", "gvs_hnsw_build", "gvs_nsg_from_hnsw", "gvs_hnswlib_load",
              "gvs_hnswlib_save", "gvs_nsg_load", "gvs_host_graph_view", "gvs_host_graph_free",
              "gvs_gen_mixture", "gvs_index_create", "gvs_index_destroy", "gvs_index_device_bytes",
-             "gvs_search_device", "gvs_search_host", "gvs_search_overflow_count", "gvs_device_copy", "gvs_bruteforce_f64_device", "gvs_server_open",
+             "gvs_search_device", "gvs_search_host", "gvs_search_overflow_count", "gvs_search_counters", "gvs_device_copy", "gvs_bruteforce_f64_device", "gvs_server_open",
              "gvs_server_submit", "gvs_server_poll", "gvs_server_poll_stats", "gvs_server_cancel",
              "gvs_server_stats", "gvs_server_close", "gvs_server_loadgen"]
     return [n for n in names if hasattr(lib, n)]
@@ -260,6 +261,12 @@ class DeviceIndex:
         c = C.c_int64(0)
         _check(lib.gvs_search_overflow_count(self._h, C.byref(c)))
         return c.value
+
+    def counters(self):
+        """Device counters of the last search: claimed, overflowed, LDS-hash spills."""
+        c = (C.c_int64 * 3)()
+        _check(lib.gvs_search_counters(self._h, c))
+        return {"claimed": c[0], "overflowed": c[1], "spilled": c[2]}
 
     def bruteforce_f64_device(self, q_ptr, nq, ldq, k, ids_ptr, dists_ptr, stream=None):
         _check(lib.gvs_bruteforce_f64_device(self._h, C.c_void_p(q_ptr), nq, ldq, k, C.c_void_p(ids_ptr),

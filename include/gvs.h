@@ -83,10 +83,12 @@ typedef struct gvs_params {
  * reads the count with gvs_search_overflow_count() after synchronising and
  * re-issues them (the synchronous form retries them itself). */
 #define GVS_FLAG_ASYNC 1
-/* Visited-set placement.  By default a query with ef_eff <= 96 keeps an exact
- * visited hash in LDS; larger ef use an LDS Bloom filter in front of an exact
- * per-query hash in HBM.  GVS_FLAG_HBM_VISITED forces the HBM form,
- * GVS_FLAG_LDS_VISITED the LDS form (visited_log2 in [10, 14] sets its size). */
+/* Visited-set placement.  By default a query with ef_eff <= 128 keeps an exact
+ * visited hash in LDS; a query that fills it to 3/4 freezes it and goes on
+ * with an exact per-query hash in HBM for the ids it meets later.  Larger ef
+ * use an LDS Bloom filter in front of the HBM hash from the start.
+ * GVS_FLAG_HBM_VISITED forces the HBM form, GVS_FLAG_LDS_VISITED the LDS form
+ * (visited_log2 in [10, 14] then sets the LDS hash size). */
 #define GVS_FLAG_HBM_VISITED 2
 #define GVS_FLAG_LDS_VISITED 4
 
@@ -160,6 +162,11 @@ GVS_API int gvs_search_device(gvs_index *ix, const float *queries, int64_t nq, i
 /* Number of queries of the last search on this index whose visited table
  * overflowed (only non-zero after a GVS_FLAG_ASYNC search; synchronises). */
 GVS_API int gvs_search_overflow_count(gvs_index *ix, int64_t *count);
+/* Device counters of the last search on this index (synchronises):
+ * [0] queries claimed from the persistent work queue, [1] queries whose
+ * visited table overflowed, [2] queries that outgrew the LDS visited hash and
+ * went on with the HBM table.  No reference counterpart (diagnostics). */
+GVS_API int gvs_search_counters(gvs_index *ix, int64_t counters[3]);
 /* Device-to-device copy through a plain streaming kernel (16-byte aligned
  * pointers and size).  No reference counterpart; used to calibrate HBM byte
  * counters against a kernel of known traffic. */

@@ -184,14 +184,19 @@ def test_bruteforce_f64_mfma_matches_oracle(metric, k):
 
 
 @pytest.mark.parametrize("mc", [1, 4])
-def test_lds_visited_overflow_restarts_exactly(mc):
-    """A query that outgrows a small LDS visited hash restarts layer 0 with the
-    HBM table inside the kernel: same results and counters as the oracle."""
+def test_lds_visited_spill_is_exact(mc):
+    """A query that outgrows a small LDS visited hash freezes it and goes on
+    with the HBM table (with and without a Bloom filter in front of it):
+    same results and counters as the oracle."""
     x, g, dev = graph(20000, 128, 16, 100)
     og = orc.OracleGraph(g.arrays())
     q = queries(200, 128, seed=15)
-    print(f"case lds overflow mc={mc}", flush=True)
+    print(f"case lds spill mc={mc}", flush=True)
     check_equal(dev, og, q, 10, 64, mc=mc, visited="lds", visited_log2=10)
+    dev.search(q, k=10, ef=64, mc=mc, visited="lds", visited_log2=10)
+    c = dev.counters()
+    assert c["spilled"] > 0 and c["overflowed"] == 0
+    check_equal(dev, og, q, 10, 64, mc=mc, visited="lds", visited_log2=10, bloom_log2=8)
     check_equal(dev, og, q, 10, 64, mc=mc, visited="hbm")
     check_equal(dev, og, q, 10, 64, mc=mc, visited="lds")
 
