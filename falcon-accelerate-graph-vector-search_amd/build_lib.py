@@ -5,7 +5,8 @@
 Host-only sources are compiled with g++ (-ffp-contract=off keeps the HNSW
 builder's scalar distance order identical to hnswlib's), device sources with
 hipcc for gfx950 only.  Objects go to _build/ (not committed); the shared
-library to lib/libgvs.so next to this file.
+library to lib/libgvs.so next to this file, with the C++ host program
+lib/gvs_search (cli/gvs_search.cpp) linked against it.
 """
 import argparse
 import concurrent.futures as cf
@@ -71,6 +72,13 @@ def build(jobs=8, verbose=False, extra_hip_flags=(), out=None, build_dir=None):
     tmp = lib + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-lpthread"])
     os.replace(tmp, lib)
+    if out is None:
+        # the C++ host program over the C ABI, next to the library
+        cli = os.path.join(LIBDIR, "gvs_search")
+        _run(["g++", "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"),
+              os.path.join(PKG, "cli", "gvs_search.cpp"), "-o", cli + ".tmp", "-L" + LIBDIR, "-lgvs",
+              "-Wl,-rpath,$ORIGIN"])
+        os.replace(cli + ".tmp", cli)
     return lib
 
 
