@@ -79,6 +79,59 @@ def serving_leg(a, cache, ef):
     return json.loads(lines[-1]) if lines else None
 
 
+def intra_latency(a, gvs, g, q):
+    """Batch=1 latency: one query per launch through a captured graph, for
+    waves per query x candidates per step x ef; p50/p99 over 190 queries
+    (10 warm-up), mean evaluations and recall@10 against FP64 ground truth."""
+    from gvs import hip as H
+    dev = gvs.DeviceIndex(g)
+    st = H.Stream()
+    nq = 200
+    dq = H.DeviceArray((1, a.dim), np.float32)
+    dl = H.DeviceArray((1, a.k), np.int64)
+    dd = H.DeviceArray((1, a.k), np.float32)
+    ds = H.DeviceArray((1, 4), np.int32)
+    dqa = H.DeviceArray.from_host(q[:nq])
+    gti = H.DeviceArray((nq, a.k), np.int32)
+    gtd = H.DeviceArray((nq, a.k), np.float64)
+    dev.bruteforce_f64_device(dqa.address, nq, a.dim, a.k, gti.address, gtd.address, st.handle)
+    st.synchronize()
+    gt = gti.to_host()
+    rows = []
+    for ef in (64, 128, 256, 512):
+        base = None
+        for W in (1, 2, 4, 8):
+            for mc in (1, 2, 4, 8):
+                if W == 8 and mc == 1:
+                    continue
+                sg = dev.capture(dq.address, 1, a.dim, dl.address, dd.address, k=a.k, ef=ef, mc=mc,
+                                 waves_per_query=W, stats_ptr=ds.address)
+                lat, ev, hits = [], [], 0
+                for i in range(nq):
+                    dq.upload(q[i:i + 1])
+                    t0 = time.perf_counter()
+                    sg.launch(st.handle)
+                    st.synchronize()
+                    lat.append(time.perf_counter() - t0)
+                    ev.append(int(ds.to_host()[0, 1]))
+                    hits += len(set(dl.to_host()[0].tolist()) & set(gt[i].tolist()))
+                sg.close()
+                lat = np.array(lat[10:]) * 1e6
+                r = {"ef": ef, "waves": W, "mc": mc, "p50_us": round(float(np.percentile(lat, 50)), 1),
+                     "p99_us": round(float(np.percentile(lat, 99)), 1), "evals": round(float(np.mean(ev)), 1),
+                     "recall_at_10": round(hits / (nq * a.k), 4)}
+                if W == 1 and mc == 1:
+                    base = r
+                r["p50_speedup_vs_plain"] = round(base["p50_us"] / r["p50_us"], 2)
+                rows.append(r)
+                log(f"[intra] {r}")
+    best = {ef: max((r for r in rows if r["ef"] == ef), key=lambda r: r["p50_speedup_vs_plain"]) for ef in (64, 128, 256, 512)}
+    print(json.dumps({"metric": "batch=1 search latency, intra-query DST vs plain", "unit": "us",
+                      "config": {"workload": f"HNSW search, {a.n}x{a.dim} fp32 L2, M={a.M}, efC={a.efc}, "
+                                             f"one query per launch (captured graph), k={a.k}"},
+                      "best_per_ef": best, "rows": rows}), flush=True)
+
+
 def serve_child(a, gvs, g, q):
     """Open-loop Poisson load at fractions of the serving loop's saturated
     throughput; latency runs from a request's arrival to the poll that sees
@@ -128,6 +181,9 @@ def main():
     ap.add_argument("--graph-cache", default="",
                     help="hnswlib .bin file: loaded if present, else written after the build")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-latency leg")
+    ap.add_argument("--intra", action="store_true",
+                    help="instead of the throughput run: batch=1 latency of intra-query DST "
+                         "(1-8 waves per query, mc 1-8, ef 64-512) against the plain search")
     ap.add_argument("--serve-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--serve-ef", type=int, default=64, help=argparse.SUPPRESS)
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "k_search_1Mx128_hbm_traffic.txt"),
@@ -158,6 +214,9 @@ def main():
     log(f"[bench] build {build_s:.1f}s")
     if a.serve_child:
         serve_child(a, gvs, g, q)
+        return
+    if a.intra:
+        intra_latency(a, gvs, g, q)
         return
     cache = a.graph_cache or f"/tmp/gvs_bench_graph_{os.getpid()}.bin"
     if not os.path.exists(cache):
@@ -284,6 +343,31 @@ def main():
     log(f"[bench] tuned {qps:.0f} QPS, plain {plain_qps:.0f} QPS; hops {avg[0]:.1f} evals {avg[1]:.1f} "
         f"upper {avg[2]:.1f}/{avg[3]:.1f}; {achieved:.0f} GB/s algorithmic")
 
+    # small batches: latency of one eager search call vs one replay of the
+    # captured launch sequence (hipGraph), plain config
+    small = []
+    for nb in (1, 64):
+        cfg_s = dict(plain_cfg)
+        reps = 200
+
+        def eager():
+            dev.search_device(dq.address, nb, a.dim, dlab.address, ddist.address, k=a.k, stream=stream.handle,
+                              asynchronous=True, **cfg_s)
+        sg = dev.capture(dq.address, nb, a.dim, dlab.address, ddist.address, k=a.k, **cfg_s)
+        res = {"batch": nb}
+        for name, fn in (("eager_us", eager), ("graph_us", lambda: sg.launch(stream.handle))):
+            for _ in range(10):
+                fn()
+            stream.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                fn()
+                stream.synchronize()
+            res[name] = round((time.perf_counter() - t0) / reps * 1e6, 1)
+        sg.close()
+        small.append(res)
+        log(f"[bench] batch {nb}: eager {res['eager_us']} us, graph replay {res['graph_us']} us per call")
+
     serving = None
     if not a.no_serving:
         serving = serving_leg(a, cache, plain_ef)
@@ -331,7 +415,7 @@ def main():
             "recall_at_10": round(tuned_rec, 4), "plain_ef": plain_ef, "plain_recall_at_10": round(plain_rec, 4),
             "plain_qps": round(plain_qps, 1), "speedup_vs_plain": round(qps / plain_qps, 3),
             "avg_hops": round(float(avg[0]), 2), "avg_dist_evals": round(float(avg[1]), 2),
-            "build_seconds": round(build_s, 1), "ef_sweep": sweep,
+            "build_seconds": round(build_s, 1), "ef_sweep": sweep, "small_batch_latency": small,
         },
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_hbm,

@@ -16,6 +16,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <iomanip>
 #include <iostream>
 #include <string>
 #include <vector>
@@ -95,6 +96,7 @@ int main(int argc, char **argv)
         const auto t0 = std::chrono::steady_clock::now();
         index.searchKnnBatch(q.data(), nq, (size_t)k, ids.data(), dists.data(), stats.data());
         const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        std::cout << std::setprecision(10);
         std::cout << "queries=" << nq << " k=" << k << " ef=" << ef << " mc=" << mc << " mg=" << mg << "\n";
         std::cout << "time_ms=" << s * 1e3 << " QPS(host buffers)=" << (double)nq / s << "\n";
         double hops = 0, evals = 0;

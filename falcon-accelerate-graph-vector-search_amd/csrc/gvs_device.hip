@@ -287,7 +287,8 @@ int plan_search(const gvs_index *ix, const gvs_params *p, int max_ef, int max_k,
         return GVS_ERR_ARG;
     }
     if (max_k < 1 || max_k > 4096 || max_ef < 1 || max_ef > 4096 || p->mc < 1 || p->mc > 8 || p->mg < 1 ||
-        p->mg > 2 || p->waves_per_query != 1) {
+        p->mg > 2 || (p->waves_per_query != 1 && p->waves_per_query != 2 && p->waves_per_query != 4 &&
+                      p->waves_per_query != 8)) {
         set_error("gvs_search: unsupported k=%d ef=%d mc=%d mg=%d waves_per_query=%d", max_k, max_ef, p->mc,
                   p->mg, p->waves_per_query);
         return GVS_ERR_UNSUPPORTED;
@@ -303,7 +304,20 @@ int plan_search(const gvs_index *ix, const gvs_params *p, int max_ef, int max_k,
     }
     const int E = p->storage == GVS_STORE_F32 ? 4 : p->storage == GVS_STORE_BF16 ? 8 : 16;
     const int nch = ix->dim_pad / (8 * E);
-    kfn_t fn = (ix->dim_pad % (8 * E) == 0) ? pick_kernel(p->storage, ix->metric, nch, p->mc * rpc) : nullptr;
+    const int W = p->waves_per_query;
+    int wrows = 0;  // adjacency rows per wave when several waves share a query
+    if (W > 1) {
+        if (p->storage != GVS_STORE_F32 || p->mg != 1) {
+            set_error("gvs_search: waves_per_query > 1 needs fp32 storage and mg = 1");
+            return GVS_ERR_UNSUPPORTED;
+        }
+        const int rpw = (p->mc * rpc + W - 1) / W;
+        wrows = rpw <= 1 ? 1 : rpw <= 2 ? 2 : 4;
+    }
+    kfn_t fn = nullptr;
+    if (ix->dim_pad % (8 * E) == 0)
+        fn = W > 1 ? (ix->metric == GVS_METRIC_L2 ? gvs_kernel_mw_f32_l2(nch, wrows) : gvs_kernel_mw_f32_ip(nch, wrows))
+                   : pick_kernel(p->storage, ix->metric, nch, p->mc * rpc);
     if (!fn) {
         set_error("gvs_search: dim %d (padded %d) not compiled for storage %d", ix->dim, ix->dim_pad, p->storage);
         return GVS_ERR_UNSUPPORTED;
@@ -334,6 +348,25 @@ int plan_search(const gvs_index *ix, const gvs_params *p, int max_ef, int max_k,
     if (lp->blog2 > 13 || lp->vlog2 > 26) {
         set_error("gvs_search: bloom_log2 <= 13 and visited_log2 <= 26 required");
         return GVS_ERR_ARG;
+    }
+    if (W > 1) {
+        // one workgroup per query: shared pool, Bloom words, selection and
+        // control words, visited hash; per wave wrows*64 keys and ids
+        int bytes = 16 * lp->ef_cap + 8 * (1 << lp->blog2) + W * wrows * WAVE * 12 + 64 +
+                    (lp->hlog2 > 0 ? 4 * (1 << lp->hlog2) : 0);
+        lp->lds_per_wave = (bytes + 15) / 16 * 16;  // per workgroup
+        lp->wpb = W;
+        if (lp->lds_per_wave > 160 * 1024) {
+            set_error("gvs_search: per-query LDS %d bytes exceeds 160 KiB (ef too large)", lp->lds_per_wave);
+            return GVS_ERR_UNSUPPORTED;
+        }
+        lp->resident_waves = ix->cu_count * W;
+        if (lp->lds_per_wave > 65536 &&
+            !hip_ok(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        lp->lds_per_wave),
+                    "hipFuncSetAttribute"))
+            return GVS_ERR_HIP;
+        return GVS_OK;
     }
     int bytes = 16 * lp->ef_cap + 8 * lp->lcap + 8 * (1 << lp->blog2) + 4 * lp->lcap + 64 +
                 (lp->hlog2 > 0 ? 4 * (1 << lp->hlog2) : 0);
@@ -377,7 +410,6 @@ int ensure_scratch(gvs_index *ix, int64_t slots, int vlog2, int64_t nstatus)
     }
     if (nstatus > ix->status_cap) {
         (void)hipFree(ix->d_status);
-    (void)hipFree(ix->d_xnorm64);
         ix->d_status = nullptr;
         ix->status_cap = 0;
         if (!hip_ok(hipMalloc(&ix->d_status, sizeof(int32_t) * nstatus), "hipMalloc status")) return GVS_ERR_NOMEM;
@@ -420,35 +452,58 @@ void fill_params(const gvs_index *ix, const gvs_params *p, const LaunchPlan &lp,
     P.status = ix->d_status;
 }
 
-int launch_search(gvs_index *ix, const gvs_params *p, const LaunchPlan &lp, KParams P, int64_t nq,
-                  hipStream_t stream)
+// Waves that need a visited table for a search of nq queries: the resident
+// grid in persistent mode, else the chunk size of the one-wave-per-query
+// launches.
+int64_t scratch_slots(const gvs_params *p, const LaunchPlan &lp, int64_t nq)
 {
-    kfn_t fn = (kfn_t)lp.fn;
-    const size_t shm = (size_t)lp.wpb * lp.lds_per_wave;
+    if (p->waves_per_query > 1)  // one workgroup (one table) per query, in chunks
+        return std::min<int64_t>(std::min<int64_t>(nq, 65536), std::max<int64_t>(1, ((int64_t)1 << 30) >> lp.vlog2));
     if (p->persistent) {
         int64_t gw = p->grid_waves > 0 ? p->grid_waves : lp.resident_waves;
         gw = std::max<int64_t>(lp.wpb, std::min<int64_t>(gw, (nq + lp.wpb - 1) / lp.wpb * lp.wpb));
-        gw = (gw + lp.wpb - 1) / lp.wpb * lp.wpb;
-        int rc = ensure_scratch(ix, gw, lp.vlog2, nq);
-        if (rc) return rc;
-        P.vtab = ix->d_vtab;
-        P.status = ix->d_status;
+        return (gw + lp.wpb - 1) / lp.wpb * lp.wpb;
+    }
+    int64_t slots = std::min<int64_t>(nq, std::max<int64_t>(lp.wpb, ((int64_t)1 << 30) >> lp.vlog2));
+    slots = std::min<int64_t>(slots, 65536);
+    return (slots + lp.wpb - 1) / lp.wpb * lp.wpb;
+}
+
+// Enqueue the search on `stream`.  vtab holds scratch_slots() tables of
+// 2^vlog2 words (all zero; every query leaves its table zero), status nq
+// words.  Enqueues kernels only, so it can be captured into a graph.
+int launch_search(gvs_index *ix, const gvs_params *p, const LaunchPlan &lp, KParams P, int64_t nq,
+                  hipStream_t stream, uint32_t *vtab, int32_t *status)
+{
+    kfn_t fn = (kfn_t)lp.fn;
+    // (with several waves per query, lds_per_wave is the whole workgroup's)
+    const size_t shm = p->waves_per_query > 1 ? (size_t)lp.lds_per_wave : (size_t)lp.wpb * lp.lds_per_wave;
+    const int64_t slots = scratch_slots(p, lp, nq);
+    P.vtab = vtab;
+    P.status = status;
+    if (p->waves_per_query > 1) {
+        // one workgroup of waves_per_query waves per query, chunks of `slots`
+        P.persistent = 0;
+        hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, stream, ix->d_counters);
+        for (int64_t b = 0; b < nq; b += slots) {
+            const int64_t cnt = std::min(slots, nq - b);
+            P.q_base = b;
+            P.nq = b + cnt;
+            hipLaunchKernelGGL(fn, dim3((unsigned)cnt), dim3(lp.wpb * WAVE), shm, stream, P);
+            if (!hip_ok(hipGetLastError(), "launch k_search_mw")) return GVS_ERR_HIP;
+        }
+        return GVS_OK;
+    }
+    if (p->persistent) {
         P.persistent = 1;
         P.nq = nq;
         P.q_base = 0;
         hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, stream, ix->d_counters);
-        hipLaunchKernelGGL(fn, dim3((unsigned)(gw / lp.wpb)), dim3(lp.wpb * WAVE), shm, stream, P);
+        hipLaunchKernelGGL(fn, dim3((unsigned)(slots / lp.wpb)), dim3(lp.wpb * WAVE), shm, stream, P);
         if (!hip_ok(hipGetLastError(), "launch k_search (persistent)")) return GVS_ERR_HIP;
         return GVS_OK;
     }
     // one wave per query; queries processed in chunks of at most `slots`
-    int64_t slots = std::min<int64_t>(nq, std::max<int64_t>(lp.wpb, ((int64_t)1 << 30) >> lp.vlog2));
-    slots = std::min<int64_t>(slots, 65536);
-    slots = (slots + lp.wpb - 1) / lp.wpb * lp.wpb;
-    int rc = ensure_scratch(ix, slots, lp.vlog2, nq);
-    if (rc) return rc;
-    P.vtab = ix->d_vtab;
-    P.status = ix->d_status;
     P.persistent = 0;
     hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, stream, ix->d_counters);
     for (int64_t b = 0; b < nq; b += slots) {
@@ -459,6 +514,15 @@ int launch_search(gvs_index *ix, const gvs_params *p, const LaunchPlan &lp, KPar
         if (!hip_ok(hipGetLastError(), "launch k_search")) return GVS_ERR_HIP;
     }
     return GVS_OK;
+}
+
+// launch_search on the index's own scratch, grown as needed
+int launch_search_ix(gvs_index *ix, const gvs_params *p, const LaunchPlan &lp, const KParams &P, int64_t nq,
+                     hipStream_t stream)
+{
+    int rc = ensure_scratch(ix, scratch_slots(p, lp, nq), lp.vlog2, nq);
+    if (rc) return rc;
+    return launch_search(ix, p, lp, P, nq, stream, ix->d_vtab, ix->d_status);
 }
 
 }  // namespace gvs
@@ -488,7 +552,7 @@ static int search_impl(gvs_index *ix, const float *queries, int64_t nq, int64_t 
     P.out_labels = labels;
     P.out_dists = dists;
     P.out_stats = stats;
-    rc = gvs::launch_search(ix, p, lp, P, nq, stream);
+    rc = gvs::launch_search_ix(ix, p, lp, P, nq, stream);
     if (rc) return rc;
     if (p->flags & GVS_FLAG_ASYNC) return GVS_OK;
     // visited-table overflow: rerun the affected queries with 4x larger tables
@@ -544,7 +608,7 @@ static int search_impl(gvs_index *ix, const float *queries, int64_t nq, int64_t 
             P2.out_labels = t_lab;
             P2.out_dists = t_d;
             P2.out_stats = t_st;
-            ok = gvs::launch_search(ix, &p2, lp2, P2, m, stream) == GVS_OK &&
+            ok = gvs::launch_search_ix(ix, &p2, lp2, P2, m, stream) == GVS_OK &&
                  hip_ok(hipStreamSynchronize(stream), "sync retry");
             for (int64_t i = 0; ok && i < m; i++) {
                 int64_t q = idx[(size_t)i];
@@ -615,6 +679,86 @@ extern "C" int gvs_search_counters(gvs_index *ix, int64_t counters[3])
         !hip_ok(hipMemcpy(c, ix->d_counters, sizeof c, hipMemcpyDeviceToHost), "read counters"))
         return GVS_ERR_HIP;
     for (int i = 0; i < 3; i++) counters[i] = c[i];
+    return GVS_OK;
+}
+
+// ---- captured launches --------------------------------------------------
+struct gvs_search_graph {
+    gvs_index *ix = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    uint32_t *vtab = nullptr;   // the graph's own visited tables (pointers are baked into the graph)
+    int32_t *status = nullptr;
+};
+
+static void free_search_graph(gvs_search_graph *g)
+{
+    if (!g) return;
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    (void)hipFree(g->vtab);
+    (void)hipFree(g->status);
+    delete g;
+}
+
+extern "C" int gvs_search_graph_create(gvs_index *ix, const float *queries, int64_t nq, int64_t ldq,
+                                       const int32_t *entries, const gvs_params *p, int64_t *labels,
+                                       float *dists, int32_t *stats, gvs_search_graph **out)
+{
+    if (!ix || !p || !out || nq <= 0 || nq > INT32_MAX || !queries || !labels || !dists || ldq < ix->dim) {
+        gvs::set_error("gvs_search_graph_create: invalid argument");
+        return GVS_ERR_ARG;
+    }
+    gvs::LaunchPlan lp;
+    int rc = gvs::plan_search(ix, p, p->ef, p->k, &lp);
+    if (rc) return rc;
+    KParams P;
+    gvs::fill_params(ix, p, lp, P);
+    P.queries = queries;
+    P.ldq = ldq;
+    P.entries = entries;
+    P.k = p->k;
+    P.ef = std::max(p->ef, p->k);
+    P.out_labels = labels;
+    P.out_dists = dists;
+    P.out_stats = stats;
+    gvs_search_graph *g = new gvs_search_graph();
+    g->ix = ix;
+    const size_t words = (size_t)gvs::scratch_slots(p, lp, nq) << lp.vlog2;
+    hipStream_t cs = nullptr;
+    bool ok = hip_ok(hipMalloc(&g->vtab, 4 * words), "hipMalloc visited tables") &&
+              hip_ok(hipMemset(g->vtab, 0, 4 * words), "hipMemset visited tables") &&
+              hip_ok(hipMalloc(&g->status, sizeof(int32_t) * nq), "hipMalloc status") &&
+              hip_ok(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking), "hipStreamCreateWithFlags") &&
+              hip_ok(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    if (ok) {
+        rc = gvs::launch_search(ix, p, lp, P, nq, cs, g->vtab, g->status);
+        hipError_t e = hipStreamEndCapture(cs, &g->graph);
+        ok = rc == GVS_OK && hip_ok(e, "hipStreamEndCapture") &&
+             hip_ok(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+    }
+    if (cs) (void)hipStreamDestroy(cs);
+    if (!ok) {
+        free_search_graph(g);
+        return rc ? rc : GVS_ERR_HIP;
+    }
+    *out = g;
+    return GVS_OK;
+}
+
+extern "C" int gvs_search_graph_launch(gvs_search_graph *g, void *stream)
+{
+    if (!g || !g->exec) {
+        gvs::set_error("gvs_search_graph_launch: invalid argument");
+        return GVS_ERR_ARG;
+    }
+    return hip_ok(hipGraphLaunch(g->exec, (hipStream_t)stream), "hipGraphLaunch") ? GVS_OK : GVS_ERR_HIP;
+}
+
+extern "C" int gvs_search_graph_destroy(gvs_search_graph *g)
+{
+    if (!g) return GVS_ERR_ARG;
+    free_search_graph(g);
     return GVS_OK;
 }
 

@@ -41,6 +41,26 @@ kfn_t pick_rows(int rows)
     return k_search<NCH, STORE, METRIC, 8>;
 }
 
+// One workgroup of P.waves_per_block waves per query (intra-query).
+template <int NCH, int STORE, int METRIC, int WROWS>
+__global__ __launch_bounds__(512) void k_search_mw(KParams P)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int64_t qi = P.q_base + blockIdx.x;
+    if (qi >= P.nq) return;
+    search_query_mw<NCH, STORE, METRIC, WROWS>(P, qi, blockIdx.x, smem);
+}
+
+// rows per wave = ceil(mc * ceil(r0/64) / waves) rounded up to 1, 2 or 4
+template <int NCH, int STORE, int METRIC>
+kfn_t pick_rows_mw(int rows_per_wave)
+{
+    if (rows_per_wave <= 1) return k_search_mw<NCH, STORE, METRIC, 1>;
+    if (rows_per_wave <= 2) return k_search_mw<NCH, STORE, METRIC, 2>;
+    if (rows_per_wave <= 4) return k_search_mw<NCH, STORE, METRIC, 4>;
+    return nullptr;
+}
+
 }  // namespace gvsk
 
 // one table per translation unit; returns nullptr when not compiled
@@ -50,3 +70,6 @@ gvsk::kfn_t gvs_kernel_bf16_l2(int nch, int rows);
 gvsk::kfn_t gvs_kernel_bf16_ip(int nch, int rows);
 gvsk::kfn_t gvs_kernel_fp8_l2(int nch, int rows);
 gvsk::kfn_t gvs_kernel_fp8_ip(int nch, int rows);
+// several waves per query (fp32 storage)
+gvsk::kfn_t gvs_kernel_mw_f32_l2(int nch, int rows_per_wave);
+gvsk::kfn_t gvs_kernel_mw_f32_ip(int nch, int rows_per_wave);

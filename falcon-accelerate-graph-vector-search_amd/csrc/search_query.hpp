@@ -315,6 +315,62 @@ __device__ __forceinline__ int expand_lds(const KParams &P, const WaveLds &L, co
     return nl;
 }
 
+// Layer-0 entry point and its distance: the given entry, or hnswlib's
+// upper-layer greedy descent (HNSWLIB/hnswalg.h:1119-1147): from the top
+// level down, move to the closest neighbour while it is strictly closer.
+template <int NCH, int STORE, int METRIC>
+__device__ __forceinline__ void find_entry(const KParams &P, int64_t qi, const WaveLds &L, const float *qf,
+                                           uint32_t &ep, float &epd, int &up_hops, int &up_evals)
+{
+    const int lane = lane_id();
+    if (P.entries) {
+        ep = (uint32_t)P.entries[qi];
+        if (lane == 0) L.lids[0] = ep;
+        wave_sync();
+        eval_rows<NCH, STORE, METRIC>(P, qf, L.lids, 1, L.lkeys);
+        wave_sync();
+        epd = key_dist(L.lkeys[0]);
+        return;
+    }
+    uint32_t cur = (uint32_t)P.entry;
+    if (lane == 0) L.lids[0] = cur;
+    wave_sync();
+    eval_rows<NCH, STORE, METRIC>(P, qf, L.lids, 1, L.lkeys);
+    wave_sync();
+    float curd = key_dist(L.lkeys[0]);
+    up_evals = 1;
+    for (int level = P.max_level; level > 0; level--) {
+        for (;;) {
+            const int64_t row = P.upper_off[cur] + (level - 1);
+            const int32_t nid = lane < P.ru ? P.adj_up[row * P.ru + lane] : -1;
+            const uint64_t bal = __ballot(nid >= 0);
+            const int n = __popcll(bal);
+            wave_sync();
+            if (nid >= 0) L.lids[mbcnt(bal)] = (uint32_t)nid;
+            wave_sync();
+            eval_rows<NCH, STORE, METRIC>(P, qf, L.lids, n, L.lkeys);
+            wave_sync();
+            up_hops++;
+            up_evals += n;
+            uint64_t best = EMPTY_KEY;
+            if (lane < n) best = ((uint64_t)ord_f(key_dist(L.lkeys[lane])) << 32) | (uint64_t)lane;
+            best = wave_min_u64(best);
+            if (n > 0) {
+                const float bd = unord_f((uint32_t)(best >> 32));
+                if (bd < curd) {  // strictly closer
+                    curd = bd;
+                    cur = L.lids[(int)(best & 63u)];
+                    wave_sync();
+                    continue;
+                }
+            }
+            break;
+        }
+    }
+    ep = cur;
+    epd = curd;
+}
+
 template <int NCH, int STORE, int METRIC, int MROWS>
 __device__ void search_query_req(const KParams &P, int64_t qi, int64_t slot, unsigned char *lds, int k, int ef,
                                  const float *Q, bool sys_query)
@@ -341,58 +397,8 @@ __device__ void search_query_req(const KParams &P, int64_t qi, int64_t slot, uns
 
     int up_hops = 0, up_evals = 0;
     uint32_t ep;
-    float epd = 0.0f;
-    bool have_epd = false;
-    if (P.entries) {
-        ep = (uint32_t)P.entries[qi];
-    } else {
-        // upper-layer greedy descent, HNSWLIB/hnswalg.h:1119-1147
-        uint32_t cur = (uint32_t)P.entry;
-        if (lane == 0) L.lids[0] = cur;
-        wave_sync();
-        eval_rows<NCH, STORE, METRIC>(P, qf, L.lids, 1, L.lkeys);
-        wave_sync();
-        float curd = key_dist(L.lkeys[0]);
-        up_evals = 1;
-        for (int level = P.max_level; level > 0; level--) {
-            for (;;) {
-                const int64_t row = P.upper_off[cur] + (level - 1);
-                const int32_t nid = lane < P.ru ? P.adj_up[row * P.ru + lane] : -1;
-                const uint64_t bal = __ballot(nid >= 0);
-                const int n = __popcll(bal);
-                wave_sync();
-                if (nid >= 0) L.lids[mbcnt(bal)] = (uint32_t)nid;
-                wave_sync();
-                eval_rows<NCH, STORE, METRIC>(P, qf, L.lids, n, L.lkeys);
-                wave_sync();
-                up_hops++;
-                up_evals += n;
-                uint64_t best = EMPTY_KEY;
-                if (lane < n) best = ((uint64_t)ord_f(key_dist(L.lkeys[lane])) << 32) | (uint64_t)lane;
-                best = wave_min_u64(best);
-                if (n > 0) {
-                    const float bd = unord_f((uint32_t)(best >> 32));
-                    if (bd < curd) {  // strictly closer
-                        curd = bd;
-                        cur = L.lids[(int)(best & 63u)];
-                        wave_sync();
-                        continue;
-                    }
-                }
-                break;
-            }
-        }
-        ep = cur;
-        epd = curd;
-        have_epd = true;
-    }
-    if (!have_epd) {
-        if (lane == 0) L.lids[0] = ep;
-        wave_sync();
-        eval_rows<NCH, STORE, METRIC>(P, qf, L.lids, 1, L.lkeys);
-        wave_sync();
-        epd = key_dist(L.lkeys[0]);
-    }
+    float epd;
+    find_entry<NCH, STORE, METRIC>(P, qi, L, qf, ep, epd, up_hops, up_evals);
 
     PROF_MARK(0);
     // layer-0 search, HNSWLIB/hnswalg.h:240-326; groups of up to mc
@@ -584,6 +590,212 @@ __device__ __forceinline__ void search_query(const KParams &P, int64_t qi, int64
     const int k = P.q_k ? P.q_k[qi] : P.k;
     const int ef = P.q_ef ? P.q_ef[qi] : P.ef;
     search_query_req<NCH, STORE, METRIC, MROWS>(P, qi, slot, lds, k, ef, P.queries + qrow * P.ldq, false);
+}
+
+// ---- several waves per query (intra-query parallelism) -------------------
+// One workgroup of W waves owns one query (the reference's INTRA design
+// spreads one query's neighbour work over channels, INTRA/utils.hpp:396-570).
+// Each step: wave 0 selects the group of up to mc candidates; the group's
+// mc * ceil(r0/64) adjacency rows are dealt round-robin to the W waves, which
+// test them against the shared visited set and gather and evaluate the new
+// ids into their own key lists; wave 0 then merges all lists into the pool.
+// The visited set is exact and shared, and the merge keeps the ef smallest
+// keys whatever the order of the lists, so the result, hops and evaluation
+// counts equal those of one wave running DST(mc, 1).
+__device__ __forceinline__ void wg_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Workgroup LDS: shared part, then WROWS*64 keys and ids per wave.
+struct WgLds {
+    uint64_t *poolA, *poolB, *bloom, *lkeys;
+    uint32_t *lids, *sel, *ctl, *vh;
+};
+
+template <int WROWS>
+__device__ __forceinline__ WgLds carve_wg(const KParams &P, unsigned char *base, int W)
+{
+    WgLds G;
+    G.poolA = (uint64_t *)base;
+    G.poolB = G.poolA + P.ef_cap;
+    G.bloom = G.poolB + P.ef_cap;
+    G.lkeys = G.bloom + (1u << P.blog2);
+    G.lids = (uint32_t *)(G.lkeys + W * WROWS * WAVE);
+    G.sel = G.lids + W * WROWS * WAVE;
+    G.ctl = G.sel + 8;
+    G.vh = G.ctl + 8;
+    return G;
+}
+
+// ctl words
+constexpr int WG_NSEL = 0, WG_EVALS = 1, WG_MODE = 2, WG_STOP = 3;
+
+template <int NCH, int STORE, int METRIC, int WROWS>
+__device__ void search_query_mw(const KParams &P, int64_t qi, int64_t slot, unsigned char *lds)
+{
+    constexpr int E = StoreTraits<STORE>::E;
+    const int lane = lane_id();
+    const int p = lane & 7;
+    const int W = P.waves_per_block;
+    const int w = threadIdx.x >> 6;
+    WgLds G = carve_wg<WROWS>(P, lds, W);
+    PROF_DECL
+    // this wave's view: its own key/id lists, the shared visited structures
+    WaveLds L;
+    L.poolA = G.poolA;
+    L.poolB = G.poolB;
+    L.lkeys = G.lkeys + w * WROWS * WAVE;
+    L.lids = G.lids + w * WROWS * WAVE;
+    L.bloom = G.bloom;
+    L.vh = G.vh;
+    L.selA = G.sel;
+    L.selB = G.sel;
+    L.nbw = 1u << P.blog2;
+
+    const int k = P.k;
+    const int ef = max(P.ef, k);
+    const float *Q = P.queries + qi * P.ldq;
+    float qf[NCH * E];
+#pragma unroll
+    for (int i = 0; i < NCH; i++)
+#pragma unroll
+        for (int t = 0; t < E; t++) {
+            const int idx = (p + 8 * i) * E + t;
+            qf[i * E + t] = idx < P.dim ? Q[idx] : 0.0f;
+        }
+    // every wave runs the (cheap, deterministic) descent on its own lists
+    int up_hops = 0, up_evals = 0;
+    uint32_t ep;
+    float epd;
+    find_entry<NCH, STORE, METRIC>(P, qi, L, qf, ep, epd, up_hops, up_evals);
+
+    bool lds_mode = P.hlog2 > 0;
+    const uint32_t *frozen = nullptr;
+    uint32_t *T = lds_mode ? nullptr : P.vtab + ((size_t)slot << P.vlog2);
+    int tlimit = lds_mode ? (int)((3u << P.hlog2) >> 2) - P.lcap : (int)((3u << P.vlog2) >> 2) - P.lcap;
+    const int tid = threadIdx.x, nthr = W * WAVE;
+    if (lds_mode) {
+        for (uint32_t i = tid; i < (1u << P.hlog2); i += nthr) G.vh[i] = 0u;
+    } else {
+        for (uint32_t i = tid; i < L.nbw; i += nthr) G.bloom[i] = 0ull;
+    }
+    if (tid < 8) G.ctl[tid] = 0u;
+    wg_sync();
+    uint64_t *pool = G.poolA, *pool2 = G.poolB;
+    int np = 1, hint = 0, hops = 0;
+    if (w == 0 && lane == 0) {
+        pool[0] = mk_key(epd, ep);
+        G.ctl[WG_EVALS] = 1u;
+        if (lds_mode) {
+            G.vh[vhash(ep, P.hlog2)] = ep + 1u;
+        } else {
+            uint32_t bw;
+            uint64_t m;
+            bloom_pos(ep, P.bloom_seed, L.nbw, bw, m);
+            G.bloom[bw] |= m;
+            atomicCAS(&T[vhash(ep, P.vlog2)], 0u, ep + 1u);
+        }
+    }
+    wg_sync();
+    const int rpc = (P.r0 + WAVE - 1) / WAVE;
+    bool overflow = false;
+    for (;;) {
+        if (w == 0) {
+            const int nsel = select_group(pool, np, P.mc, hint, G.sel);
+            hops += nsel;
+            if (lane == 0) G.ctl[WG_NSEL] = (uint32_t)nsel;
+        }
+        wg_sync();
+        const int nsel = (int)G.ctl[WG_NSEL];
+        if (nsel == 0) break;
+        const int nrows = nsel * rpc;
+        int32_t ids[WROWS];
+        int myrows = 0;
+#pragma unroll
+        for (int j = 0; j < WROWS; j++) {
+            ids[j] = -1;
+            const int r = w + W * j;
+            if (r < nrows) {
+                const int ci = r / rpc, jj = (r - ci * rpc) * WAVE + lane;
+                if (jj < P.r0) ids[j] = P.adj0[(size_t)G.sel[ci] * P.r0 + jj];
+                myrows = j + 1;
+            }
+        }
+        int evals_w = 0, nres = 0;
+        if (myrows > 0) {
+            if (lds_mode)
+                nres = expand_lds<NCH, STORE, METRIC, WROWS>(P, L, qf, ids, myrows, evals_w EXP_PROF_ARGS);
+            else
+                nres = expand_group<NCH, STORE, METRIC, WROWS>(P, L, qf, T, ids, myrows, evals_w,
+                                                               frozen EXP_PROF_ARGS);
+        }
+        for (int i = nres + lane; i < WROWS * WAVE; i += WAVE) L.lkeys[i] = EMPTY_KEY;
+        if (lane == 0 && evals_w) atomicAdd(&G.ctl[WG_EVALS], (uint32_t)evals_w);
+        wg_sync();
+        if (w == 0) {
+            np = merge_into_pool(pool, pool2, np, ef, G.lkeys, W * WROWS * WAVE, hint);
+            const int evals = (int)G.ctl[WG_EVALS];
+            if (evals > tlimit) {
+                if (lds_mode && P.vtab) {
+                    for (uint32_t i = lane; i < L.nbw; i += WAVE) G.bloom[i] = 0ull;
+                    if (lane == 0) {
+                        G.ctl[WG_MODE] = 1u;
+                        atomicAdd(&P.counters[2], 1u);
+                    }
+                    tlimit = evals + (int)((3u << P.vlog2) >> 2) - P.lcap;
+                } else if (lane == 0) {
+                    G.ctl[WG_STOP] = 1u;
+                }
+            }
+        }
+        wg_sync();
+        if (G.ctl[WG_STOP]) {
+            overflow = true;
+            break;
+        }
+        if (lds_mode && G.ctl[WG_MODE]) {
+            lds_mode = false;
+            frozen = G.vh;
+            T = P.vtab + ((size_t)slot << P.vlog2);
+            tlimit = (int)G.ctl[WG_EVALS] + (int)((3u << P.vlog2) >> 2) - P.lcap;
+        }
+    }
+
+    if (w == 0) {
+        for (int i = lane; i < P.k; i += WAVE) {
+            const uint64_t key = (!overflow && i < np && i < k) ? pool[i] : EMPTY_KEY;
+            int64_t lab = -1;
+            float d = __builtin_inff();
+            if (key != EMPTY_KEY) {
+                const uint32_t id = key_id(key);
+                lab = P.labels ? P.labels[id] : (int64_t)id;
+                d = key_dist(key);
+            }
+            P.out_labels[qi * P.k + i] = lab;
+            P.out_dists[qi * P.k + i] = d;
+        }
+        if (lane == 0) {
+            if (P.out_stats) {
+                P.out_stats[qi * 4 + 0] = hops;
+                P.out_stats[qi * 4 + 1] = (int)G.ctl[WG_EVALS];
+                P.out_stats[qi * 4 + 2] = up_hops;
+                P.out_stats[qi * 4 + 3] = up_evals;
+            }
+            if (P.status) P.status[qi] = overflow ? 1 : 0;
+            if (overflow) atomicAdd(&P.counters[1], 1u);
+        }
+    }
+    // leave the query's visited table empty for the next query of this slot
+    if (T) {
+        uint4 *T4 = (uint4 *)T;
+        const uint4 z = make_uint4(0u, 0u, 0u, 0u);
+        for (uint32_t i = tid; i < ((1u << P.vlog2) >> 2); i += nthr) T4[i] = z;
+        __builtin_amdgcn_s_waitcnt(0);
+    }
+    wg_sync();
 }
 
 }  // namespace gvsk

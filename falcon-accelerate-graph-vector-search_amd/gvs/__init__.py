@@ -77,6 +77,9 @@ def _load():
         "gvs_search_host": (C.c_int, [P, P, I64, I64, P, P, P, P, P]),
         "gvs_search_overflow_count": (C.c_int, [P, P]),
         "gvs_search_counters": (C.c_int, [P, P]),
+        "gvs_search_graph_create": (C.c_int, [P, P, I64, I64, P, P, P, P, P, P]),
+        "gvs_search_graph_launch": (C.c_int, [P, P]),
+        "gvs_search_graph_destroy": (C.c_int, [P]),
         "gvs_device_copy": (C.c_int, [P, P, I64, P]),
         "gvs_bruteforce_f64_device": (C.c_int, [P, P, I64, I64, I32, P, P, P]),
         "gvs_server_open": (C.c_int, [P, P, P]),
@@ -105,7 +108,8 @@ def exported_symbols():
     names = ["gvs_last_error", "gvs_version", "gvs_hnsw_build", "gvs_nsg_from_hnsw", "gvs_hnswlib_load",
              "gvs_hnswlib_save", "gvs_nsg_load", "gvs_host_graph_view", "gvs_host_graph_free",
              "gvs_gen_mixture", "gvs_index_create", "gvs_index_destroy", "gvs_index_device_bytes",
-             "gvs_search_device", "gvs_search_host", "gvs_search_overflow_count", "gvs_search_counters", "gvs_device_copy", "gvs_bruteforce_f64_device", "gvs_server_open",
+             "gvs_search_device", "gvs_search_host", "gvs_search_overflow_count", "gvs_search_counters", "gvs_search_graph_create",
+             "gvs_search_graph_launch", "gvs_search_graph_destroy", "gvs_device_copy", "gvs_bruteforce_f64_device", "gvs_server_open",
              "gvs_server_submit", "gvs_server_poll", "gvs_server_poll_stats", "gvs_server_cancel",
              "gvs_server_stats", "gvs_server_close", "gvs_server_loadgen"]
     return [n for n in names if hasattr(lib, n)]
@@ -257,6 +261,16 @@ class DeviceIndex:
                                      C.c_void_p(stats_ptr) if stats_ptr else None,
                                      C.c_void_p(stream) if stream else None))
 
+    def capture(self, q_ptr, nq, ldq, labels_ptr, dists_ptr, k=10, ef=64, entries_ptr=None, stats_ptr=None, **kw):
+        """Record the search of these device buffers into a hipGraph (SearchGraph)."""
+        p = make_params(k=k, ef=ef, **kw)
+        h = C.c_void_p()
+        _check(lib.gvs_search_graph_create(self._h, C.c_void_p(q_ptr), nq, ldq,
+                                           C.c_void_p(entries_ptr) if entries_ptr else None, C.byref(p),
+                                           C.c_void_p(labels_ptr), C.c_void_p(dists_ptr),
+                                           C.c_void_p(stats_ptr) if stats_ptr else None, C.byref(h)))
+        return SearchGraph(h, self)
+
     def overflow_count(self):
         c = C.c_int64(0)
         _check(lib.gvs_search_overflow_count(self._h, C.byref(c)))
@@ -277,6 +291,28 @@ class DeviceIndex:
         if h is not None and h.value and lib is not None:
             lib.gvs_index_destroy(h)
             self._h = None
+
+
+class SearchGraph:
+    """A captured search (C ABI gvs_search_graph_*): launch() replays it."""
+
+    def __init__(self, h, dev):
+        self._h = h
+        self.dev = dev  # keeps the index alive
+
+    def launch(self, stream=None):
+        _check(lib.gvs_search_graph_launch(self._h, C.c_void_p(stream) if stream else None))
+
+    def close(self):
+        if self._h is not None and self._h.value:
+            _check(lib.gvs_search_graph_destroy(self._h))
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Server:

@@ -57,6 +57,13 @@ class DeviceArray:
         check(hip.hipMemcpy(d.ptr, a.ctypes.data_as(_P), a.nbytes, H2D), "hipMemcpy H2D")
         return d
 
+    def upload(self, a):
+        """Copy a host array of the same byte size into this buffer."""
+        a = np.ascontiguousarray(a)
+        if a.nbytes != self.nbytes:
+            raise ValueError(f"upload of {a.nbytes} bytes into a {self.nbytes}-byte buffer")
+        check(hip.hipMemcpy(self.ptr, a.ctypes.data_as(_P), a.nbytes, H2D), "hipMemcpy H2D")
+
     @property
     def address(self):
         return self.ptr.value

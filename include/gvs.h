@@ -171,6 +171,21 @@ GVS_API int gvs_search_counters(gvs_index *ix, int64_t counters[3]);
  * pointers and size).  No reference counterpart; used to calibrate HBM byte
  * counters against a kernel of known traffic. */
 GVS_API int gvs_device_copy(void *dst, const void *src, int64_t bytes, void *stream);
+/* Captured search: the kernels of one gvs_search_device call (same
+ * arguments; the device buffers are fixed at capture time) recorded into a
+ * hipGraph, with visited tables of its own.  Each launch replays the search
+ * on whatever the query buffer then holds, with one graph launch instead of
+ * the per-call planning and kernel launches.  Replays behave like
+ * GVS_FLAG_ASYNC: a query whose visited table overflows returns label -1 and
+ * is counted by gvs_search_overflow_count().  Launches of one graph must not
+ * overlap (they share its tables).  No reference counterpart: the FPGA host
+ * re-enqueues its kernel per batch (INTER/host.cpp:960-995). */
+typedef struct gvs_search_graph gvs_search_graph;
+GVS_API int gvs_search_graph_create(gvs_index *ix, const float *queries, int64_t nq, int64_t ldq,
+                                    const int32_t *entries, const gvs_params *p, int64_t *labels,
+                                    float *dists, int32_t *stats, gvs_search_graph **out);
+GVS_API int gvs_search_graph_launch(gvs_search_graph *g, void *stream);
+GVS_API int gvs_search_graph_destroy(gvs_search_graph *g);
 /* Same with HOST buffers; synchronous (copies in and out). */
 GVS_API int gvs_search_host(gvs_index *ix, const float *queries, int64_t nq, int64_t ldq,
                             const int32_t *entries, const gvs_params *p,

@@ -183,6 +183,24 @@ def test_bruteforce_f64_mfma_matches_oracle(metric, k):
     assert mism.mean() < 0.01
 
 
+def test_bruteforce_before_and_after_search_on_one_index():
+    """Growing the search scratch must leave the brute-force norms intact."""
+    from gvs import hip as H
+    x, g, _ = graph(20000, 128, 16, 100)
+    dev = gvs.DeviceIndex(g)
+    q = queries(120, 128, seed=21)
+    oi, od = orc.bruteforce_f64(x, q, 10)
+    qt = H.DeviceArray.from_host(q)
+    for step in range(2):
+        print(f"case bruteforce around search step={step}", flush=True)
+        ids = H.DeviceArray((120, 10), np.int32)
+        ds = H.DeviceArray((120, 10), np.float64)
+        dev.bruteforce_f64_device(qt.address, 120, 128, 10, ids.address, ds.address)
+        H.synchronize()
+        np.testing.assert_allclose(ds.to_host(), od, rtol=1e-9, atol=1e-9)
+        dev.search(queries(3000 * (step + 1), 128, seed=22), k=10, ef=64)
+
+
 @pytest.mark.parametrize("mc", [1, 4])
 def test_lds_visited_spill_is_exact(mc):
     """A query that outgrows a small LDS visited hash freezes it and goes on
@@ -248,3 +266,74 @@ def test_low_precision_traversal_and_fp32_rerank(storage):
           f"(no rerank {orc.recall_at(lab_nr, gt, 10):.4f}); top-1 traversal distance rel. error "
           f"mean {rel.mean():.2e} max {rel.max():.2e}", flush=True)
     assert rlp >= r32 - 0.02
+
+
+@pytest.mark.parametrize("persistent", [False, True])
+def test_captured_graph_replays_match_eager(persistent):
+    """A captured search replayed several times, with new queries in its
+    buffer and eager searches on the same index between replays (one of
+    them large enough to regrow the index's own scratch), gives the eager
+    result every time."""
+    from gvs import hip as H
+    x, g, dev = graph(20000, 128, 16, 100)
+    nq, k, ef = 400, 10, 64
+    batches = [queries(nq, 128, seed=30 + i) for i in range(3)]
+    dq = H.DeviceArray.from_host(batches[0])
+    dl = H.DeviceArray((nq, k), np.int64)
+    dd = H.DeviceArray((nq, k), np.float32)
+    ds = H.DeviceArray((nq, 4), np.int32)
+    st = H.Stream()
+    sg = dev.capture(dq.address, nq, 128, dl.address, dd.address, k=k, ef=ef, mc=2, stats_ptr=ds.address,
+                     persistent=persistent)
+    for rep in range(5):
+        b = batches[rep % 3]
+        print(f"case graph replay persistent={persistent} rep={rep}", flush=True)
+        dq.upload(b)
+        sg.launch(st.handle)
+        st.synchronize()
+        lab, dist, cnt = dev.search(b, k=k, ef=ef, mc=2, with_stats=True)
+        np.testing.assert_array_equal(dl.to_host(), lab)
+        np.testing.assert_array_equal(dd.to_host(), dist)
+        np.testing.assert_array_equal(ds.to_host(), cnt)
+        if rep == 2:
+            dev.search(queries(6000, 128, seed=40), k=10, ef=200)
+    sg.close()
+
+
+@pytest.mark.parametrize("W,mc", [(2, 1), (2, 2), (4, 4), (8, 8), (8, 2), (4, 8)])
+def test_intra_query_waves_match_single_wave_dst(W, mc):
+    """Several waves per query (one workgroup) give the ids, distances and
+    counters of one wave running DST(mc, 1), i.e. the oracle's."""
+    x, g, dev = graph(20000, 128, 16, 100)
+    og = orc.OracleGraph(g.arrays())
+    q = queries(150, 128, seed=50)
+    for ef in (32, 128):
+        print(f"case intra W={W} mc={mc} ef={ef}", flush=True)
+        check_equal(dev, og, q, 10, ef, mc=mc, waves_per_query=W)
+
+
+@pytest.mark.parametrize("W", [2, 8])
+def test_intra_query_waves_hbm_spill_ip_and_entries(W):
+    """The intra-query path through the HBM visited table (large ef), a
+    spilling LDS hash, inner product, and given entry points."""
+    x, g, dev = graph(20000, 128, 16, 100)
+    og = orc.OracleGraph(g.arrays())
+    q = queries(100, 128, seed=51)
+    print(f"case intra hbm W={W}", flush=True)
+    check_equal(dev, og, q, 10, 200, mc=4, waves_per_query=W)
+    check_equal(dev, og, q, 10, 64, mc=2, waves_per_query=W, visited="lds", visited_log2=10)
+    ent = np.random.default_rng(3).integers(0, 20000, size=100).astype(np.int32)
+    check_equal(dev, og, q, 10, 64, mc=2, entries=ent, waves_per_query=W)
+    xi, gi, devi = graph(8000, 96, 12, 100, metric=gvs.IP, normalize=True)
+    ogi = orc.OracleGraph(gi.arrays())
+    qi = queries(100, 96, seed=52, normalize=True)
+    check_equal(devi, ogi, qi, 10, 64, mc=4, waves_per_query=W)
+
+
+def test_intra_query_rejects_unsupported():
+    x, g, dev = graph(20000, 128, 16, 100)
+    q = queries(4, 128, seed=53)
+    with pytest.raises(gvs.GvsError):
+        dev.search(q, k=10, ef=64, mc=2, mg=2, waves_per_query=4)
+    with pytest.raises(gvs.GvsError):
+        dev.search(q, k=10, ef=64, waves_per_query=3)
