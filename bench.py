@@ -50,6 +50,8 @@ def read_traffic(path, cfg):
     if not os.path.exists(path):
         return None
     key = f"ef={cfg['ef']} mc={cfg['mc']} mg={cfg.get('mg', 1)} persistent={int(bool(cfg['persistent']))}"
+    if cfg.get("grid_waves", 0):
+        key += f" grid_waves={cfg['grid_waves']}"
     for line in open(path):
         line = line.strip()
         if line.startswith("traffic ") and key in line:
@@ -132,16 +134,17 @@ def intra_latency(a, gvs, g, q):
                       "best_per_ef": best, "rows": rows}), flush=True)
 
 
-def serve_child(a, gvs, g, q):
+def serve_child(a, gvs, g, q, ks=None, efs=None, metric_name="L2"):
     """Open-loop Poisson load at fractions of the serving loop's saturated
     throughput; latency runs from a request's arrival to the poll that sees
-    its result."""
+    its result.  ks / efs: per-request k and ef (default: a.k, a.serve_ef)."""
     dev = gvs.DeviceIndex(g)
     nreq = min(len(q), 20000)
     qq = q[:nreq]
-    ks = np.full(nreq, a.k, dtype=np.int32)
-    efs = np.full(nreq, a.serve_ef, dtype=np.int32)
-    srv = gvs.Server(dev, max_inflight=4096, max_k=a.k, max_ef=max(a.serve_ef, a.k))
+    ks = np.full(nreq, a.k, dtype=np.int32) if ks is None else ks[:nreq]
+    efs = np.full(nreq, a.serve_ef, dtype=np.int32) if efs is None else efs[:nreq]
+    max_k, max_ef = int(ks.max()), int(max(efs.max(), ks.max()))
+    srv = gvs.Server(dev, max_inflight=4096, max_k=max_k, max_ef=max_ef)
     _, _, lat, wall = srv.loadgen(qq, 1e9, ks, efs, seed=1)
     peak = nreq / wall
     log(f"[serve] saturated {peak:.0f} QPS, p50 {np.percentile(lat, 50):.0f} us")
@@ -153,8 +156,11 @@ def serve_child(a, gvs, g, q):
                       "p99_us": round(float(np.percentile(lat, 99)), 1)})
         log(f"[serve] load {frac:.2f}: p50 {loads[-1]['p50_us']} us p99 {loads[-1]['p99_us']} us")
     srv.close()
-    print(json.dumps({"requests": nreq, "k": a.k, "ef": a.serve_ef, "peak_qps": round(peak, 1),
-                      "max_inflight": 4096, "loads": loads}), flush=True)
+    kdesc = a.k if len(set(ks.tolist())) == 1 else sorted(set(ks.tolist()))
+    edesc = a.serve_ef if len(set(efs.tolist())) == 1 else [int(efs.min()), int(efs.max())]
+    print(json.dumps({"requests": nreq, "k": kdesc, "ef": edesc, "peak_qps": round(peak, 1),
+                      "max_inflight": 4096, "workload": f"{a.n}x{a.dim} {metric_name}",
+                      "loads": loads}), flush=True)
 
 
 def main():
@@ -184,6 +190,9 @@ def main():
     ap.add_argument("--intra", action="store_true",
                     help="instead of the throughput run: batch=1 latency of intra-query DST "
                          "(1-8 waves per query, mc 1-8, ef 64-512) against the plain search")
+    ap.add_argument("--c5", action="store_true",
+                    help="instead of the throughput run: serving on 1M x 384 unit-norm inner product, "
+                         "Poisson arrivals at 25/50/75/90%% of peak, k in {1,10,100}, ef uniform in [32,512]")
     ap.add_argument("--serve-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--serve-ef", type=int, default=64, help=argparse.SUPPRESS)
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "k_search_1Mx128_hbm_traffic.txt"),
@@ -196,24 +205,37 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    os.environ.setdefault("GVS_VERBOSE", "1")  # the graph builder reports its progress
     import gvs
     from gvs import hip as H
 
     t0 = time.time()
-    x = gvs.gen_mixture(a.n, a.dim, a.centres, 1234, 1235, nthreads=a.build_threads)
-    q = gvs.gen_mixture(a.nq, a.dim, a.centres, 1234, 98765, nthreads=a.build_threads)
+    metric = gvs.L2
+    if a.c5:
+        a.dim = 384
+        metric = gvs.IP
+    x = gvs.gen_mixture(a.n, a.dim, a.centres, 1234, 1235, normalize=a.c5, nthreads=a.build_threads)
+    q = gvs.gen_mixture(a.nq, a.dim, a.centres, 1234, 98765, normalize=a.c5, nthreads=a.build_threads)
     log(f"[bench] data {a.n}x{a.dim} + {a.nq} queries in {time.time() - t0:.1f}s; building HNSW "
         f"M={a.M} efC={a.efc} on {a.build_threads} threads")
     t0 = time.time()
     if a.graph_cache and os.path.exists(a.graph_cache):
-        g = gvs.HostGraph.load_hnswlib(a.graph_cache, a.dim)
+        g = gvs.HostGraph.load_hnswlib(a.graph_cache, a.dim, metric=metric)
         log(f"[bench] graph loaded from {a.graph_cache}")
     else:
-        g = gvs.HostGraph.build_hnsw(x, M=a.M, ef_construction=a.efc, nthreads=a.build_threads)
+        g = gvs.HostGraph.build_hnsw(x, M=a.M, ef_construction=a.efc, metric=metric, nthreads=a.build_threads)
     build_s = time.time() - t0
     log(f"[bench] build {build_s:.1f}s")
     if a.serve_child:
         serve_child(a, gvs, g, q)
+        return
+    if a.c5:
+        if a.graph_cache and not os.path.exists(a.graph_cache):
+            g.save_hnswlib(a.graph_cache)
+        rng = np.random.default_rng(5)
+        ks = rng.choice([1, 10, 100], size=len(q)).astype(np.int32)
+        efs = rng.integers(32, 513, size=len(q)).astype(np.int32)
+        serve_child(a, gvs, g, q, ks=ks, efs=efs, metric_name="inner product, unit-norm")
         return
     if a.intra:
         intra_latency(a, gvs, g, q)
@@ -273,6 +295,8 @@ def main():
         kv = dict(item.split("=") for item in a.config.split(","))
         fixed = dict(ef=int(kv.get("ef", 64)), mc=int(kv.get("mc", 1)), mg=int(kv.get("mg", 1)),
                      persistent=bool(int(kv.get("persistent", 0))))
+        if int(kv.get("grid_waves", 0)):
+            fixed["grid_waves"] = int(kv["grid_waves"])
         efs = [fixed["ef"]]
     plain_ef, plain_rec = None, None
     sweep = []
@@ -288,22 +312,23 @@ def main():
         plain_ef, plain_rec = efs[-1], rec
     plain_cfg = dict(ef=plain_ef, mc=1, persistent=False)
 
-    # autotune the batched path: persistent work queue x DST candidates per step x ef
+    # autotune the batched path with the library's autotuner (gvs_autotune:
+    # mc x mg x persistent x resident waves) at each ef up to the plain one;
+    # keep the fastest configuration whose recall holds the target
     best = None
     if a.config:
         best = (0.0, fixed, measure_recall(fixed))
     for ef in ([] if a.config else [e for e in efs if e <= plain_ef]):
-        for mc in (1, 2, 4):
-            for mg in (1, 2):
-                for persistent in (False, True):
-                    cfg = dict(ef=ef, mc=mc, mg=mg, persistent=persistent)
-                    rec = measure_recall(cfg)
-                    if rec < a.target:
-                        continue
-                    ms = timed(cfg, 2)
-                    log(f"[bench] tune ef={ef} mc={mc} mg={mg} persistent={persistent}: recall={rec:.4f} {ms:.2f} ms")
-                    if best is None or ms < best[0]:
-                        best = (ms, cfg, rec)
+        cfg, ms = dev.autotune(dq.address, a.nq, a.dim, dlab.address, ddist.address, k=a.k, ef=ef,
+                               stream=stream.handle)
+        cfg = {kk: v for kk, v in cfg.items() if not (kk == "grid_waves" and v == 0) and kk != "waves_per_query"}
+        rec = measure_recall(cfg)
+        log(f"[bench] autotune ef={ef}: {cfg} recall={rec:.4f} {ms:.3f} ms")
+        if rec < a.target:
+            continue
+        ms = timed(cfg, 3)
+        if best is None or ms < best[0]:
+            best = (ms, cfg, rec)
     if best is None:
         best = (timed(plain_cfg, 2), plain_cfg, plain_rec)
     tuned_cfg, tuned_rec = best[1], best[2]
@@ -400,6 +425,17 @@ def main():
                          f"{threads} host threads",
                "recall_at_10": round(recall_at(ids, gt[:ns], a.k), 4)}
         log(f"[bench] CPU oracle {cpu['value']} QPS on {threads} threads ({ns} queries, {cs:.1f}s)")
+        # batch=1 latency, one host thread, one query per call (the reference's
+        # CPU latency mode: run_all_hnsw_search.py:104-127, first and last dropped)
+        lat = []
+        for i in range(502):
+            t0 = time.perf_counter()
+            og.search(q[i:i + 1], a.k, plain_ef, order=orc.SEQ, nthreads=1)
+            lat.append(time.perf_counter() - t0)
+        lat = np.array(lat[1:-1]) * 1e6
+        cpu["batch1_latency_us"] = {"p50": round(float(np.percentile(lat, 50)), 1),
+                                    "p99": round(float(np.percentile(lat, 99)), 1), "queries": 500, "cores": 1}
+        log(f"[bench] CPU oracle batch=1 latency p50 {cpu['batch1_latency_us']['p50']} us")
 
     out = {
         "metric": METRIC, "value": round(qps, 1), "unit": "QPS", "n_gpus": 1, "steps": a.steps,
@@ -411,7 +447,7 @@ def main():
             "workload": f"HNSW search, {a.n}x{a.dim} fp32 L2, M={a.M} (layer-0 degree {r0}), efC={a.efc}, "
                         f"{a.nq} queries per step, k={a.k}, recall@10>={a.target}",
             "ef": tuned_cfg["ef"], "mc": tuned_cfg["mc"], "mg": tuned_cfg.get("mg", 1),
-            "persistent": tuned_cfg["persistent"],
+            "persistent": tuned_cfg["persistent"], "grid_waves": tuned_cfg.get("grid_waves", 0),
             "recall_at_10": round(tuned_rec, 4), "plain_ef": plain_ef, "plain_recall_at_10": round(plain_rec, 4),
             "plain_qps": round(plain_qps, 1), "speedup_vs_plain": round(qps / plain_qps, 3),
             "avg_hops": round(float(avg[0]), 2), "avg_dist_evals": round(float(avg[1]), 2),

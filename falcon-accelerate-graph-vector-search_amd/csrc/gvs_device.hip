@@ -682,6 +682,96 @@ extern "C" int gvs_search_counters(gvs_index *ix, int64_t counters[3])
     return GVS_OK;
 }
 
+// ---- autotuner ----------------------------------------------------------
+// Times the search of the caller's device queries over a grid of launch
+// configurations and returns the fastest (the reference tunes (mc, mg) by
+// exhaustive kernel timing, perf_test_scripts/perf_test_throughput.py:83-133,
+// selection networked_FPGA/CPU_programs/launch_CPU_and_FPGA.py:172-183).
+extern "C" int gvs_autotune(gvs_index *ix, const float *queries, int64_t nq, int64_t ldq, const gvs_params *p,
+                            int64_t *labels, float *dists, void *stream, gvs_params *best, float *best_ms)
+{
+    if (!ix || !p || !best || !best_ms || nq <= 0 || !queries || !labels || !dists || ldq < ix->dim) {
+        gvs::set_error("gvs_autotune: invalid argument");
+        return GVS_ERR_ARG;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    const int rpc = (ix->r0 + WAVE - 1) / WAVE;
+    std::vector<gvs_params> cand;
+    for (int mc : {1, 2, 4, 8}) {
+        if (mc * rpc > MAX_ROWS) continue;
+        for (int mg : {1, 2})
+            for (int pers : {0, 1}) {
+                gvs_params c = *p;
+                c.mc = mc;
+                c.mg = mg;
+                c.persistent = pers;
+                c.waves_per_query = 1;
+                c.grid_waves = 0;
+                c.flags = (p->flags | GVS_FLAG_ASYNC);
+                cand.push_back(c);
+                if (pers) {
+                    gvs::LaunchPlan lp;
+                    if (gvs::plan_search(ix, &c, c.ef, c.k, &lp) == GVS_OK && lp.resident_waves >= 2 * lp.wpb) {
+                        c.grid_waves = lp.resident_waves / 2;  // half the resident waves
+                        cand.push_back(c);
+                    }
+                }
+            }
+        // several waves per query pay off only when there are few queries
+        if (nq <= 64 && p->storage == GVS_STORE_F32)
+            for (int W : {2, 4, 8}) {
+                gvs_params c = *p;
+                c.mc = mc;
+                c.mg = 1;
+                c.persistent = 0;
+                c.waves_per_query = W;
+                c.grid_waves = 0;
+                c.flags = (p->flags | GVS_FLAG_ASYNC);
+                cand.push_back(c);
+            }
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (!hip_ok(hipEventCreate(&e0), "hipEventCreate") || !hip_ok(hipEventCreate(&e1), "hipEventCreate")) {
+        if (e0) (void)hipEventDestroy(e0);
+        return GVS_ERR_HIP;
+    }
+    float best_t = -1.0f;
+    int rc = GVS_OK;
+    for (const gvs_params &c : cand) {
+        gvs::LaunchPlan lp;
+        if (gvs::plan_search(ix, &c, c.ef, c.k, &lp) != GVS_OK) continue;  // not compiled / unsupported
+        // one warm-up launch, then three timed ones
+        if ((rc = search_impl(ix, queries, nq, ldq, nullptr, &c, labels, dists, nullptr, st)) != GVS_OK) break;
+        if (!hip_ok(hipEventRecord(e0, st), "hipEventRecord")) {
+            rc = GVS_ERR_HIP;
+            break;
+        }
+        for (int r = 0; r < 3 && rc == GVS_OK; r++) rc = search_impl(ix, queries, nq, ldq, nullptr, &c, labels, dists, nullptr, st);
+        float ms = 0.0f;
+        if (rc != GVS_OK || !hip_ok(hipEventRecord(e1, st), "hipEventRecord") ||
+            !hip_ok(hipEventSynchronize(e1), "hipEventSynchronize") ||
+            !hip_ok(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime")) {
+            rc = rc ? rc : GVS_ERR_HIP;
+            break;
+        }
+        ms /= 3.0f;
+        if (best_t < 0.0f || ms < best_t) {
+            best_t = ms;
+            *best = c;
+            best->flags = p->flags;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    if (best_t < 0.0f) {
+        gvs::set_error("gvs_autotune: no configuration is supported for this index");
+        return GVS_ERR_UNSUPPORTED;
+    }
+    *best_ms = best_t;
+    return GVS_OK;
+}
+
 // ---- captured launches --------------------------------------------------
 struct gvs_search_graph {
     gvs_index *ix = nullptr;

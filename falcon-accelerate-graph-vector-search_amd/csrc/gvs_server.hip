@@ -78,6 +78,10 @@ struct gvs_server {
     std::mutex mu;
     std::vector<Slot> slots;
     std::vector<int> free_slots, inflight;
+    std::vector<int> inflight_pos;   // slot -> index in inflight
+    uint64_t *cring = nullptr;       // pinned completion ring (written by the device)
+    uint64_t chead = 0;              // completions consumed
+    double last_query_us = -1e30;    // last hipStreamQuery
     std::vector<uint8_t> ring_busy;  // ring entry still names an incomplete request
     uint64_t next_seq = 1, tail = 0;
     int64_t next_req = 0;
@@ -95,6 +99,7 @@ static void serve_free(gvs_server *s)
     (void)hipHostFree(s->qef);
     (void)hipHostFree(s->qseq);
     (void)hipHostFree(s->done);
+    (void)hipHostFree(s->cring);
     (void)hipHostFree(s->labels);
     (void)hipHostFree(s->dists);
     (void)hipHostFree(s->stats);
@@ -109,6 +114,11 @@ static void serve_free(gvs_server *s)
 static int serve_ensure_running(gvs_server *s)
 {
     if (s->launched) {
+        // the loop leaves only after 200 ms without work: looking at the
+        // stream every 100 us is enough and keeps the runtime off the path
+        const double t = now_us();
+        if (t - s->last_query_us < 100.0) return GVS_OK;
+        s->last_query_us = t;
         hipError_t q = hipStreamQuery(s->stream);
         if (q == hipErrorNotReady) return GVS_OK;
         if (q != hipSuccess) {
@@ -186,6 +196,7 @@ extern "C" int gvs_server_open(gvs_index *ix, const gvs_server_config *cfg, gvs_
               gvs::hip_ok(hipHostMalloc((void **)&s->qef, 4 * (size_t)R, hipHostMallocCoherent), "hipHostMalloc") &&
               gvs::hip_ok(hipHostMalloc((void **)&s->qseq, 8 * (size_t)R, hipHostMallocCoherent), "hipHostMalloc") &&
               gvs::hip_ok(hipHostMalloc((void **)&s->done, 8 * (size_t)R, hipHostMallocCoherent), "hipHostMalloc") &&
+              gvs::hip_ok(hipHostMalloc((void **)&s->cring, 8 * (size_t)rcap, hipHostMallocCoherent), "hipHostMalloc") &&
               gvs::hip_ok(hipHostMalloc((void **)&s->labels, 8 * (size_t)R * cfg->max_k, hipHostMallocCoherent), "hipHostMalloc") &&
               gvs::hip_ok(hipHostMalloc((void **)&s->dists, 4 * (size_t)R * cfg->max_k, hipHostMallocCoherent), "hipHostMalloc") &&
               gvs::hip_ok(hipHostMalloc((void **)&s->stats, 16 * (size_t)R, hipHostMallocCoherent), "hipHostMalloc") &&
@@ -203,6 +214,8 @@ extern "C" int gvs_server_open(gvs_index *ix, const gvs_server_config *cfg, gvs_
     }
     std::memset(s->ctl, 0, sizeof(ServeCtl));
     std::memset(s->done, 0, 8 * (size_t)R);
+    std::memset(s->cring, 0, 8 * (size_t)rcap);
+    s->inflight_pos.assign((size_t)R, -1);
     std::memset(s->qseq, 0, 8 * (size_t)R);
     s->slots.assign((size_t)R, Slot());
     s->ring_busy.assign(rcap, 0);
@@ -228,6 +241,8 @@ extern "C" int gvs_server_open(gvs_index *ix, const gvs_server_config *cfg, gvs_
     s->args.qef = s->qef;
     s->args.qseq = s->qseq;
     s->args.done = s->done;
+    s->args.cring = s->cring;
+    s->args.ccap = rcap;
     s->args.head = s->d_head;
     s->args.served = s->d_served;
     s->args.idle_ticks = 20 * 1000 * 1000;  // 200 ms at 100 MHz
@@ -265,6 +280,7 @@ static int64_t submit_locked(gvs_server *s, const float *query, int32_t k, int32
     sl.t_arrival = t_arrival;
     sl.busy = true;
     sl.cancelled = false;
+    s->inflight_pos[(size_t)slot] = (int)s->inflight.size();
     s->inflight.push_back(slot);
     sl.ring_idx = s->tail;
     s->ring_busy[s->tail & (uint64_t)(s->args.rcap - 1u)] = 1;
@@ -293,14 +309,19 @@ static int poll_locked(gvs_server *s, int32_t max_done, int64_t *req_ids, int64_
                        double *latency_us, int32_t *stats_out)
 {
     const int K = s->cfg.max_k;
+    const uint64_t cmask = (uint64_t)s->args.ccap - 1u;
     int n = 0;
-    for (size_t i = 0; i < s->inflight.size() && n < max_done;) {
-        const int slot = s->inflight[i];
-        const uint64_t d = __atomic_load_n(&s->done[slot], __ATOMIC_ACQUIRE);
-        if (d != s->qseq[slot]) {
-            i++;
-            continue;
+    // completions in the order the device finished them: entry i carries
+    // (i + 1) in its high word once it is written
+    while (n < max_done) {
+        const uint64_t v = __atomic_load_n(&s->cring[s->chead & cmask], __ATOMIC_ACQUIRE);
+        if ((uint32_t)(v >> 32) != (uint32_t)(s->chead + 1u)) break;
+        const int slot = (int)(uint32_t)v;
+        if (slot < 0 || slot >= s->cfg.max_inflight || !s->slots[(size_t)slot].busy) {
+            gvs::set_error("gvs_server_poll: completion ring names slot %d, which is not in flight", slot);
+            return GVS_ERR_STATE;
         }
+        s->chead++;
         const double t = now_us();
         Slot &sl = s->slots[(size_t)slot];
         if (!sl.cancelled) {
@@ -315,8 +336,12 @@ static int poll_locked(gvs_server *s, int32_t max_done, int64_t *req_ids, int64_
         sl.busy = false;
         s->ring_busy[sl.ring_idx & (uint64_t)(s->args.rcap - 1u)] = 0;
         s->free_slots.push_back(slot);
-        s->inflight[i] = s->inflight.back();
+        const int ip = s->inflight_pos[(size_t)slot];
+        const int last = s->inflight.back();
+        s->inflight[(size_t)ip] = last;
+        s->inflight_pos[(size_t)last] = ip;
         s->inflight.pop_back();
+        s->inflight_pos[(size_t)slot] = -1;
     }
     if (!s->inflight.empty()) {
         int rc = serve_ensure_running(s);  // a launch that went idle leaves work behind

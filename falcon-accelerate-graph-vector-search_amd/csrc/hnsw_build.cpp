@@ -20,6 +20,8 @@
 #include <mutex>
 #include <queue>
 #include <random>
+#include <cstdio>
+#include <cstdlib>
 #include <thread>
 #include <vector>
 
@@ -299,6 +301,9 @@ extern "C" int gvs_hnsw_build(const float *x, int64_t n, int32_t dim, int32_t me
             b.insert(0, tag, epoch);
         }
         std::atomic<int64_t> next(1);
+        // GVS_VERBOSE=1: one stderr line per tenth of the points inserted
+        const char *verbose = std::getenv("GVS_VERBOSE");
+        const int64_t report = (verbose && verbose[0] == '1') ? std::max<int64_t>(1, n / 10) : 0;
         auto work = [&]() {
             std::vector<uint32_t> tag((size_t)n, 0);
             uint32_t epoch = 0;
@@ -306,6 +311,9 @@ extern "C" int gvs_hnsw_build(const float *x, int64_t n, int32_t dim, int32_t me
                 int64_t i = next.fetch_add(1);
                 if (i >= n) break;
                 b.insert((uint32_t)i, tag, epoch);
+                if (report && (i + 1) % report == 0)
+                    std::fprintf(stderr, "gvs_hnsw_build: %lld of %lld points inserted\n", (long long)(i + 1),
+                                 (long long)n);
             }
         };
         if (nt == 1) {

@@ -24,8 +24,10 @@ struct ServeArgs {
     const int32_t *qef;      // pinned, per slot
     const uint64_t *qseq;    // pinned, per slot: sequence number of the request in the slot
     uint64_t *done;          // pinned, per slot: sequence number of the last completed request
+    uint64_t *cring;         // pinned completion ring: entry i = (i + 1) << 32 | slot of the i-th completion
+    uint32_t ccap;           // power of two, >= slots
     uint64_t *head;          // device: next request index to claim
-    uint64_t *served;        // device: completed requests
+    uint64_t *served;        // device: completed requests (completion ring position)
     int64_t idle_ticks;      // 100 MHz ticks without work before a wave leaves the loop
 };
 
@@ -97,7 +99,9 @@ __global__ __launch_bounds__(256, GVS_MIN_WAVES_PER_EU) void k_serve(ServeArgs A
         __threadfence_system();
         if (lane == 0) {
             __hip_atomic_store(&A.done[slot], seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
-            atomicAdd((unsigned long long *)A.served, 1ull);
+            const uint64_t pos = atomicAdd((unsigned long long *)A.served, 1ull);
+            __hip_atomic_store(&A.cring[pos & (uint64_t)(A.ccap - 1u)], ((pos + 1ull) << 32) | (uint64_t)slot,
+                               __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
         }
         last = wall_clock64();
     }

@@ -77,6 +77,7 @@ def _load():
         "gvs_search_host": (C.c_int, [P, P, I64, I64, P, P, P, P, P]),
         "gvs_search_overflow_count": (C.c_int, [P, P]),
         "gvs_search_counters": (C.c_int, [P, P]),
+        "gvs_autotune": (C.c_int, [P, P, I64, I64, P, P, P, P, P, P]),
         "gvs_search_graph_create": (C.c_int, [P, P, I64, I64, P, P, P, P, P, P]),
         "gvs_search_graph_launch": (C.c_int, [P, P]),
         "gvs_search_graph_destroy": (C.c_int, [P]),
@@ -108,7 +109,7 @@ def exported_symbols():
     names = ["gvs_last_error", "gvs_version", "gvs_hnsw_build", "gvs_nsg_from_hnsw", "gvs_hnswlib_load",
              "gvs_hnswlib_save", "gvs_nsg_load", "gvs_host_graph_view", "gvs_host_graph_free",
              "gvs_gen_mixture", "gvs_index_create", "gvs_index_destroy", "gvs_index_device_bytes",
-             "gvs_search_device", "gvs_search_host", "gvs_search_overflow_count", "gvs_search_counters", "gvs_search_graph_create",
+             "gvs_search_device", "gvs_search_host", "gvs_search_overflow_count", "gvs_search_counters", "gvs_autotune", "gvs_search_graph_create",
              "gvs_search_graph_launch", "gvs_search_graph_destroy", "gvs_device_copy", "gvs_bruteforce_f64_device", "gvs_server_open",
              "gvs_server_submit", "gvs_server_poll", "gvs_server_poll_stats", "gvs_server_cancel",
              "gvs_server_stats", "gvs_server_close", "gvs_server_loadgen"]
@@ -260,6 +261,18 @@ class DeviceIndex:
                                      C.c_void_p(labels_ptr), C.c_void_p(dists_ptr),
                                      C.c_void_p(stats_ptr) if stats_ptr else None,
                                      C.c_void_p(stream) if stream else None))
+
+    def autotune(self, q_ptr, nq, ldq, labels_ptr, dists_ptr, k=10, ef=64, stream=None, **kw):
+        """Fastest launch configuration for this batch -> (config dict, ms per launch)."""
+        p = make_params(k=k, ef=ef, **kw)
+        best = Params()
+        ms = C.c_float(0.0)
+        _check(lib.gvs_autotune(self._h, C.c_void_p(q_ptr), nq, ldq, C.byref(p), C.c_void_p(labels_ptr),
+                                C.c_void_p(dists_ptr), C.c_void_p(stream) if stream else None, C.byref(best),
+                                C.byref(ms)))
+        cfg = dict(ef=best.ef, mc=best.mc, mg=best.mg, persistent=bool(best.persistent),
+                   waves_per_query=best.waves_per_query, grid_waves=best.grid_waves)
+        return cfg, ms.value
 
     def capture(self, q_ptr, nq, ldq, labels_ptr, dists_ptr, k=10, ef=64, entries_ptr=None, stats_ptr=None, **kw):
         """Record the search of these device buffers into a hipGraph (SearchGraph)."""

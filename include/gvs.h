@@ -171,6 +171,16 @@ GVS_API int gvs_search_counters(gvs_index *ix, int64_t counters[3]);
  * pointers and size).  No reference counterpart; used to calibrate HBM byte
  * counters against a kernel of known traffic. */
 GVS_API int gvs_device_copy(void *dst, const void *src, int64_t bytes, void *stream);
+/* Autotuner: times the search of nq DEVICE queries (k, ef, storage, rerank
+ * and flags from *p; labels/dists are device result buffers it may overwrite)
+ * over mc in {1,2,4,8} (as r0 allows) x mg in {1,2} x persistent in {0,1}
+ * (persistent also at half the resident waves), and for nq <= 64 also
+ * waves_per_query in {2,4,8}; writes the fastest configuration to *best and
+ * its time per launch to *best_ms.  DST settings trade evaluations for steps
+ * and can change recall: checking recall is the caller's part.  Replaces the
+ * reference's exhaustive (mc, mg) timing, perf_test_scripts/perf_test_throughput.py:83-133. */
+GVS_API int gvs_autotune(gvs_index *ix, const float *queries, int64_t nq, int64_t ldq, const gvs_params *p,
+                         int64_t *labels, float *dists, void *stream, gvs_params *best, float *best_ms);
 /* Captured search: the kernels of one gvs_search_device call (same
  * arguments; the device buffers are fixed at capture time) recorded into a
  * hipGraph, with visited tables of its own.  Each launch replays the search

@@ -337,3 +337,22 @@ def test_intra_query_rejects_unsupported():
         dev.search(q, k=10, ef=64, mc=2, mg=2, waves_per_query=4)
     with pytest.raises(gvs.GvsError):
         dev.search(q, k=10, ef=64, waves_per_query=3)
+
+
+@pytest.mark.parametrize("nq", [16, 500])
+def test_autotuner_returns_a_working_configuration(nq):
+    """gvs_autotune picks a supported configuration; searching with it
+    matches the oracle's DST with the same mc and mg."""
+    from gvs import hip as H
+    x, g, dev = graph(20000, 128, 16, 100)
+    og = orc.OracleGraph(g.arrays())
+    q = queries(nq, 128, seed=60)
+    dq = H.DeviceArray.from_host(q)
+    dl = H.DeviceArray((nq, 10), np.int64)
+    dd = H.DeviceArray((nq, 10), np.float32)
+    print(f"case autotune nq={nq}", flush=True)
+    cfg, ms = dev.autotune(dq.address, nq, 128, dl.address, dd.address, k=10, ef=64)
+    assert ms > 0 and cfg["ef"] == 64 and cfg["mc"] in (1, 2, 4, 8) and cfg["mg"] in (1, 2)
+    assert cfg["waves_per_query"] in ((1, 2, 4, 8) if nq <= 64 else (1,))
+    kw = {kk: v for kk, v in cfg.items() if kk not in ("ef", "mc", "mg")}
+    check_equal(dev, og, q, 10, 64, mc=cfg["mc"], mg=cfg["mg"], **kw)
