@@ -81,6 +81,63 @@ def serving_leg(a, cache, ef):
     return json.loads(lines[-1]) if lines else None
 
 
+def precision_c4(a, gvs, g, x, q):
+    """Traversal over fp32 / BF16 / FP8-e4m3 copies of the vectors (fp32
+    accumulation), with and without an exact fp32 rerank of the final ef
+    candidates; recall@10 and returned-distance error against FP64."""
+    from gvs import hip as H
+    ef = 128
+    t0 = time.time()
+    nsg = g.to_nsg(64)
+    log(f"[c4] NSG-style graph (out-degree <= 64, navigating node {nsg.view.entry}) in {time.time() - t0:.1f}s")
+    dev = gvs.DeviceIndex(nsg, storages=(gvs.F32, gvs.BF16, gvs.FP8))
+    st = H.Stream()
+    nq = len(q)
+    dq = H.DeviceArray.from_host(q)
+    gti = H.DeviceArray((nq, a.k), np.int32)
+    gtd = H.DeviceArray((nq, a.k), np.float64)
+    t0 = time.time()
+    dev.bruteforce_f64_device(dq.address, nq, a.dim, a.k, gti.address, gtd.address, st.handle)
+    st.synchronize()
+    gt = gti.to_host()
+    log(f"[c4] FP64 ground truth {time.time() - t0:.1f}s")
+    dl = H.DeviceArray((nq, a.k), np.int64)
+    dd = H.DeviceArray((nq, a.k), np.float32)
+    rows = []
+    names = {gvs.F32: "fp32", gvs.BF16: "bf16", gvs.FP8: "fp8_e4m3"}
+    for storage, rerank in ((gvs.F32, False), (gvs.BF16, False), (gvs.BF16, True), (gvs.FP8, False),
+                            (gvs.FP8, True)):
+        kw = dict(k=a.k, ef=ef, storage=storage, rerank=rerank)
+        dev.search_device(dq.address, nq, a.dim, dl.address, dd.address, stream=st.handle, **kw)
+        st.synchronize()
+        lab, dist = dl.to_host(), dd.to_host()
+        e0, e1 = H.Event(), H.Event()
+        e0.record(st)
+        for _ in range(3):
+            dev.search_device(dq.address, nq, a.dim, dl.address, dd.address, stream=st.handle, asynchronous=True,
+                              **kw)
+        e1.record(st)
+        e1.synchronize()
+        ms = e0.elapsed_ms(e1) / 3
+        # FP64 distance of every returned id
+        ok = lab >= 0
+        xs = x[np.where(ok, lab, 0)].astype(np.float64)
+        d64 = ((xs - q[:, None, :].astype(np.float64)) ** 2).sum(-1)
+        rel = np.abs(dist.astype(np.float64) - d64) / np.maximum(np.abs(d64), 1e-12)
+        rel = rel[ok]
+        r = {"storage": names[storage], "rerank": rerank, "recall_at_10": round(recall_at(lab, gt, a.k), 4),
+             "qps": round(nq / (ms / 1e3), 1), "ms_per_batch": round(ms, 3),
+             "dist_rel_err_mean": float(f"{rel.mean():.3e}"), "dist_rel_err_max": float(f"{rel.max():.3e}")}
+        rows.append(r)
+        log(f"[c4] {r}")
+    vb = dev.device_bytes() if hasattr(dev, "device_bytes") else None
+    print(json.dumps({"metric": "recall@10 and distance error vs FP64, low-precision traversal + fp32 rerank",
+                      "config": {"workload": f"NSG-style flat graph from HNSW M={a.M} efC={a.efc} layer 0, "
+                                             f"out-degree <= 64, navigating-node entry; {a.n}x{a.dim} fp32 L2, "
+                                             f"{a.centres} centres; {nq} queries, k={a.k}, ef={ef}"},
+                      "device_bytes": vb, "rows": rows}), flush=True)
+
+
 def intra_latency(a, gvs, g, q):
     """Batch=1 latency: one query per launch through a captured graph, for
     waves per query x candidates per step x ef; p50/p99 over 190 queries
@@ -193,6 +250,10 @@ def main():
     ap.add_argument("--c5", action="store_true",
                     help="instead of the throughput run: serving on 1M x 384 unit-norm inner product, "
                          "Poisson arrivals at 25/50/75/90%% of peak, k in {1,10,100}, ef uniform in [32,512]")
+    ap.add_argument("--c4", action="store_true",
+                    help="instead of the throughput run: NSG-style flat graph (out-degree 64, navigating-node "
+                         "entry) on 10M x 128 (1024 centres), fp32 / BF16 / FP8-e4m3 traversal with and without "
+                         "exact fp32 rerank, ef=128, k=10: recall and distance error against FP64")
     ap.add_argument("--serve-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--serve-ef", type=int, default=64, help=argparse.SUPPRESS)
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "k_search_1Mx128_hbm_traffic.txt"),
@@ -211,6 +272,10 @@ def main():
 
     t0 = time.time()
     metric = gvs.L2
+    if a.c4:
+        if a.n == 1_000_000:
+            a.n = 10_000_000
+        a.centres = 1024
     if a.c5:
         a.dim = 384
         metric = gvs.IP
@@ -228,6 +293,9 @@ def main():
     log(f"[bench] build {build_s:.1f}s")
     if a.serve_child:
         serve_child(a, gvs, g, q)
+        return
+    if a.c4:
+        precision_c4(a, gvs, g, x, q)
         return
     if a.c5:
         if a.graph_cache and not os.path.exists(a.graph_cache):

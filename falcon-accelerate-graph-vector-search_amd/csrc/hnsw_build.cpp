@@ -387,32 +387,52 @@ extern "C" int gvs_nsg_from_hnsw(const gvs_host_graph *from, int32_t R, gvs_host
     g->r0 = R;
     g->adj0.assign((size_t)n * R, -1);
     auto vec = [&](int64_t i) { return from->vectors.data() + (size_t)i * dim; };
+    const bool hier = from->max_level > 0 && !from->upper_off.empty() && from->levels.size() == (size_t)n;
     for (int64_t i = 0; i < n; i++) {
         const int32_t *row = &from->adj0[(size_t)i * from->r0];
         int cnt = 0;
         while (cnt < from->r0 && row[cnt] >= 0) cnt++;
-        if (cnt <= R) {
-            for (int j = 0; j < cnt; j++) g->adj0[(size_t)i * R + j] = row[j];
-            continue;
-        }
-        std::vector<std::pair<float, int32_t>> c;
-        for (int j = 0; j < cnt; j++) c.emplace_back(dist_fast(vec(i), vec(row[j]), dim, from->metric), row[j]);
-        std::sort(c.begin(), c.end());
+        // Long-range edges: a node of level L >= 1 keeps its upper-layer
+        // links (top level first, at most R/2).  A flat graph searched from
+        // one navigating node needs them: layer 0 alone only links near
+        // neighbours, so greedy search from one entry stalls between
+        // clusters.  (NSG gets the same property from its construction.)
         std::vector<int32_t> keep;
-        for (auto &p : c) {
-            if ((int)keep.size() >= R) break;
-            bool good = true;
-            for (int32_t s : keep)
-                if (dist_fast(vec(s), vec(p.second), dim, from->metric) < p.first) {
-                    good = false;
-                    break;
-                }
-            if (good) keep.push_back(p.second);
+        auto has = [&](int32_t u) { return std::find(keep.begin(), keep.end(), u) != keep.end(); };
+        if (hier && from->levels[(size_t)i] > 0) {
+            for (int l = from->levels[(size_t)i]; l >= 1 && (int)keep.size() < R / 2; l--) {
+                const int32_t *urow = &from->adj_up[(size_t)(from->upper_off[(size_t)i] + l - 1) * from->ru];
+                for (int j = 0; j < from->ru && (int)keep.size() < R / 2; j++)
+                    if (urow[j] >= 0 && urow[j] != (int32_t)i && !has(urow[j])) keep.push_back(urow[j]);
+            }
         }
-        // fill up to R with the nearest pruned ones (fixed out-degree)
-        for (auto &p : c) {
-            if ((int)keep.size() >= R) break;
-            if (std::find(keep.begin(), keep.end(), p.second) == keep.end()) keep.push_back(p.second);
+        std::vector<int32_t> rest;
+        for (int j = 0; j < cnt; j++)
+            if (!has(row[j])) rest.push_back(row[j]);
+        if ((int)(keep.size() + rest.size()) <= R) {
+            keep.insert(keep.end(), rest.begin(), rest.end());
+        } else {
+            // occlusion rule over the layer-0 candidates (closest first): a
+            // candidate closer to an already kept neighbour than to the node
+            // is skipped; then fill up to R with the nearest skipped ones
+            std::vector<std::pair<float, int32_t>> c;
+            for (int32_t u : rest) c.emplace_back(dist_fast(vec(i), vec(u), dim, from->metric), u);
+            std::sort(c.begin(), c.end());
+            const size_t base = keep.size();
+            for (auto &p : c) {
+                if ((int)keep.size() >= R) break;
+                bool good = true;
+                for (size_t t = base; t < keep.size(); t++)
+                    if (dist_fast(vec(keep[t]), vec(p.second), dim, from->metric) < p.first) {
+                        good = false;
+                        break;
+                    }
+                if (good) keep.push_back(p.second);
+            }
+            for (auto &p : c) {
+                if ((int)keep.size() >= R) break;
+                if (!has(p.second)) keep.push_back(p.second);
+            }
         }
         for (size_t j = 0; j < keep.size(); j++) g->adj0[(size_t)i * R + j] = keep[j];
     }
@@ -422,9 +442,12 @@ extern "C" int gvs_nsg_from_hnsw(const gvs_host_graph *from, int32_t R, gvs_host
         for (int d = 0; d < dim; d++) cen[(size_t)d] += vec(i)[d];
     std::vector<float> cf((size_t)dim);
     for (int d = 0; d < dim; d++) cf[(size_t)d] = (float)(cen[(size_t)d] / (double)n);
+    // the navigating node: the node nearest to the centroid, among the nodes
+    // with long-range edges when the source graph has upper layers
     float best = INFINITY;
     int32_t bi = 0;
     for (int64_t i = 0; i < n; i++) {
+        if (hier && from->levels[(size_t)i] == 0) continue;
         float d = dist_fast(cf.data(), vec(i), dim, GVS_METRIC_L2);
         if (d < best) {
             best = d;

@@ -44,6 +44,9 @@ __device__ __forceinline__ int32_t ld_sys32i(const int32_t *p)
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
+// waves that stay while requests flow (one per CU on MI355X)
+constexpr int64_t SERVE_KEEP = 256;
+
 #ifndef GVS_MIN_WAVES_PER_EU
 #define GVS_MIN_WAVES_PER_EU 1
 #endif
@@ -64,6 +67,12 @@ __global__ __launch_bounds__(256, GVS_MIN_WAVES_PER_EU) void k_serve(ServeArgs A
         uint32_t state = 0u;
         uint64_t item = 0ull;
         if (lane == 0) {
+            // Idle waves poll the host ring with exponential back-off (up to
+            // 32 sleeps of ~4 us): every idle wave's poll is a read across
+            // the bus, and thousands of them at full rate load the host's
+            // memory; with many waves some wave still looks every few ns.
+            int backoff = 0;
+            uint64_t seen_h = ~0ull;
             for (;;) {
                 if (ld_sys32(&A.ctl->stop)) {
                     state = 1u;
@@ -71,6 +80,17 @@ __global__ __launch_bounds__(256, GVS_MIN_WAVES_PER_EU) void k_serve(ServeArgs A
                 }
                 const uint64_t t = ld_sys64(&A.ctl->tail);
                 const uint64_t h = __hip_atomic_load(A.head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                // Idle rule.  Every idle wave's poll is an acquire across the
+                // bus (it invalidates its XCD's L2), so idle waves should not
+                // linger: waves gw >= SERVE_KEEP leave after idle_ticks
+                // without work of their own; the first SERVE_KEEP waves run
+                // their clock from the last claim by ANY wave, so they stay
+                // while requests flow and leave together once the loop is
+                // idle.  The pool never shrinks below SERVE_KEEP waves.
+                if (gw < SERVE_KEEP && h != seen_h) {
+                    seen_h = h;
+                    last = wall_clock64();
+                }
                 if (h < t) {
                     if (atomicCAS((unsigned long long *)A.head, (unsigned long long)h, (unsigned long long)(h + 1)) == h) {
                         item = h;
@@ -82,7 +102,8 @@ __global__ __launch_bounds__(256, GVS_MIN_WAVES_PER_EU) void k_serve(ServeArgs A
                     state = 2u;
                     break;
                 }
-                __builtin_amdgcn_s_sleep(8);
+                for (int i = 0; i < (1 << backoff); i++) __builtin_amdgcn_s_sleep(127);
+                if (backoff < 5) backoff++;
             }
         }
         state = __builtin_amdgcn_readfirstlane(state);

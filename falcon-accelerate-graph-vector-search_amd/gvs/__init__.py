@@ -262,6 +262,12 @@ class DeviceIndex:
                                      C.c_void_p(stats_ptr) if stats_ptr else None,
                                      C.c_void_p(stream) if stream else None))
 
+    def device_bytes(self):
+        """Bytes resident on the device: {"graph": ..., "vectors": ...}."""
+        gb, vb = C.c_int64(0), C.c_int64(0)
+        _check(lib.gvs_index_device_bytes(self._h, C.byref(gb), C.byref(vb)))
+        return {"graph": gb.value, "vectors": vb.value}
+
     def autotune(self, q_ptr, nq, ldq, labels_ptr, dists_ptr, k=10, ef=64, stream=None, **kw):
         """Fastest launch configuration for this batch -> (config dict, ms per launch)."""
         p = make_params(k=k, ef=ef, **kw)

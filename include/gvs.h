@@ -114,10 +114,14 @@ GVS_API const char *gvs_version(void);
 GVS_API int gvs_hnsw_build(const float *x, int64_t n, int32_t dim, int32_t metric,
                            int32_t M, int32_t ef_construction, uint64_t seed,
                            int32_t nthreads, int32_t exact_order, gvs_host_graph **out);
-/* NSG-style flat graph: prune the HNSW layer-0 lists of `from` to out-degree
- * <= R with the same occlusion rule and set the navigating node (the node
- * nearest to the dataset centroid) as entry; vector_search_baselines/scripts_nsg/nsg_to_FPGA.py:56-137
- * defines the format this mirrors (fixed-width rows, single entry point). */
+/* NSG-style flat graph from an HNSW graph: one row of <= R neighbours per
+ * node and a single navigating entry node, the form the FPGA path searches
+ * (vector_search_baselines/scripts_nsg/nsg_to_FPGA.py:56-137: fixed-width
+ * rows, one entry point).  A node of HNSW level >= 1 keeps up to R/2 of its
+ * upper-layer links as long-range edges; the rest of each row is its
+ * layer-0 list, pruned with the occlusion rule when it does not fit.  The
+ * navigating node is the node nearest to the dataset centroid (among nodes
+ * of level >= 1 when `from` has upper layers). */
 GVS_API int gvs_nsg_from_hnsw(const gvs_host_graph *from, int32_t R, gvs_host_graph **out);
 /* hnswlib .bin reader / writer (HNSWLIB/hnswalg.h:584-700). */
 GVS_API int gvs_hnswlib_load(const char *path, int32_t dim, int32_t metric, gvs_host_graph **out);
