@@ -52,6 +52,8 @@ def read_traffic(path, cfg):
     key = f"ef={cfg['ef']} mc={cfg['mc']} mg={cfg.get('mg', 1)} persistent={int(bool(cfg['persistent']))}"
     if cfg.get("grid_waves", 0):
         key += f" grid_waves={cfg['grid_waves']}"
+    if cfg.get("region_order"):
+        key += " region_order=1"
     for line in open(path):
         line = line.strip()
         if line.startswith("traffic ") and key in line:
@@ -365,6 +367,8 @@ def main():
                      persistent=bool(int(kv.get("persistent", 0))))
         if int(kv.get("grid_waves", 0)):
             fixed["grid_waves"] = int(kv["grid_waves"])
+        if int(kv.get("region_order", 0)):
+            fixed["region_order"] = True
         efs = [fixed["ef"]]
     plain_ef, plain_rec = None, None
     sweep = []
@@ -389,7 +393,8 @@ def main():
     for ef in ([] if a.config else [e for e in efs if e <= plain_ef]):
         cfg, ms = dev.autotune(dq.address, a.nq, a.dim, dlab.address, ddist.address, k=a.k, ef=ef,
                                stream=stream.handle)
-        cfg = {kk: v for kk, v in cfg.items() if not (kk == "grid_waves" and v == 0) and kk != "waves_per_query"}
+        cfg = {kk: v for kk, v in cfg.items()
+               if not (kk in ("grid_waves", "region_order") and not v) and kk != "waves_per_query"}
         rec = measure_recall(cfg)
         log(f"[bench] autotune ef={ef}: {cfg} recall={rec:.4f} {ms:.3f} ms")
         if rec < a.target:
@@ -516,6 +521,7 @@ def main():
                         f"{a.nq} queries per step, k={a.k}, recall@10>={a.target}",
             "ef": tuned_cfg["ef"], "mc": tuned_cfg["mc"], "mg": tuned_cfg.get("mg", 1),
             "persistent": tuned_cfg["persistent"], "grid_waves": tuned_cfg.get("grid_waves", 0),
+            "region_order": bool(tuned_cfg.get("region_order", False)),
             "recall_at_10": round(tuned_rec, 4), "plain_ef": plain_ef, "plain_recall_at_10": round(plain_rec, 4),
             "plain_qps": round(plain_qps, 1), "speedup_vs_plain": round(qps / plain_qps, 3),
             "avg_hops": round(float(avg[0]), 2), "avg_dist_evals": round(float(avg[1]), 2),

@@ -7,6 +7,8 @@
 #include <mutex>
 #include <vector>
 
+#include <rocprim/device/device_radix_sort.hpp>
+
 #include "gvs_internal.hpp"
 #include "search_kernels.hpp"
 
@@ -19,7 +21,7 @@ using namespace gvsk;
 // launch sequence can be captured in a hipGraph)
 __global__ void k_reset_counters(uint32_t *c)
 {
-    if (threadIdx.x < 4) c[threadIdx.x] = 0u;
+    if (threadIdx.x < 12) c[threadIdx.x] = 0u;
 }
 
 // plain streaming copy (16 bytes per lane per iteration)
@@ -27,6 +29,15 @@ __global__ __launch_bounds__(256) void k_copy16(const uint4 *__restrict__ src, u
 {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (int64_t)gridDim.x * blockDim.x)
         dst[i] = src[i];
+}
+
+static dfn_t pick_descend(int storage, int metric, int nch)
+{
+    const bool l2 = metric == GVS_METRIC_L2;
+    if (storage == GVS_STORE_F32) return l2 ? gvs_descend_f32_l2(nch) : gvs_descend_f32_ip(nch);
+    if (storage == GVS_STORE_BF16) return l2 ? gvs_descend_bf16_l2(nch) : gvs_descend_bf16_ip(nch);
+    if (storage == GVS_STORE_FP8) return l2 ? gvs_descend_fp8_l2(nch) : gvs_descend_fp8_ip(nch);
+    return nullptr;
 }
 
 static kfn_t pick_kernel(int storage, int metric, int nch, int rows)
@@ -106,6 +117,10 @@ static void free_index(gvs_index *ix)
     (void)hipFree(ix->d_counters);
     (void)hipFree(ix->d_status);
     (void)hipFree(ix->d_xnorm64);
+    (void)hipFree(ix->d_rep);
+    (void)hipFree(ix->d_rkey);
+    (void)hipFree(ix->d_rperm);
+    (void)hipFree(ix->d_rtmp);
     delete ix;
 }
 
@@ -324,6 +339,7 @@ int plan_search(const gvs_index *ix, const gvs_params *p, int max_ef, int max_k,
     }
     int efe = std::max(max_ef, max_k);
     lp->fn = (void *)fn;
+    lp->dfn = W == 1 ? (void *)pick_descend(p->storage, ix->metric, nch) : nullptr;
     lp->ef_cap = (efe + 63) / 64 * 64;
     lp->lcap = p->mc * rpc * WAVE;
     lp->blog2 = p->bloom_log2 > 0 ? p->bloom_log2 : std::min(12, std::max(8, ilog2_ceil((int64_t)efe * 8)));
@@ -472,8 +488,49 @@ int64_t scratch_slots(const gvs_params *p, const LaunchPlan &lp, int64_t nq)
 // Enqueue the search on `stream`.  vtab holds scratch_slots() tables of
 // 2^vlog2 words (all zero; every query leaves its table zero), status nq
 // words.  Enqueues kernels only, so it can be captured into a graph.
+// Region order applies to persistent searches of whole batches that run the
+// upper-layer descent on a graph with at least two upper levels.
+static bool region_order(const gvs_index *ix, const gvs_params *p, const LaunchPlan &lp, const KParams &P,
+                         int64_t nq)
+{
+    return (p->flags & GVS_FLAG_REGION_ORDER) && p->persistent && p->waves_per_query == 1 && !P.entries &&
+           !P.q_index && ix->max_level >= 2 && lp.dfn && nq >= 2 * lp.wpb;
+}
+
+// entries, keys and permutation (double-buffered for the sort) for nq queries
+static int ensure_region_scratch(gvs_index *ix, int64_t nq)
+{
+    if (nq > ix->rcap) {
+        (void)hipFree(ix->d_rep);
+        (void)hipFree(ix->d_rkey);
+        (void)hipFree(ix->d_rperm);
+        ix->d_rep = nullptr;
+        ix->d_rkey = nullptr;
+        ix->d_rperm = nullptr;
+        ix->rcap = 0;
+        if (!hip_ok(hipMalloc(&ix->d_rep, sizeof(int32_t) * nq), "hipMalloc region entries") ||
+            !hip_ok(hipMalloc(&ix->d_rkey, 2 * sizeof(uint64_t) * nq), "hipMalloc region keys") ||
+            !hip_ok(hipMalloc(&ix->d_rperm, 2 * sizeof(int32_t) * nq), "hipMalloc region order"))
+            return GVS_ERR_NOMEM;
+        ix->rcap = nq;
+    }
+    size_t bytes = 0;
+    if (!hip_ok(rocprim::radix_sort_pairs(nullptr, bytes, ix->d_rkey, ix->d_rkey + ix->rcap, ix->d_rperm,
+                                         ix->d_rperm + ix->rcap, (unsigned int)nq, 0, 62),
+                "rocprim::radix_sort_pairs (size)"))
+        return GVS_ERR_HIP;
+    if (bytes > ix->rtmp_bytes) {
+        (void)hipFree(ix->d_rtmp);
+        ix->d_rtmp = nullptr;
+        ix->rtmp_bytes = 0;
+        if (!hip_ok(hipMalloc(&ix->d_rtmp, bytes), "hipMalloc sort storage")) return GVS_ERR_NOMEM;
+        ix->rtmp_bytes = bytes;
+    }
+    return GVS_OK;
+}
+
 int launch_search(gvs_index *ix, const gvs_params *p, const LaunchPlan &lp, KParams P, int64_t nq,
-                  hipStream_t stream, uint32_t *vtab, int32_t *status)
+                  hipStream_t stream, uint32_t *vtab, int32_t *status, bool region)
 {
     kfn_t fn = (kfn_t)lp.fn;
     // (with several waves per query, lds_per_wave is the whole workgroup's)
@@ -499,6 +556,21 @@ int launch_search(gvs_index *ix, const gvs_params *p, const LaunchPlan &lp, KPar
         P.nq = nq;
         P.q_base = 0;
         hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, stream, ix->d_counters);
+        if (region) {
+            // descent, then sort the batch by region, then layer 0 in that order
+            hipLaunchKernelGGL((dfn_t)lp.dfn, dim3((unsigned)((nq + 3) / 4)), dim3(256), 0, stream, P, ix->d_rep,
+                               ix->d_rkey, ix->d_rperm);
+            if (!hip_ok(hipGetLastError(), "launch k_descend")) return GVS_ERR_HIP;
+            size_t bytes = ix->rtmp_bytes;
+            if (!hip_ok(rocprim::radix_sort_pairs(ix->d_rtmp, bytes, ix->d_rkey, ix->d_rkey + ix->rcap, ix->d_rperm,
+                                                 ix->d_rperm + ix->rcap, (unsigned int)nq, 0, 62, stream),
+                        "rocprim::radix_sort_pairs"))
+                return GVS_ERR_HIP;
+            P.entries = ix->d_rep;
+            P.q_perm = ix->d_rperm + ix->rcap;
+            P.keep_upper_stats = 1;
+            P.xcd_queues = 1;
+        }
         hipLaunchKernelGGL(fn, dim3((unsigned)(slots / lp.wpb)), dim3(lp.wpb * WAVE), shm, stream, P);
         if (!hip_ok(hipGetLastError(), "launch k_search (persistent)")) return GVS_ERR_HIP;
         return GVS_OK;
@@ -522,7 +594,9 @@ int launch_search_ix(gvs_index *ix, const gvs_params *p, const LaunchPlan &lp, c
 {
     int rc = ensure_scratch(ix, scratch_slots(p, lp, nq), lp.vlog2, nq);
     if (rc) return rc;
-    return launch_search(ix, p, lp, P, nq, stream, ix->d_vtab, ix->d_status);
+    const bool region = region_order(ix, p, lp, P, nq);
+    if (region && (rc = ensure_region_scratch(ix, nq))) return rc;
+    return launch_search(ix, p, lp, P, nq, stream, ix->d_vtab, ix->d_status, region);
 }
 
 }  // namespace gvs
@@ -709,6 +783,11 @@ extern "C" int gvs_autotune(gvs_index *ix, const float *queries, int64_t nq, int
                 c.grid_waves = 0;
                 c.flags = (p->flags | GVS_FLAG_ASYNC);
                 cand.push_back(c);
+                if (pers && ix->max_level >= 2 && nq >= 4096) {
+                    gvs_params r = c;
+                    r.flags |= GVS_FLAG_REGION_ORDER;
+                    cand.push_back(r);
+                }
                 if (pers) {
                     gvs::LaunchPlan lp;
                     if (gvs::plan_search(ix, &c, c.ef, c.k, &lp) == GVS_OK && lp.resident_waves >= 2 * lp.wpb) {
@@ -758,7 +837,7 @@ extern "C" int gvs_autotune(gvs_index *ix, const float *queries, int64_t nq, int
         if (best_t < 0.0f || ms < best_t) {
             best_t = ms;
             *best = c;
-            best->flags = p->flags;
+            best->flags = (p->flags & ~GVS_FLAG_REGION_ORDER) | (c.flags & GVS_FLAG_REGION_ORDER);
         }
     }
     (void)hipEventDestroy(e0);
@@ -822,7 +901,7 @@ extern "C" int gvs_search_graph_create(gvs_index *ix, const float *queries, int6
               hip_ok(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking), "hipStreamCreateWithFlags") &&
               hip_ok(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
     if (ok) {
-        rc = gvs::launch_search(ix, p, lp, P, nq, cs, g->vtab, g->status);
+        rc = gvs::launch_search(ix, p, lp, P, nq, cs, g->vtab, g->status, false);
         hipError_t e = hipStreamEndCapture(cs, &g->graph);
         ok = rc == GVS_OK && hip_ok(e, "hipStreamEndCapture") &&
              hip_ok(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0), "hipGraphInstantiate");

@@ -29,6 +29,13 @@ struct gvs_index {
     int32_t *d_status = nullptr;
     int64_t status_cap = 0;
     double *d_xnorm64 = nullptr;   // |x|^2 in fp64 for the brute-force GT kernel
+    // region order scratch (grown on demand): entries, keys, permutation, sort storage
+    int32_t *d_rep = nullptr;
+    uint64_t *d_rkey = nullptr;    // 2 * rcap
+    int32_t *d_rperm = nullptr;    // 2 * rcap
+    void *d_rtmp = nullptr;
+    int64_t rcap = 0;
+    size_t rtmp_bytes = 0;
 };
 
 namespace gvs {
@@ -36,6 +43,7 @@ bool hip_ok(hipError_t e, const char *what);
 
 struct LaunchPlan {
     void *fn = nullptr;
+    void *dfn = nullptr;           // upper-layer descent kernel (region order)
     int ef_cap = 0, lcap = 0, blog2 = 0, vlog2 = 0, hlog2 = 0, lds_per_wave = 0, wpb = 1;
     int64_t resident_waves = 0;
 };

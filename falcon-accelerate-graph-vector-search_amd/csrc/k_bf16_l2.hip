@@ -15,3 +15,16 @@ kfn_t gvs_kernel_bf16_l2(int nch, int rows)
     default: return nullptr;
     }
 }
+
+dfn_t gvs_descend_bf16_l2(int nch)
+{
+    switch (nch) {
+    case 1: return k_descend<1, STORE_BF16, METRIC_L2>;
+    case 2: return k_descend<2, STORE_BF16, METRIC_L2>;
+    case 3: return k_descend<3, STORE_BF16, METRIC_L2>;
+    case 4: return k_descend<4, STORE_BF16, METRIC_L2>;
+    case 6: return k_descend<6, STORE_BF16, METRIC_L2>;
+    case 8: return k_descend<8, STORE_BF16, METRIC_L2>;
+    default: return nullptr;
+    }
+}

@@ -32,3 +32,18 @@ kfn_t gvs_kernel_mw_f32_ip(int nch, int rows_per_wave)
     default: return nullptr;
     }
 }
+
+dfn_t gvs_descend_f32_ip(int nch)
+{
+    switch (nch) {
+    case 1: return k_descend<1, STORE_F32, METRIC_IP>;
+    case 2: return k_descend<2, STORE_F32, METRIC_IP>;
+    case 3: return k_descend<3, STORE_F32, METRIC_IP>;
+    case 4: return k_descend<4, STORE_F32, METRIC_IP>;
+    case 6: return k_descend<6, STORE_F32, METRIC_IP>;
+    case 8: return k_descend<8, STORE_F32, METRIC_IP>;
+    case 12: return k_descend<12, STORE_F32, METRIC_IP>;
+    case 16: return k_descend<16, STORE_F32, METRIC_IP>;
+    default: return nullptr;
+    }
+}

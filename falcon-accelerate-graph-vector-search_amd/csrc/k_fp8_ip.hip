@@ -13,3 +13,14 @@ kfn_t gvs_kernel_fp8_ip(int nch, int rows)
     default: return nullptr;
     }
 }
+
+dfn_t gvs_descend_fp8_ip(int nch)
+{
+    switch (nch) {
+    case 1: return k_descend<1, STORE_FP8, METRIC_IP>;
+    case 2: return k_descend<2, STORE_FP8, METRIC_IP>;
+    case 3: return k_descend<3, STORE_FP8, METRIC_IP>;
+    case 4: return k_descend<4, STORE_FP8, METRIC_IP>;
+    default: return nullptr;
+    }
+}

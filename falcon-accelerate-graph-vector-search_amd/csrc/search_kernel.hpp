@@ -76,6 +76,11 @@ struct KParams {
     const int32_t *q_k;
     const int32_t *q_ef;
     const int64_t *q_index;    // serving: slot -> query row; NULL = identity
+    // region order (persistent mode): work item w is query q_perm[w]; the
+    // entries and the upper-layer counters come from the descent kernel
+    const int32_t *q_perm;
+    int keep_upper_stats;      // leave out_stats[.., 2..3] as the descent wrote them
+    int xcd_queues;            // 8 work queues (counters[4..11]), one per XCD segment of the order
 };
 
 __device__ __forceinline__ uint32_t ord_f(float f)
@@ -107,6 +112,13 @@ __device__ __forceinline__ void wave_sync()
 }
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+// XCD (accelerator complex die) this wave runs on: HW_REG_XCC_ID (hwreg 20),
+// bits 3:0.  Used only to pick a work queue; any value is correct.
+__device__ __forceinline__ uint32_t xcc_id()
+{
+    return (uint32_t)__builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20);
+}
 
 __device__ __forceinline__ int mbcnt(uint64_t m)
 {

@@ -16,14 +16,31 @@ __global__ __launch_bounds__(256, GVS_MIN_WAVES_PER_EU) void k_search(KParams P)
     const int wib = threadIdx.x >> 6;
     unsigned char *lds = smem + (size_t)wib * P.lds_per_wave;
     const int64_t gw = (int64_t)blockIdx.x * P.waves_per_block + wib;
-    if (P.persistent) {
+    if (P.persistent && P.xcd_queues) {
+        // region order: the ordered work is cut into 8 segments, one per XCD
+        // (so the queries a XCD runs at once share its L2); a wave drains
+        // its XCD's segment, then helps with the others
+        const int x0 = (int)(xcc_id() & 7u);
+        for (int t = 0; t < 8; t++) {
+            const int x = (x0 + t) & 7;
+            const int64_t s0 = P.nq * x / 8, s1 = P.nq * (x + 1) / 8;
+            for (;;) {
+                uint32_t w = 0;
+                if (lane_id() == 0) w = atomicAdd(&P.counters[4 + x], 1u);
+                w = __builtin_amdgcn_readfirstlane(w);
+                if (s0 + (int64_t)w >= s1) break;
+                const int64_t item = s0 + (int64_t)w;
+                search_query<NCH, STORE, METRIC, MROWS>(P, P.q_perm ? (int64_t)P.q_perm[item] : item, gw, lds);
+            }
+        }
+    } else if (P.persistent) {
         // device work queue: each wave takes the next query index
         for (;;) {
             uint32_t q = 0;
             if (lane_id() == 0) q = atomicAdd(&P.counters[0], 1u);
             q = __builtin_amdgcn_readfirstlane(q);
             if ((int64_t)q >= P.nq) break;
-            search_query<NCH, STORE, METRIC, MROWS>(P, (int64_t)q, gw, lds);
+            search_query<NCH, STORE, METRIC, MROWS>(P, P.q_perm ? (int64_t)P.q_perm[q] : (int64_t)q, gw, lds);
         }
     } else {
         const int64_t qi = P.q_base + gw;
@@ -31,6 +48,48 @@ __global__ __launch_bounds__(256, GVS_MIN_WAVES_PER_EU) void k_search(KParams P)
         search_query<NCH, STORE, METRIC, MROWS>(P, qi, gw, lds);
     }
 }
+
+// Upper-layer descent only, one wave per query: the layer-0 entry, the
+// upper-layer counters (into out_stats[.., 2..3]) and a region key from the
+// nodes the descent settled on at levels 3 and 2, for ordering the batch.
+template <int NCH, int STORE, int METRIC>
+__global__ __launch_bounds__(256) void k_descend(KParams P, int32_t *ep_out, uint64_t *key_out, int32_t *perm_out)
+{
+    constexpr int E = StoreTraits<STORE>::E;
+    __shared__ uint64_t keys[4][WAVE];
+    __shared__ uint32_t ids[4][WAVE];
+    const int wib = threadIdx.x >> 6;
+    const int64_t qi = (int64_t)blockIdx.x * 4 + wib;
+    if (qi >= P.nq) return;
+    WaveLds L;
+    L.lkeys = keys[wib];
+    L.lids = ids[wib];
+    const int lane = lane_id(), p = lane & 7;
+    const float *Q = P.queries + qi * P.ldq;
+    float qf[NCH * E];
+#pragma unroll
+    for (int i = 0; i < NCH; i++)
+#pragma unroll
+        for (int t = 0; t < E; t++) {
+            const int idx = (p + 8 * i) * E + t;
+            qf[i * E + t] = idx < P.dim ? Q[idx] : 0.0f;
+        }
+    uint32_t ep, n2, n3;
+    float epd;
+    int up_hops = 0, up_evals = 0;
+    find_entry_path<NCH, STORE, METRIC>(P, qi, L, qf, ep, epd, up_hops, up_evals, n2, n3);
+    if (lane == 0) {
+        ep_out[qi] = (int32_t)ep;
+        key_out[qi] = ((uint64_t)(n3 & 0x7FFFFFFFu) << 31) | (uint64_t)(n2 & 0x7FFFFFFFu);
+        perm_out[qi] = (int32_t)qi;
+        if (P.out_stats) {
+            P.out_stats[qi * 4 + 2] = up_hops;
+            P.out_stats[qi * 4 + 3] = up_evals;
+        }
+    }
+}
+
+typedef void (*dfn_t)(KParams, int32_t *, uint64_t *, int32_t *);
 
 // rows = mc * ceil(r0 / 64) rounded up to 1, 2 or 8
 template <int NCH, int STORE, int METRIC>
@@ -70,6 +129,13 @@ gvsk::kfn_t gvs_kernel_bf16_l2(int nch, int rows);
 gvsk::kfn_t gvs_kernel_bf16_ip(int nch, int rows);
 gvsk::kfn_t gvs_kernel_fp8_l2(int nch, int rows);
 gvsk::kfn_t gvs_kernel_fp8_ip(int nch, int rows);
+// upper-layer descent (region order)
+gvsk::dfn_t gvs_descend_f32_l2(int nch);
+gvsk::dfn_t gvs_descend_f32_ip(int nch);
+gvsk::dfn_t gvs_descend_bf16_l2(int nch);
+gvsk::dfn_t gvs_descend_bf16_ip(int nch);
+gvsk::dfn_t gvs_descend_fp8_l2(int nch);
+gvsk::dfn_t gvs_descend_fp8_ip(int nch);
 // several waves per query (fp32 storage)
 gvsk::kfn_t gvs_kernel_mw_f32_l2(int nch, int rows_per_wave);
 gvsk::kfn_t gvs_kernel_mw_f32_ip(int nch, int rows_per_wave);

@@ -318,9 +318,12 @@ __device__ __forceinline__ int expand_lds(const KParams &P, const WaveLds &L, co
 // Layer-0 entry point and its distance: the given entry, or hnswlib's
 // upper-layer greedy descent (HNSWLIB/hnswalg.h:1119-1147): from the top
 // level down, move to the closest neighbour while it is strictly closer.
+// Also returns the nodes the descent settled on at levels 3 and 2 (the entry
+// when the graph has fewer levels): a coarse address of the query's region.
 template <int NCH, int STORE, int METRIC>
-__device__ __forceinline__ void find_entry(const KParams &P, int64_t qi, const WaveLds &L, const float *qf,
-                                           uint32_t &ep, float &epd, int &up_hops, int &up_evals)
+__device__ __forceinline__ void find_entry_path(const KParams &P, int64_t qi, const WaveLds &L, const float *qf,
+                                                uint32_t &ep, float &epd, int &up_hops, int &up_evals,
+                                                uint32_t &n2, uint32_t &n3)
 {
     const int lane = lane_id();
     if (P.entries) {
@@ -330,9 +333,11 @@ __device__ __forceinline__ void find_entry(const KParams &P, int64_t qi, const W
         eval_rows<NCH, STORE, METRIC>(P, qf, L.lids, 1, L.lkeys);
         wave_sync();
         epd = key_dist(L.lkeys[0]);
+        n2 = n3 = ep;
         return;
     }
     uint32_t cur = (uint32_t)P.entry;
+    n2 = n3 = cur;
     if (lane == 0) L.lids[0] = cur;
     wave_sync();
     eval_rows<NCH, STORE, METRIC>(P, qf, L.lids, 1, L.lkeys);
@@ -366,10 +371,23 @@ __device__ __forceinline__ void find_entry(const KParams &P, int64_t qi, const W
             }
             break;
         }
+        if (level == 3) n3 = cur;
+        if (level == 2) n2 = cur;
     }
     ep = cur;
     epd = curd;
+    if (P.max_level < 2) n2 = cur;
+    if (P.max_level < 3) n3 = n2;
 }
+
+template <int NCH, int STORE, int METRIC>
+__device__ __forceinline__ void find_entry(const KParams &P, int64_t qi, const WaveLds &L, const float *qf,
+                                           uint32_t &ep, float &epd, int &up_hops, int &up_evals)
+{
+    uint32_t n2, n3;
+    find_entry_path<NCH, STORE, METRIC>(P, qi, L, qf, ep, epd, up_hops, up_evals, n2, n3);
+}
+
 
 template <int NCH, int STORE, int METRIC, int MROWS>
 __device__ void search_query_req(const KParams &P, int64_t qi, int64_t slot, unsigned char *lds, int k, int ef,
@@ -564,8 +582,10 @@ __device__ void search_query_req(const KParams &P, int64_t qi, int64_t slot, uns
         if (P.out_stats) {
             P.out_stats[qi * 4 + 0] = hops;
             P.out_stats[qi * 4 + 1] = evals;
-            P.out_stats[qi * 4 + 2] = up_hops;
-            P.out_stats[qi * 4 + 3] = up_evals;
+            if (!P.keep_upper_stats) {
+                P.out_stats[qi * 4 + 2] = up_hops;
+                P.out_stats[qi * 4 + 3] = up_evals;
+            }
         }
 #endif
         if (P.status) P.status[qi] = overflow ? 1 : 0;

@@ -82,7 +82,7 @@ __global__ __launch_bounds__(256, GVS_MIN_WAVES_PER_EU) void k_serve(ServeArgs A
                 const uint64_t h = __hip_atomic_load(A.head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 // Idle rule.  Every idle wave's poll is an acquire across the
                 // bus (it invalidates its XCD's L2), so idle waves should not
-                // linger: waves gw >= SERVE_KEEP leave after idle_ticks
+                // linger: waves gw >= SERVE_KEEP leave after idle_ticks / 10
                 // without work of their own; the first SERVE_KEEP waves run
                 // their clock from the last claim by ANY wave, so they stay
                 // while requests flow and leave together once the loop is
@@ -98,7 +98,7 @@ __global__ __launch_bounds__(256, GVS_MIN_WAVES_PER_EU) void k_serve(ServeArgs A
                     }
                     continue;
                 }
-                if ((int64_t)(wall_clock64() - last) > A.idle_ticks) {
+                if ((int64_t)(wall_clock64() - last) > (gw < SERVE_KEEP ? A.idle_ticks : A.idle_ticks / 10)) {
                     state = 2u;
                     break;
                 }

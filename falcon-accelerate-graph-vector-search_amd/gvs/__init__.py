@@ -20,6 +20,7 @@ GVS_OK = 0
 FLAG_ASYNC = 1
 FLAG_HBM_VISITED = 2
 FLAG_LDS_VISITED = 4
+FLAG_REGION_ORDER = 8
 ERRORS = {-1: "ARG", -2: "HIP", -3: "NOMEM", -4: "UNSUPPORTED", -5: "OVERFLOW", -6: "IO", -7: "STATE"}
 L2, IP = 0, 1
 F32, BF16, FP8 = 0, 1, 2
@@ -209,9 +210,12 @@ class HostGraph:
 
 
 def make_params(k=10, ef=64, mc=1, mg=1, waves_per_query=1, storage=F32, rerank=False, persistent=False,
-                visited_log2=0, bloom_log2=0, grid_waves=0, asynchronous=False, visited=None):
-    """visited: None (auto), "hbm" (LDS Bloom + HBM hash) or "lds" (exact LDS hash)."""
+                visited_log2=0, bloom_log2=0, grid_waves=0, asynchronous=False, visited=None, region_order=False):
+    """visited: None (auto), "hbm" (LDS Bloom + HBM hash) or "lds" (exact LDS hash).
+    region_order: run the batch in region order (persistent mode; GVS_FLAG_REGION_ORDER)."""
     flags = FLAG_ASYNC if asynchronous else 0
+    if region_order:
+        flags |= FLAG_REGION_ORDER
     if visited == "hbm":
         flags |= FLAG_HBM_VISITED
     elif visited == "lds":
@@ -277,7 +281,8 @@ class DeviceIndex:
                                 C.c_void_p(dists_ptr), C.c_void_p(stream) if stream else None, C.byref(best),
                                 C.byref(ms)))
         cfg = dict(ef=best.ef, mc=best.mc, mg=best.mg, persistent=bool(best.persistent),
-                   waves_per_query=best.waves_per_query, grid_waves=best.grid_waves)
+                   waves_per_query=best.waves_per_query, grid_waves=best.grid_waves,
+                   region_order=bool(best.flags & FLAG_REGION_ORDER))
         return cfg, ms.value
 
     def capture(self, q_ptr, nq, ldq, labels_ptr, dists_ptr, k=10, ef=64, entries_ptr=None, stats_ptr=None, **kw):

@@ -91,6 +91,15 @@ typedef struct gvs_params {
  * (visited_log2 in [10, 14] then sets the LDS hash size). */
 #define GVS_FLAG_HBM_VISITED 2
 #define GVS_FLAG_LDS_VISITED 4
+/* Region order (persistent mode, no given entries, graphs with >= 2 upper
+ * levels): an upper-layer descent kernel first finds every query's entry
+ * and the nodes it settled on at levels 3 and 2; the batch is sorted by
+ * those (a coarse address of the query's region) and the layer-0 search
+ * runs in that order from eight work queues, one per XCD, so queries of
+ * one region run together on one XCD and share its L2.  Results and
+ * counters are those of the unordered search.  Ignored by captured
+ * searches (gvs_search_graph_create). */
+#define GVS_FLAG_REGION_ORDER 8
 
 /* Per-query debug counters written when stats != NULL (4 int32 per query):
  * layer-0 hops, layer-0 distance evaluations, upper-layer list scans,

@@ -356,3 +356,16 @@ def test_autotuner_returns_a_working_configuration(nq):
     assert cfg["waves_per_query"] in ((1, 2, 4, 8) if nq <= 64 else (1,))
     kw = {kk: v for kk, v in cfg.items() if kk not in ("ef", "mc", "mg")}
     check_equal(dev, og, q, 10, 64, mc=cfg["mc"], mg=cfg["mg"], **kw)
+
+
+@pytest.mark.parametrize("mc", [1, 2])
+def test_region_order_matches_oracle(mc):
+    """Persistent search in region order (descent kernel, sort by region,
+    per-XCD queues): ids, distances and all four counters as the oracle."""
+    x, g, dev = graph(20000, 128, 16, 100)
+    assert g.view.max_level >= 2
+    og = orc.OracleGraph(g.arrays())
+    q = queries(3000, 128, seed=70)
+    print(f"case region order mc={mc}", flush=True)
+    check_equal(dev, og, q, 10, 64, mc=mc, persistent=True, region_order=True)
+    check_equal(dev, og, q[:700], 10, 200, mc=mc, persistent=True, region_order=True)

@@ -275,3 +275,36 @@ def test_vectorised_test_encoders_match_oracle():
     xs = np.clip(x, -448, 448)
     o8 = np.array([orc.lib.orc_f32_to_e4m3(float(v)) for v in xs], dtype=np.uint8)
     np.testing.assert_array_equal(lowprec.to_e4m3(xs), o8)
+
+
+def test_nsg_style_flat_graph_keeps_long_range_edges():
+    """to_nsg(R): rows of <= R ids, every node of level >= 1 keeps (up to
+    R/2 of) its upper-layer links, the entry is such a node, and greedy
+    search from that single entry finds the true neighbours."""
+    x = gvs.gen_mixture(20000, 32, 64, 7, 8)
+    q = gvs.gen_mixture(200, 32, 64, 7, 9)
+    g = gvs.HostGraph.build_hnsw(x, M=8, ef_construction=100, nthreads=8)
+    a = g.arrays()
+    f = g.to_nsg(16).arrays()
+    assert f["max_level"] == 0 and f["r0"] == 16
+    lev = np.zeros(a["n"], dtype=np.int64)
+    up = a["upper_off"]
+    has_up = up >= 0
+    # levels from consecutive upper rows
+    offs = np.sort(up[has_up])
+    ends = np.append(offs[1:], a["adj_up"].shape[0])
+    lev_by_off = dict(zip(offs.tolist(), (ends - offs).tolist()))
+    for v in np.nonzero(has_up)[0]:
+        lev[v] = lev_by_off[int(up[v])]
+    assert lev[f["entry"]] >= 1
+    for v in np.nonzero(lev >= 1)[0][:300]:
+        row = set(f["adj0"][v][f["adj0"][v] >= 0].tolist())
+        links = []
+        for l in range(int(lev[v]), 0, -1):
+            r = a["adj_up"][up[v] + l - 1]
+            links += [u for u in r.tolist() if u >= 0 and u != v and u not in links]
+        assert set(links[:8]) <= row
+    og = orc.OracleGraph(f)
+    ids, _, _ = og.search(q, 10, 64)
+    gt, _ = orc.bruteforce_f64(x, q, 10)
+    assert orc.recall_at(ids, gt, 10) > 0.95
