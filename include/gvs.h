@@ -233,7 +233,7 @@ typedef struct gvs_server_config {
     int32_t max_inflight;      /* device query slots */
     int32_t max_k;             /* largest k accepted */
     int32_t max_ef;            /* largest ef accepted */
-    int32_t queue_capacity;    /* host submission queue length */
+    int32_t queue_capacity;    /* reserved, 0 (requests wait in the caller until a slot frees) */
     int32_t grid_waves;        /* persistent workers, 0 = auto */
     int32_t mc;                /* DST candidates per step used for every request */
     int32_t storage;           /* vector copy traversed */
@@ -242,7 +242,9 @@ typedef struct gvs_server_config {
 
 GVS_API int gvs_server_open(gvs_index *ix, const gvs_server_config *cfg, gvs_server **out);
 /* Enqueue one request (host query of dim floats).  Returns a request id >= 0,
- * or GVS_ERR_STATE when the submission queue is full (caller retries). */
+ * GVS_ERR_ARG when k or ef is outside the admitted range (counted as
+ * rejected), or GVS_ERR_STATE when all max_inflight device slots are busy
+ * (the caller polls and retries). */
 GVS_API int64_t gvs_server_submit(gvs_server *s, const float *query, int32_t k, int32_t ef);
 /* Copy out up to max_done finished requests: ids[i], labels[i*max_k..],
  * dists[i*max_k..], latency_us[i] (submit -> completion observed on host).

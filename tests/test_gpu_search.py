@@ -369,3 +369,15 @@ def test_region_order_matches_oracle(mc):
     print(f"case region order mc={mc}", flush=True)
     check_equal(dev, og, q, 10, 64, mc=mc, persistent=True, region_order=True)
     check_equal(dev, og, q[:700], 10, 200, mc=mc, persistent=True, region_order=True)
+
+
+@pytest.mark.parametrize("ef", [32, 64, 128, 256])
+def test_c2_shape_d96_degree64_matches_oracle(ef):
+    """The plain inter-query configuration's shape (D=96, M=32 so layer-0
+    degree 64, L2, one batched launch) at reduced N, over its ef sweep."""
+    x, g, dev = graph(30000, 96, 32, 200, centres=64)
+    og = orc.OracleGraph(g.arrays())
+    q = queries(300, 96, centres=64, seed=80)
+    print(f"case c2 shape ef={ef}", flush=True)
+    check_equal(dev, og, q, 10, ef)
+    check_equal(dev, og, q, 10, ef, mc=2, persistent=True, region_order=True)
