@@ -280,27 +280,34 @@ __device__ __forceinline__ int expand_lds(const KParams &P, const WaveLds &L, co
                                           const int32_t (&ids)[MROWS], int nrows, int &evals EXP_PROF)
 {
     const uint32_t hmask = (1u << P.hlog2) - 1u;
+    // first probe of every row, issued back to back so the LDS atomics of
+    // the rows overlap; collisions are then resolved row by row (linear
+    // probing in the same order, so an id met in two rows is won once)
+    uint32_t pos[MROWS], prev[MROWS];
+#pragma unroll
+    for (int r = 0; r < MROWS; r++) {
+        pos[r] = 0u;
+        prev[r] = ~0u;
+        if (r < nrows && ids[r] >= 0) {
+            pos[r] = vhash((uint32_t)ids[r], P.hlog2);
+            prev[r] = atomicCAS(&L.vh[pos[r]], 0u, (uint32_t)ids[r] + 1u);
+        }
+    }
     int nl = 0;
 #pragma unroll
     for (int r = 0; r < MROWS; r++) {
         if (r < nrows) {
             const int32_t id = ids[r];
             const uint32_t want = (uint32_t)id + 1u;
-            uint32_t pos = id >= 0 ? vhash((uint32_t)id, P.hlog2) : 0u;
-            bool pending = id >= 0, won = false;
+            bool pending = id >= 0 && prev[r] != 0u && prev[r] != want;
             while (__ballot(pending)) {
                 if (pending) {
-                    const uint32_t prev = atomicCAS(&L.vh[pos], 0u, want);
-                    if (prev == 0u) {
-                        won = true;
-                        pending = false;
-                    } else if (prev == want) {
-                        pending = false;
-                    } else {
-                        pos = (pos + 1u) & hmask;
-                    }
+                    pos[r] = (pos[r] + 1u) & hmask;
+                    prev[r] = atomicCAS(&L.vh[pos[r]], 0u, want);
+                    pending = prev[r] != 0u && prev[r] != want;
                 }
             }
+            const bool won = id >= 0 && prev[r] == 0u;
             const uint64_t bal = __ballot(won);
             if (won) L.lids[nl + mbcnt(bal)] = (uint32_t)id;
             nl += __popcll(bal);
@@ -371,8 +378,8 @@ __device__ __forceinline__ void find_entry_path(const KParams &P, int64_t qi, co
             }
             break;
         }
-        if (level == 3) n3 = cur;
-        if (level == 2) n2 = cur;
+        n3 = level == 3 ? cur : n3;
+        n2 = level == 2 ? cur : n2;
     }
     ep = cur;
     epd = curd;
