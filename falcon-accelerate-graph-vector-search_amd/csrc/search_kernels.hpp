@@ -83,8 +83,13 @@ __global__ __launch_bounds__(256) void k_descend(KParams P, int32_t *ep_out, uin
         key_out[qi] = ((uint64_t)(n3 & 0x7FFFFFFFu) << 31) | (uint64_t)(n2 & 0x7FFFFFFFu);
         perm_out[qi] = (int32_t)qi;
         if (P.out_stats) {
-            P.out_stats[qi * 4 + 2] = up_hops;
-            P.out_stats[qi * 4 + 3] = up_evals;
+#ifdef GVS_PROFILE
+            constexpr int SW = 16;  // the profile build's stats row
+#else
+            constexpr int SW = 4;
+#endif
+            P.out_stats[qi * SW + 2] = up_hops;
+            P.out_stats[qi * SW + 3] = up_evals;
         }
     }
 }

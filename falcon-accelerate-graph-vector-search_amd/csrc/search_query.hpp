@@ -581,8 +581,10 @@ __device__ void search_query_req(const KParams &P, int64_t qi, int64_t slot, uns
         if (P.out_stats) {
             P.out_stats[qi * 16 + 0] = hops;
             P.out_stats[qi * 16 + 1] = evals;
-            P.out_stats[qi * 16 + 2] = up_hops;
-            P.out_stats[qi * 16 + 3] = up_evals;
+            if (!P.keep_upper_stats) {
+                P.out_stats[qi * 16 + 2] = up_hops;
+                P.out_stats[qi * 16 + 3] = up_evals;
+            }
             for (int ph = 0; ph < 8; ph++) P.out_stats[qi * 16 + 4 + ph] = (int32_t)(prof_acc[ph] >> 6);
         }
 #else
