@@ -68,6 +68,8 @@ def _load():
         "gvs_hnswlib_load": (C.c_int, [C.c_char_p, I32, I32, P]),
         "gvs_hnswlib_save": (C.c_int, [P, C.c_char_p]),
         "gvs_nsg_load": (C.c_int, [C.c_char_p, P, I64, I32, I32, P]),
+        "gvs_fpga_load": (C.c_int, [C.c_char_p, I32, I32, I32, I32, P]),
+        "gvs_fpga_save": (C.c_int, [P, C.c_char_p, I32]),
         "gvs_host_graph_view": (C.c_int, [P, P]),
         "gvs_host_graph_free": (None, [P]),
         "gvs_gen_mixture": (C.c_int, [U64, U64, I64, I32, I32, C.c_float, C.c_float, I32, P, I32]),
@@ -108,7 +110,7 @@ lib = _load()
 def exported_symbols():
     """Names of the C ABI functions this binding knows, present in the library."""
     names = ["gvs_last_error", "gvs_version", "gvs_hnsw_build", "gvs_nsg_from_hnsw", "gvs_hnswlib_load",
-             "gvs_hnswlib_save", "gvs_nsg_load", "gvs_host_graph_view", "gvs_host_graph_free",
+             "gvs_hnswlib_save", "gvs_nsg_load", "gvs_fpga_load", "gvs_fpga_save", "gvs_host_graph_view", "gvs_host_graph_free",
              "gvs_gen_mixture", "gvs_index_create", "gvs_index_destroy", "gvs_index_device_bytes",
              "gvs_search_device", "gvs_search_host", "gvs_search_overflow_count", "gvs_search_counters", "gvs_autotune", "gvs_search_graph_create",
              "gvs_search_graph_launch", "gvs_search_graph_destroy", "gvs_device_copy", "gvs_bruteforce_f64_device", "gvs_server_open",
@@ -168,6 +170,16 @@ class HostGraph:
         h = C.c_void_p()
         _check(lib.gvs_nsg_load(path.encode(), _ptr(x), x.shape[0], x.shape[1], metric, C.byref(h)))
         return cls(h)
+
+    @classmethod
+    def load_fpga(cls, directory, kind, dim, nchannels=1, metric=L2):
+        """The reference's FPGA-layout directory (kind 0 = HNSW, 1 = NSG)."""
+        h = C.c_void_p()
+        _check(lib.gvs_fpga_load(str(directory).encode(), kind, dim, nchannels, metric, C.byref(h)))
+        return cls(h)
+
+    def save_fpga(self, directory, nchannels=1):
+        _check(lib.gvs_fpga_save(self._h, str(directory).encode(), nchannels))
 
     def to_nsg(self, R):
         h = C.c_void_p()

@@ -140,6 +140,18 @@ GVS_API int gvs_hnswlib_save(const gvs_host_graph *g, const char *path);
  * attached from x (n x dim, copied). */
 GVS_API int gvs_nsg_load(const char *path, const float *x, int64_t n, int32_t dim,
                          int32_t metric, gvs_host_graph **out);
+/* The reference's FPGA-layout index directory (the files the FPGA host
+ * loads, INTER/host.cpp:203-224, :537-632), written by
+ * vector_search_baselines/scripts_hnsw/hnsw.py:264-377 (kind 0, HNSW: meta.bin
+ * = n, maxlevel, entry, maxM, maxM0; upper_links.bin + upper_links_pointers.bin;
+ * ground_labels.bin) and scripts_nsg/nsg_to_FPGA.py:56-137 (kind 1, NSG:
+ * meta.bin = n, entry, width): per node a 64-byte-aligned link record and a
+ * vector record with the FPGA's visited pad, split round-robin over
+ * nchannels files ground_{links,vectors}_<nchannels>_chan_<c>.bin.  Save
+ * writes kind 0 for graphs with upper layers, kind 1 otherwise. */
+GVS_API int gvs_fpga_load(const char *dir, int32_t kind, int32_t dim, int32_t nchannels, int32_t metric,
+                          gvs_host_graph **out);
+GVS_API int gvs_fpga_save(const gvs_host_graph *g, const char *dir, int32_t nchannels);
 GVS_API int gvs_host_graph_view(const gvs_host_graph *g, gvs_graph *view);
 GVS_API void gvs_host_graph_free(gvs_host_graph *g);
 

@@ -308,3 +308,55 @@ def test_nsg_style_flat_graph_keeps_long_range_edges():
     ids, _, _ = og.search(q, 10, 64)
     gt, _ = orc.bruteforce_f64(x, q, 10)
     assert orc.recall_at(ids, gt, 10) > 0.95
+
+
+@pytest.mark.parametrize("nc", [1, 4])
+def test_fpga_layout_roundtrip_and_bytes(tmp_path, nc):
+    """gvs_fpga_save / gvs_fpga_load: the reference's FPGA-layout directory
+    (scripts_hnsw/hnsw.py:264-377, scripts_nsg/nsg_to_FPGA.py:56-137):
+    byte layout of the records, and a lossless round trip (HNSW and NSG)."""
+    x = gvs.gen_mixture(3000, 24, 16, 5, 6)
+    g = gvs.HostGraph.build_hnsw(x, M=6, ef_construction=60, nthreads=4)
+    a = g.arrays()
+    d = tmp_path / "hnsw"
+    d.mkdir()
+    g.save_fpga(d, nc)
+    meta = np.fromfile(d / "meta.bin", dtype=np.uint32)
+    assert meta.tolist() == [a["n"], a["max_level"], a["entry"], a["ru"], a["r0"]]
+    lrec = (a["r0"] + 15) // 16 * 64 + 64
+    vrec = (24 + 15) // 16 * 64 + 64
+    l0 = np.fromfile(d / f"ground_links_{nc}_chan_0.bin", dtype=np.uint8).reshape(-1, lrec)
+    v0 = np.fromfile(d / f"ground_vectors_{nc}_chan_0.bin", dtype=np.uint8).reshape(-1, vrec)
+    assert l0.shape[0] == (a["n"] + nc - 1) // nc
+    for j in (0, 1, 7):
+        i = j * nc  # channel 0 holds nodes 0, nc, 2nc, ...
+        row = a["adj0"][i]
+        cnt = int((row >= 0).sum())
+        rec = l0[j]
+        assert rec[:4].view(np.uint32)[0] == cnt and not rec[4:64].any()
+        assert (rec[64:64 + 4 * cnt].view(np.int32) == row[:cnt]).all()
+        assert (rec[64:64 + 4 * a["r0"]].view(np.int32)[cnt:] == 0).all()
+        vr = v0[j]
+        assert (vr[:4 * 24].view(np.float32) == x[i]).all()
+        assert vr[(24 + 15) // 16 * 64:][:4].view(np.int32)[0] == -1
+    h = gvs.HostGraph.load_fpga(d, 0, 24, nc).arrays()
+    for key in ("n", "r0", "max_level", "entry", "ru"):
+        assert h[key] == a[key], key
+    for key in ("adj0", "upper_off", "adj_up", "vectors"):
+        np.testing.assert_array_equal(h[key], a[key])
+    q = gvs.gen_mixture(50, 24, 16, 5, 7)
+    i1 = orc.OracleGraph(a).search(q, 5, 32)[0]
+    i2 = orc.OracleGraph(h).search(q, 5, 32)[0]
+    np.testing.assert_array_equal(i1, i2)
+    # NSG-style flat graph: kind 1, meta = n, entry, width
+    f = g.to_nsg(8)
+    fd = tmp_path / "nsg"
+    fd.mkdir()
+    f.save_fpga(fd, nc)
+    fa = f.arrays()
+    assert np.fromfile(fd / "meta.bin", dtype=np.uint32).tolist() == [fa["n"], fa["entry"], 8]
+    fh = gvs.HostGraph.load_fpga(fd, 1, 24, nc).arrays()
+    np.testing.assert_array_equal(fh["adj0"], fa["adj0"])
+    assert fh["entry"] == fa["entry"] and fh["max_level"] == 0
+    with pytest.raises(gvs.GvsError):
+        gvs.HostGraph.load_fpga(fd, 1, 24, nc + 1)  # wrong channel count: files missing
