@@ -140,6 +140,19 @@ __device__ __forceinline__ uint64_t wave_min_u64(uint64_t v)
     }
     return v;
 }
+// a value the code keeps equal on every lane, made visibly wave-uniform so
+// loops bounded by it branch on scalars instead of exec masks
+__device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
+
+// v with lane l replaced by the wave-uniform x: v_writelane_b32 with the
+// lane select in M0 (a VALU op may read one SGPR besides M0), after one wait
+// state for the M0 write the compiler puts in front of it
+__device__ __forceinline__ int writelane_i32(int v, int x, int l)
+{
+    asm("s_nop 0\n\tv_writelane_b32 %0, %1, m0" : "+v"(v) : "s"(x), "{m0}"(l));
+    return v;
+}
+
 __device__ __forceinline__ uint64_t readlane_u64(uint64_t v, int l)
 {
     uint32_t lo = __builtin_amdgcn_readlane((uint32_t)v, l);
@@ -322,6 +335,7 @@ template <int NCH, int STORE, int METRIC>
 __device__ __forceinline__ void eval_rows(const KParams &P, const float *qf, const uint32_t *lids, int n,
                                           uint64_t *keys)
 {
+    n = uni(n);
     for (int b0 = 0; b0 < n; b0 += 32) {
         const int nb = n - b0;
         if (nb > 16) eval_block<NCH, STORE, METRIC, 4>(P, qf, lids, b0, n, keys);

@@ -55,6 +55,9 @@ __device__ __forceinline__ uint32_t vhash(uint32_t id, int vlog2) { return (id *
 __device__ __forceinline__ int select_group(uint64_t *pool, int np, int mc, int &hint, uint32_t *sel)
 {
     const int lane = lane_id();
+    np = uni(np);
+    mc = uni(mc);
+    hint = uni(hint);
     int nsel = 0;
     for (int base = hint & ~(WAVE - 1); base < np && nsel < mc; base += WAVE) {
         const int i = base + lane;
@@ -83,6 +86,9 @@ __device__ __forceinline__ int merge_into_pool(uint64_t *&pool, uint64_t *&pool2
                                                int n, int &hint)
 {
     const int lane = lane_id();
+    np = uni(np);
+    ef = uni(ef);
+    n = uni(n);
     const uint64_t thr = (np >= ef) ? pool[ef - 1] : EMPTY_KEY;
     int ns = 0;
     for (int b = 0; b < n; b += WAVE) {
@@ -98,39 +104,56 @@ __device__ __forceinline__ int merge_into_pool(uint64_t *&pool, uint64_t *&pool2
     if (ns == 0) return np;
     // Merge survivors in batches of up to 64 (merging batch by batch gives the
     // same ef best).  Survivor s_k is broadcast by readlane; one compare per
-    // lane gives both the pool entries' shift counts and, through the ballot
-    // mask, s_k's rank in the pool.
+    // lane (pool entry < s_k) gives, through the ballot mask, s_k's rank in
+    // the pool (written to lane k) and, added up over k, how many survivors
+    // do not precede the pool entry.  Pool and survivor keys are distinct
+    // (distinct ids), so no compare needs an equality case.
     int mind = ef;
     for (int b = 0; b < ns; b += WAVE) {
-        const int cnt = min(WAVE, ns - b);
+        const int cnt = uni(min(WAVE, ns - b));
         const uint64_t sv = lane < cnt ? keys[b + lane] : EMPTY_KEY;
         int lo = 0, rs = 0;
-        for (int t = 0; t * WAVE < np; t++) {
-            const int i = t * WAVE + lane;
-            const uint64_t pv = i < np ? pool[i] : EMPTY_KEY;
-            int pc = 0;
+        {
+            // first 64 pool entries, with the survivors' ranks among themselves
+            const uint64_t pv = lane < np ? pool[lane] : EMPTY_KEY;
+            int lo_t = 0, after = 0;
             for (int kk = 0; kk < cnt; kk++) {
                 const uint64_t s = readlane_u64(sv, kk);
-                const int c = __popcll(__ballot(pv < s));
-                lo += (lane == kk) ? c : 0;
-                pc += (s < pv) ? 1 : 0;
-                if (t == 0) rs += (s < sv) ? 1 : 0;
+                const bool lt = pv < s;
+                lo_t = writelane_i32(lo_t, __popcll(__ballot(lt)), kk);
+                after += lt ? 1 : 0;
+                rs += (s < sv) ? 1 : 0;
             }
-            const int dest = i + pc;
+            lo = lo_t;
+            const int dest = lane + (cnt - after);
+            if (lane < np && dest < ef) pool2[dest] = pv;
+        }
+        for (int t = 1; t * WAVE < np; t++) {
+            const int i = t * WAVE + lane;
+            const uint64_t pv = i < np ? pool[i] : EMPTY_KEY;
+            int lo_t = 0, after = 0;
+            for (int kk = 0; kk < cnt; kk++) {
+                const uint64_t s = readlane_u64(sv, kk);
+                const bool lt = pv < s;
+                lo_t = writelane_i32(lo_t, __popcll(__ballot(lt)), kk);
+                after += lt ? 1 : 0;
+            }
+            lo += lo_t;
+            const int dest = i + (cnt - after);
             if (i < np && dest < ef) pool2[dest] = pv;
         }
         const int dest = lo + rs;
-        if (lane < cnt && dest < ef) {
-            pool2[dest] = sv;
-            mind = min(mind, dest);
-        }
+        if (lane < cnt && dest < ef) pool2[dest] = sv;
+        // the batch's smallest survivor (rank 0 among them) lands first
+        const uint64_t first = __ballot(lane < cnt && rs == 0);
+        const int d0 = __builtin_amdgcn_readlane(lo, __ffsll((unsigned long long)first) - 1);
+        mind = min(mind, d0);
         wave_sync();
         uint64_t *t = pool;
         pool = pool2;
         pool2 = t;
         np = min(ef, np + cnt);
     }
-    for (int m = 32; m >= 1; m >>= 1) mind = min(mind, __shfl_xor(mind, m));
     hint = min(hint, mind);
     return np;
 }
