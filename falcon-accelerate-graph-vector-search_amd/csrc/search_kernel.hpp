@@ -234,7 +234,8 @@ __device__ __forceinline__ float row_partial(const KParams &P, const float *qf, 
 {
     constexpr int E = StoreTraits<STORE>::E;
     constexpr int B = StoreTraits<STORE>::BYTES;
-    const unsigned char *row = (const unsigned char *)P.vec + (size_t)id * (size_t)P.row_elems * B;
+    // row stride in bytes fits 32 bits: one 32x32+64 multiply-add per row
+    const unsigned char *row = (const unsigned char *)P.vec + (uint64_t)id * (uint32_t)(P.row_elems * B);
     float scale = 1.0f;
     if constexpr (STORE == STORE_FP8) scale = P.vscale[id];
     float acc = 0.0f;
@@ -295,7 +296,7 @@ __device__ __forceinline__ void eval_block(const KParams &P, const float *qf, co
     for (int t = 0; t < NP; t++) {
         const int r = min(b0 + t * 8 + g, n - 1);
         const uint32_t id = lids[r];
-        const unsigned char *row = (const unsigned char *)P.vec + (size_t)id * (size_t)P.row_elems * B;
+        const unsigned char *row = (const unsigned char *)P.vec + (uint64_t)id * (uint32_t)(P.row_elems * B);
 #pragma unroll
         for (int i = 0; i < NCH; i++) raw[t][i] = *(const uint4 *)(row + (size_t)(p + 8 * i) * 16);
         scale[t] = 1.0f;
@@ -319,7 +320,7 @@ __device__ __forceinline__ void eval_block(const KParams &P, const float *qf, co
                 }
             }
         }
-        acc[t] = (b0 + t * 8 + g < n) ? a : 0.0f;
+        acc[t] = a;  // rows past n only meet lanes of the same row, and are dropped below
     }
     const float v = reduce8x4(acc, p);
     const int r = b0 + (p & 3) * 8 + g;

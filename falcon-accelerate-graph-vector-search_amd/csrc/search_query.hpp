@@ -63,17 +63,18 @@ __device__ __forceinline__ int select_group(uint64_t *pool, int np, int mc, int 
         const int i = base + lane;
         const uint64_t key = i < np ? pool[i] : EMPTY_KEY;
         const bool unexp = i < np && i >= hint && !(key & 1ull);
-        uint64_t bal = __ballot(unexp);
-        while (bal && nsel < mc) {
-            const int l = __ffsll((unsigned long long)bal) - 1;
-            bal &= bal - 1;
-            if (lane == l) {
-                pool[i] = key | 1ull;
-                sel[nsel] = key_id(key);
-            }
-            nsel++;
+        const uint64_t bal = __ballot(unexp);
+        // the first `need` unexpanded entries of the chunk, in pool order
+        const int need = mc - nsel;
+        const int rank = mbcnt(bal);
+        if (unexp && rank < need) {
+            pool[i] = key | 1ull;
+            sel[nsel + rank] = key_id(key);
         }
-        hint = bal ? base + __ffsll((unsigned long long)bal) - 1 : min(np, base + WAVE);
+        nsel += min(need, __popcll(bal));
+        // next unexpanded entry: the chunk's first one not taken, else past it
+        const uint64_t rest = __ballot(unexp && rank >= need);
+        hint = rest ? base + __ffsll((unsigned long long)rest) - 1 : min(np, base + WAVE);
     }
     wave_sync();
     return nsel;
