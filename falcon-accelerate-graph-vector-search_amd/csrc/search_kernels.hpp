@@ -16,35 +16,45 @@ __global__ __launch_bounds__(256, GVS_MIN_WAVES_PER_EU) void k_search(KParams P)
     const int wib = threadIdx.x >> 6;
     unsigned char *lds = smem + (size_t)wib * P.lds_per_wave;
     const int64_t gw = (int64_t)blockIdx.x * P.waves_per_block + wib;
-    if (P.persistent && P.xcd_queues) {
-        // region order: the ordered work is cut into 8 segments, one per XCD
-        // (so the queries a XCD runs at once share its L2); a wave drains
-        // its XCD's segment, then helps with the others
-        const int x0 = (int)(xcc_id() & 7u);
-        for (int t = 0; t < 8; t++) {
-            const int x = (x0 + t) & 7;
-            const int64_t s0 = P.nq * x / 8, s1 = P.nq * (x + 1) / 8;
-            for (;;) {
+    // One call site for the search, so the kernel holds one copy of it.
+    // Work sources: region order - the ordered work is cut into 8 segments,
+    // one per XCD (so the queries a XCD runs at once share its L2), and a
+    // wave drains its XCD's segment, then helps with the others; persistent -
+    // a device work queue, each wave takes the next query; otherwise one
+    // query per wave.
+    const int x0 = (int)(xcc_id() & 7u);
+    int seg = 0;
+    bool once = true;
+    for (;;) {
+        int64_t qi;
+        if (P.persistent && P.xcd_queues) {
+            int64_t item = -1;
+            while (seg < 8) {
+                const int x = (x0 + seg) & 7;
+                const int64_t s0 = P.nq * x / 8, s1 = P.nq * (x + 1) / 8;
                 uint32_t w = 0;
                 if (lane_id() == 0) w = atomicAdd(&P.counters[4 + x], 1u);
                 w = __builtin_amdgcn_readfirstlane(w);
-                if (s0 + (int64_t)w >= s1) break;
-                const int64_t item = s0 + (int64_t)w;
-                search_query<NCH, STORE, METRIC, MROWS>(P, P.q_perm ? (int64_t)P.q_perm[item] : item, gw, lds);
+                if (s0 + (int64_t)w < s1) {
+                    item = s0 + (int64_t)w;
+                    break;
+                }
+                seg++;
             }
-        }
-    } else if (P.persistent) {
-        // device work queue: each wave takes the next query index
-        for (;;) {
+            if (item < 0) break;
+            qi = P.q_perm ? (int64_t)P.q_perm[item] : item;
+        } else if (P.persistent) {
             uint32_t q = 0;
             if (lane_id() == 0) q = atomicAdd(&P.counters[0], 1u);
             q = __builtin_amdgcn_readfirstlane(q);
             if ((int64_t)q >= P.nq) break;
-            search_query<NCH, STORE, METRIC, MROWS>(P, P.q_perm ? (int64_t)P.q_perm[q] : (int64_t)q, gw, lds);
+            qi = P.q_perm ? (int64_t)P.q_perm[q] : (int64_t)q;
+        } else {
+            if (!once) break;
+            once = false;
+            qi = P.q_base + gw;
+            if (qi >= P.nq) break;
         }
-    } else {
-        const int64_t qi = P.q_base + gw;
-        if (qi >= P.nq) return;
         search_query<NCH, STORE, METRIC, MROWS>(P, qi, gw, lds);
     }
 }
