@@ -257,6 +257,24 @@ __device__ __forceinline__ float row_partial(const KParams &P, const float *qf, 
     return acc;
 }
 
+// Cross-lane moves inside 8-lane groups as DPP operand moves (VALU, no LDS
+// round trip): xor 1 and xor 2 are quad permutes; xor 4 is a row shift by 4
+// towards the lower half (lanes 0-3 of a group read lane + 4: row_shl) or the
+// upper half (lanes 4-7 read lane - 4: row_shr).
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float xor1_f(float v) { return dpp_f<0xB1>(v); }  // quad_perm [1,0,3,2]
+__device__ __forceinline__ float xor2_f(float v) { return dpp_f<0x4E>(v); }  // quad_perm [2,3,0,1]
+__device__ __forceinline__ float xor4_f(float v, bool upper)
+{
+    const float dn = dpp_f<0x104>(v);  // row_shl:4, lane i reads lane i + 4
+    const float up = dpp_f<0x114>(v);  // row_shr:4, lane i reads lane i - 4
+    return upper ? up : dn;
+}
+
 // 8-lane reduction of 4 passes: in acc[t] = this lane's partial of pass t.
 // xor 4 adds the partner's partials (no halving), xor 2 and xor 1 halve, so
 // lane p ends with the full sum of pass (p & 3) (lanes p and p^4 agree).
@@ -264,19 +282,19 @@ __device__ __forceinline__ float row_partial(const KParams &P, const float *qf, 
 // swaps, which IEEE addition does not see.
 __device__ __forceinline__ float reduce8x4(const float acc[4], int p)
 {
-    const bool b1 = p & 2, b0 = p & 1;
+    const bool b2 = p & 4, b1 = p & 2, b0 = p & 1;
     float v4[4], v2[2];
 #pragma unroll
-    for (int t = 0; t < 4; t++) v4[t] = acc[t] + __shfl_xor(acc[t], 4);
+    for (int t = 0; t < 4; t++) v4[t] = acc[t] + xor4_f(acc[t], b2);
 #pragma unroll
     for (int t = 0; t < 2; t++) {
         float send = b1 ? v4[t] : v4[t + 2];
         float keep = b1 ? v4[t + 2] : v4[t];
-        v2[t] = keep + __shfl_xor(send, 2);
+        v2[t] = keep + xor2_f(send);
     }
     float send = b0 ? v2[0] : v2[1];
     float keep = b0 ? v2[1] : v2[0];
-    return keep + __shfl_xor(send, 1);
+    return keep + xor1_f(send);
 }
 
 // One block of NP passes (8*NP rows from b0).  Every lane loads its chunks
