@@ -259,20 +259,30 @@ __device__ __forceinline__ int expand_group(const KParams &P, const WaveLds &L, 
     eval_rows<NCH, STORE, METRIC>(P, qf, L.lids, nl, L.lkeys);
     PROF_MARK(3);
     int nl2 = nl;
+    // table collisions of all rows resolved together, one slot per round
+    // (as in expand_lds: an id met in two rows moves in step and is won once)
+    uint32_t pend = 0u;
+#pragma unroll
+    for (int r = 0; r < MROWS; r++) {
+        const uint32_t want = (uint32_t)idv[r] + 1u;
+        if (r < nrows && idv[r] >= 0 && prev[r] != 0u && prev[r] != want) pend |= 1u << r;
+    }
+    while (__ballot(pend != 0u)) {
+#pragma unroll
+        for (int r = 0; r < MROWS; r++) {
+            if (pend & (1u << r)) {
+                const uint32_t want = (uint32_t)idv[r] + 1u;
+                pos[r] = (pos[r] + 1u) & tmask;
+                prev[r] = atomicCAS(&T[pos[r]], 0u, want);
+                if (prev[r] == 0u || prev[r] == want) pend &= ~(1u << r);
+            }
+        }
+    }
 #pragma unroll
     for (int r = 0; r < MROWS; r++) {
         if (r < nrows) {
             const int32_t id = idv[r];
             const bool valid = id >= 0;
-            const uint32_t want = (uint32_t)id + 1u;
-            bool pending = valid && prev[r] != 0u && prev[r] != want;
-            while (__ballot(pending)) {
-                if (pending) {
-                    pos[r] = (pos[r] + 1u) & tmask;
-                    prev[r] = atomicCAS(&T[pos[r]], 0u, want);
-                    pending = prev[r] != 0u && prev[r] != want;
-                }
-            }
             const bool won = valid && prev[r] == 0u;
             const bool spec = (spec_bits >> r) & 1u;
             if (spec && !won) L.lkeys[spos[r]] = EMPTY_KEY;  // same id twice in one group
@@ -304,10 +314,15 @@ __device__ __forceinline__ int expand_lds(const KParams &P, const WaveLds &L, co
                                           const int32_t (&ids)[MROWS], int nrows, int &evals EXP_PROF)
 {
     const uint32_t hmask = (1u << P.hlog2) - 1u;
-    // first probe of every row, issued back to back so the LDS atomics of
-    // the rows overlap; collisions are then resolved row by row (linear
-    // probing in the same order, so an id met in two rows is won once)
+    // First probe of every row, issued back to back so the LDS atomics of
+    // the rows overlap.  Collisions are then resolved for all rows together:
+    // each round moves every pending (lane, row) one slot on, so the wave
+    // waits for the longest probe sequence of the group, not for the sum of
+    // each row's longest.  Within a round the rows probe in row order, and
+    // an id met in two rows starts at the same slot in the same round and
+    // moves in step, so it is won once.
     uint32_t pos[MROWS], prev[MROWS];
+    uint32_t pend = 0u;  // bit r: row r still probing
 #pragma unroll
     for (int r = 0; r < MROWS; r++) {
         pos[r] = 0u;
@@ -317,20 +332,27 @@ __device__ __forceinline__ int expand_lds(const KParams &P, const WaveLds &L, co
             prev[r] = atomicCAS(&L.vh[pos[r]], 0u, (uint32_t)ids[r] + 1u);
         }
     }
+#pragma unroll
+    for (int r = 0; r < MROWS; r++) {
+        const uint32_t want = (uint32_t)ids[r] + 1u;
+        if (r < nrows && ids[r] >= 0 && prev[r] != 0u && prev[r] != want) pend |= 1u << r;
+    }
+    while (__ballot(pend != 0u)) {
+#pragma unroll
+        for (int r = 0; r < MROWS; r++) {
+            if (pend & (1u << r)) {
+                const uint32_t want = (uint32_t)ids[r] + 1u;
+                pos[r] = (pos[r] + 1u) & hmask;
+                prev[r] = atomicCAS(&L.vh[pos[r]], 0u, want);
+                if (prev[r] == 0u || prev[r] == want) pend &= ~(1u << r);
+            }
+        }
+    }
     int nl = 0;
 #pragma unroll
     for (int r = 0; r < MROWS; r++) {
         if (r < nrows) {
             const int32_t id = ids[r];
-            const uint32_t want = (uint32_t)id + 1u;
-            bool pending = id >= 0 && prev[r] != 0u && prev[r] != want;
-            while (__ballot(pending)) {
-                if (pending) {
-                    pos[r] = (pos[r] + 1u) & hmask;
-                    prev[r] = atomicCAS(&L.vh[pos[r]], 0u, want);
-                    pending = prev[r] != 0u && prev[r] != want;
-                }
-            }
             const bool won = id >= 0 && prev[r] == 0u;
             const uint64_t bal = __ballot(won);
             if (won) L.lids[nl + mbcnt(bal)] = (uint32_t)id;
