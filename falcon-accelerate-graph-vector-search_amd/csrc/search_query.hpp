@@ -216,7 +216,9 @@ __device__ __forceinline__ int expand_group(const KParams &P, const WaveLds &L, 
                 const uint32_t want = (uint32_t)id + 1u;
                 uint32_t hp = valid ? vhash((uint32_t)id, P.hlog2) : 0u;
                 bool probing = valid;
+                uint32_t step = 0u;  // every probing lane is at the same step
                 while (__ballot(probing)) {
+                    step++;
                     if (probing) {
                         const uint32_t v = frozen[hp];
                         if (v == want) {
@@ -225,7 +227,7 @@ __device__ __forceinline__ int expand_group(const KParams &P, const WaveLds &L, 
                         } else if (v == 0u) {
                             probing = false;
                         } else {
-                            hp = (hp + 1u) & hmask;
+                            hp = (hp + step) & hmask;
                         }
                     }
                 }
@@ -267,12 +269,12 @@ __device__ __forceinline__ int expand_group(const KParams &P, const WaveLds &L, 
         const uint32_t want = (uint32_t)idv[r] + 1u;
         if (r < nrows && idv[r] >= 0 && prev[r] != 0u && prev[r] != want) pend |= 1u << r;
     }
-    while (__ballot(pend != 0u)) {
+    for (uint32_t round = 1u; __ballot(pend != 0u); round++) {
 #pragma unroll
         for (int r = 0; r < MROWS; r++) {
             if (pend & (1u << r)) {
                 const uint32_t want = (uint32_t)idv[r] + 1u;
-                pos[r] = (pos[r] + 1u) & tmask;
+                pos[r] = (pos[r] + round) & tmask;
                 prev[r] = atomicCAS(&T[pos[r]], 0u, want);
                 if (prev[r] == 0u || prev[r] == want) pend &= ~(1u << r);
             }
@@ -306,7 +308,10 @@ __device__ __forceinline__ int expand_group(const KParams &P, const WaveLds &L, 
 // ---- exact visited set in LDS (small ef) -------------------------------
 // Open addressing over 2^hlog2 slots of this wave's LDS (key = id + 1), so a
 // lookup-and-insert is one LDS compare-and-swap and collisions cost LDS
-// latency only.  LDS CAS serialises lanes, so an id met twice in one group
+// latency only.  Every table (LDS hash, HBM table) probes triangularly:
+// step d looks at home + d(d+1)/2, which reaches every slot of a
+// power-of-two table and avoids the long clusters of linear probing near
+// 3/4 load; all lanes probing together are at the same step.  LDS CAS serialises lanes, so an id met twice in one group
 // is evaluated once.  Used while the visited count stays below 3/4 of the
 // slots; a query that outgrows it is rerun with the HBM table.
 template <int NCH, int STORE, int METRIC, int MROWS>
@@ -337,12 +342,12 @@ __device__ __forceinline__ int expand_lds(const KParams &P, const WaveLds &L, co
         const uint32_t want = (uint32_t)ids[r] + 1u;
         if (r < nrows && ids[r] >= 0 && prev[r] != 0u && prev[r] != want) pend |= 1u << r;
     }
-    while (__ballot(pend != 0u)) {
+    for (uint32_t round = 1u; __ballot(pend != 0u); round++) {
 #pragma unroll
         for (int r = 0; r < MROWS; r++) {
             if (pend & (1u << r)) {
                 const uint32_t want = (uint32_t)ids[r] + 1u;
-                pos[r] = (pos[r] + 1u) & hmask;
+                pos[r] = (pos[r] + round) & hmask;
                 prev[r] = atomicCAS(&L.vh[pos[r]], 0u, want);
                 if (prev[r] == 0u || prev[r] == want) pend &= ~(1u << r);
             }
