@@ -79,7 +79,7 @@ __global__ __launch_bounds__(256, GVS_MIN_WAVES_PER_EU) void k_serve(ServeArgs A
                     break;
                 }
                 const uint64_t t = ld_sys64(&A.ctl->tail);
-                const uint64_t h = __hip_atomic_load(A.head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                uint64_t h = __hip_atomic_load(A.head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 // Idle rule.  Every idle wave's poll is an acquire across the
                 // bus (it invalidates its XCD's L2), so idle waves should not
                 // linger: waves gw >= SERVE_KEEP leave after idle_ticks / 10
@@ -92,7 +92,21 @@ __global__ __launch_bounds__(256, GVS_MIN_WAVES_PER_EU) void k_serve(ServeArgs A
                     last = wall_clock64();
                 }
                 if (h < t) {
-                    if (atomicCAS((unsigned long long *)A.head, (unsigned long long)h, (unsigned long long)(h + 1)) == h) {
+                    // claim: a failed compare-and-swap returns the head another
+                    // wave moved it to, which is tried at once while it is
+                    // below the tail read above (the tail only grows), so
+                    // contending waves do not reread the host's tail per try
+                    bool got = false;
+                    while (h < t) {
+                        const uint64_t o = atomicCAS((unsigned long long *)A.head, (unsigned long long)h,
+                                                     (unsigned long long)(h + 1));
+                        if (o == h) {
+                            got = true;
+                            break;
+                        }
+                        h = o;
+                    }
+                    if (got) {
                         item = h;
                         break;
                     }
