@@ -557,13 +557,17 @@ int launch_search(gvs_index *ix, const gvs_params *p, const LaunchPlan &lp, KPar
         P.q_base = 0;
         hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, stream, ix->d_counters);
         if (region) {
-            // descent, then sort the batch by region, then layer 0 in that order
+            // descent, then sort the batch by region, then layer 0 in that order;
+            // the key is two node ids of ceil(log2 n) bits, so the radix sort
+            // passes over only those bits
+            P.key_bits = std::max(1, ilog2_ceil(std::max<int64_t>(ix->n, 2)));
             hipLaunchKernelGGL((dfn_t)lp.dfn, dim3((unsigned)((nq + 3) / 4)), dim3(256), 0, stream, P, ix->d_rep,
                                ix->d_rkey, ix->d_rperm);
             if (!hip_ok(hipGetLastError(), "launch k_descend")) return GVS_ERR_HIP;
             size_t bytes = ix->rtmp_bytes;
             if (!hip_ok(rocprim::radix_sort_pairs(ix->d_rtmp, bytes, ix->d_rkey, ix->d_rkey + ix->rcap, ix->d_rperm,
-                                                 ix->d_rperm + ix->rcap, (unsigned int)nq, 0, 62, stream),
+                                                 ix->d_rperm + ix->rcap, (unsigned int)nq, 0, 2 * P.key_bits,
+                                                 stream),
                         "rocprim::radix_sort_pairs"))
                 return GVS_ERR_HIP;
             P.entries = ix->d_rep;

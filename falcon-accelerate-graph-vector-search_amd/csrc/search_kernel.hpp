@@ -80,6 +80,7 @@ struct KParams {
     // entries and the upper-layer counters come from the descent kernel
     const int32_t *q_perm;
     int keep_upper_stats;      // leave out_stats[.., 2..3] as the descent wrote them
+    int key_bits;              // region key: bits per node id (level-3 node above level-2 node)
     int xcd_queues;            // 8 work queues (counters[4..11]), one per XCD segment of the order
 };
 

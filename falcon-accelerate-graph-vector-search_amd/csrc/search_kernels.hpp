@@ -90,7 +90,7 @@ __global__ __launch_bounds__(256) void k_descend(KParams P, int32_t *ep_out, uin
     find_entry_path<NCH, STORE, METRIC>(P, qi, L, qf, ep, epd, up_hops, up_evals, n2, n3);
     if (lane == 0) {
         ep_out[qi] = (int32_t)ep;
-        key_out[qi] = ((uint64_t)(n3 & 0x7FFFFFFFu) << 31) | (uint64_t)(n2 & 0x7FFFFFFFu);
+        key_out[qi] = ((uint64_t)n3 << P.key_bits) | (uint64_t)n2;
         perm_out[qi] = (int32_t)qi;
         if (P.out_stats) {
 #ifdef GVS_PROFILE
