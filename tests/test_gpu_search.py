@@ -53,6 +53,9 @@ def test_dst_multi_candidate_matches_oracle(mc):
     q = queries(300, 128, seed=3)
     print(f"case dst mc={mc}", flush=True)
     check_equal(dev, og, q, 10, 64, mc=mc)
+    # pools of several 64-entry chunks, merged with up to mc * 64 new keys
+    check_equal(dev, og, q, 10, 200, mc=mc)
+    check_equal(dev, og, q, 100, 400, mc=mc, persistent=True, region_order=True)
 
 
 @pytest.mark.parametrize("mc,mg", [(1, 2), (2, 2), (4, 2)])
