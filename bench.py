@@ -239,7 +239,7 @@ def main():
     ap.add_argument("--nq", type=int, default=10_000)
     ap.add_argument("--k", type=int, default=10)
     ap.add_argument("--target", type=float, default=0.95)
-    ap.add_argument("--efs", default="16,24,32,40,48,52,56,60,64,96,128,192,256",
+    ap.add_argument("--efs", default="16,24,32,40,48,52,56,58,60,62,64,96,128,192,256",
                     help="ef grid of the plain sweep; the autotuner tries every ef of it up to the "
                          "plain path's first ef that reaches the recall target")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
