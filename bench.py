@@ -56,10 +56,8 @@ def read_traffic(path, cfg):
         key += " region_order=1"
     for line in open(path):
         line = line.strip()
-        if line.startswith("traffic ") and key in line:
-            for tok in line.split():
-                if tok.startswith("bytes="):
-                    return int(tok[6:])
+        if line.startswith("traffic ") and " bytes=" in line and line.split(" bytes=")[0] == "traffic " + key:
+            return int(line.split(" bytes=")[1].split()[0])
     return None
 
 
