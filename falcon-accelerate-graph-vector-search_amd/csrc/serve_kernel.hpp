@@ -43,6 +43,16 @@ __device__ __forceinline__ int32_t ld_sys32i(const int32_t *p)
 {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
+// relaxed system-scope loads: read memory (not a cached copy) without the
+// L2 invalidation an acquire brings
+__device__ __forceinline__ uint64_t ld_sys64r(const uint64_t *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+__device__ __forceinline__ uint32_t ld_sys32r(const uint32_t *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
 
 // waves that stay while requests flow (one per CU on MI355X)
 constexpr int64_t SERVE_KEEP = 256;
@@ -71,22 +81,27 @@ __global__ __launch_bounds__(256, GVS_MIN_WAVES_PER_EU) void k_serve(ServeArgs A
             // 32 sleeps of ~4 us): every idle wave's poll is a read across
             // the bus, and thousands of them at full rate load the host's
             // memory; with many waves some wave still looks every few ns.
+            // The polls are relaxed: an acquire invalidates the XCD's L2
+            // under the searches that run there, so it is taken once per
+            // claimed request (below), not once per poll.
             int backoff = 0;
             uint64_t seen_h = ~0ull;
+            uint64_t h_max = 0ull;  // the highest head seen (it only grows)
             for (;;) {
-                if (ld_sys32(&A.ctl->stop)) {
+                if (ld_sys32r(&A.ctl->stop)) {
                     state = 1u;
                     break;
                 }
-                const uint64_t t = ld_sys64(&A.ctl->tail);
+                const uint64_t t = ld_sys64r(&A.ctl->tail);
+                // the local L2 may hold an old head: never go below one seen
                 uint64_t h = __hip_atomic_load(A.head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                // Idle rule.  Every idle wave's poll is an acquire across the
-                // bus (it invalidates its XCD's L2), so idle waves should not
-                // linger: waves gw >= SERVE_KEEP leave after idle_ticks / 10
-                // without work of their own; the first SERVE_KEEP waves run
-                // their clock from the last claim by ANY wave, so they stay
-                // while requests flow and leave together once the loop is
-                // idle.  The pool never shrinks below SERVE_KEEP waves.
+                h = h > h_max ? h : h_max;
+                // Idle rule.  Waves gw >= SERVE_KEEP leave after
+                // idle_ticks / 10 without work of their own; the first
+                // SERVE_KEEP waves run their clock from the last claim by
+                // ANY wave, so they stay while requests flow and leave
+                // together once the loop is idle.  The pool never shrinks
+                // below SERVE_KEEP waves.
                 if (gw < SERVE_KEEP && h != seen_h) {
                     seen_h = h;
                     last = wall_clock64();
@@ -94,8 +109,7 @@ __global__ __launch_bounds__(256, GVS_MIN_WAVES_PER_EU) void k_serve(ServeArgs A
                 if (h < t) {
                     // claim: a failed compare-and-swap returns the head another
                     // wave moved it to, which is tried at once while it is
-                    // below the tail read above (the tail only grows), so
-                    // contending waves do not reread the host's tail per try
+                    // below the tail read above (the tail only grows)
                     bool got = false;
                     while (h < t) {
                         const uint64_t o = atomicCAS((unsigned long long *)A.head, (unsigned long long)h,
@@ -110,7 +124,7 @@ __global__ __launch_bounds__(256, GVS_MIN_WAVES_PER_EU) void k_serve(ServeArgs A
                         item = h;
                         break;
                     }
-                    continue;
+                    h_max = h;  // every request up to the tail was claimed: back off
                 }
                 if ((int64_t)(wall_clock64() - last) > (gw < SERVE_KEEP ? A.idle_ticks : A.idle_ticks / 10)) {
                     state = 2u;
@@ -124,14 +138,19 @@ __global__ __launch_bounds__(256, GVS_MIN_WAVES_PER_EU) void k_serve(ServeArgs A
         if (state != 0u) break;
         item = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(item >> 32)) << 32) |
                (uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)item);
-        const uint32_t slot = ld_sys32(&A.ring[item & (uint64_t)(A.rcap - 1u)]);
+        // The tail read (relaxed) showed the host's release of this request:
+        // an acquire fence orders every read of its fields after it.
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");
+        const uint32_t slot = ld_sys32r(&A.ring[item & (uint64_t)(A.rcap - 1u)]);
         const int k = ld_sys32i(&A.qk[slot]);
         const int ef = ld_sys32i(&A.qef[slot]);
-        const uint64_t seq = ld_sys64(&A.qseq[slot]);
+        const uint64_t seq = ld_sys64r(&A.qseq[slot]);
         search_query_req<NCH, STORE, METRIC, MROWS>(A.P, (int64_t)slot, gw, lds, k, ef,
                                                     A.qbuf + (size_t)slot * A.P.ldq, true);
-        // results (plain stores to pinned memory) become visible before the flag
-        __threadfence_system();
+        // results (plain stores to pinned memory) become visible before the
+        // flag: a release fence (write-back, no L2 invalidation; the next
+        // claim takes its own acquire)
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
         if (lane == 0) {
             __hip_atomic_store(&A.done[slot], seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
             const uint64_t pos = atomicAdd((unsigned long long *)A.served, 1ull);
