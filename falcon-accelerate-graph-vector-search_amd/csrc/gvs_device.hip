@@ -360,6 +360,18 @@ int plan_search(const gvs_index *ix, const gvs_params *p, int max_ef, int max_k,
         // none (2 words keep the hash 16-byte aligned): those ids are then
         // all gathered speculatively
         lp->blog2 = p->bloom_log2 > 0 ? p->bloom_log2 : 1;
+    } else if (!(p->flags & GVS_FLAG_HBM_VISITED) && W == 1) {
+        // larger ef: an exact LDS hash of 2^13 2-byte slots (a node's
+        // quotient and probe step) in the 16 KB a Bloom filter would take,
+        // when a node's quotient fits 8 bits; it holds ~6,000 evaluations
+        // before the query freezes it and goes on with the HBM table
+        const int nbits = std::max(13, ilog2_ceil(std::max<int64_t>(ix->n, 2)));
+        if (nbits <= 13 + 8) {
+            lp->hlog2 = 13;
+            lp->hnarrow = 1;
+            lp->hbits = nbits;
+            lp->blog2 = p->bloom_log2 > 0 ? p->bloom_log2 : 1;
+        }
     }
     if (lp->blog2 > 13 || lp->vlog2 > 26) {
         set_error("gvs_search: bloom_log2 <= 13 and visited_log2 <= 26 required");
@@ -385,7 +397,7 @@ int plan_search(const gvs_index *ix, const gvs_params *p, int max_ef, int max_k,
         return GVS_OK;
     }
     int bytes = 16 * lp->ef_cap + 8 * lp->lcap + 8 * (1 << lp->blog2) + 4 * lp->lcap + 64 +
-                (lp->hlog2 > 0 ? 4 * (1 << lp->hlog2) : 0);
+                (lp->hlog2 > 0 ? (lp->hnarrow ? 2 : 4) * (1 << lp->hlog2) : 0);
     lp->lds_per_wave = (bytes + 15) / 16 * 16;
     // waves per block: the most resident waves per CU (160 KiB LDS, at most
     // 16 workgroups per CU, at most 64 KiB per block)
@@ -459,6 +471,8 @@ void fill_params(const gvs_index *ix, const gvs_params *p, const LaunchPlan &lp,
     P.vlog2 = lp.vlog2;
     P.blog2 = lp.blog2;
     P.hlog2 = lp.hlog2;
+    P.hnarrow = lp.hnarrow;
+    P.hbits = lp.hbits;
     P.bloom_seed = 0x2545F491u;
     P.ef_cap = lp.ef_cap;
     P.lcap = lp.lcap;

@@ -44,7 +44,7 @@ bool hip_ok(hipError_t e, const char *what);
 struct LaunchPlan {
     void *fn = nullptr;
     void *dfn = nullptr;           // upper-layer descent kernel (region order)
-    int ef_cap = 0, lcap = 0, blog2 = 0, vlog2 = 0, hlog2 = 0, lds_per_wave = 0, wpb = 1;
+    int ef_cap = 0, lcap = 0, blog2 = 0, vlog2 = 0, hlog2 = 0, hnarrow = 0, hbits = 0, lds_per_wave = 0, wpb = 1;
     int64_t resident_waves = 0;
 };
 }  // namespace gvs

@@ -50,6 +50,24 @@ __device__ __forceinline__ void bloom_pos(uint32_t id, uint32_t seed, uint32_t n
 
 __device__ __forceinline__ uint32_t vhash(uint32_t id, int vlog2) { return (id * 0x9E3779B1u) >> (32 - vlog2); }
 
+// ---- 2-byte LDS hash slots (ef above 128) ------------------------------
+// For graphs of at most 2^(hlog2 + 8) nodes.  y = id * odd mod 2^hbits is a
+// bijection of the ids; its top hlog2 bits are the node's home slot, the
+// other hbits - hlog2 (<= 8) bits its quotient.  A node stored at probe step
+// d (the same triangular sequence as the 4-byte tables) holds
+// 1 + (quotient << 7 | d).  A slot whose step and quotient both match is the
+// node itself (same home, same quotient, same y), so membership stays exact.
+// A probe sequence past step 127 (probability about load^128) makes the
+// query overflow.  Slots pair up in 32-bit words, claimed with a 32-bit LDS
+// compare-and-swap.
+__device__ __forceinline__ void nkey(uint32_t id, int hbits, int hlog2, uint32_t &home, uint32_t &q)
+{
+    const uint32_t y = (id * 0x9E3779B1u) & ((1u << hbits) - 1u);
+    home = y >> (hbits - hlog2);
+    q = y & ((1u << (hbits - hlog2)) - 1u);
+}
+__device__ __forceinline__ uint32_t nentry(uint32_t q, uint32_t d) { return 1u + ((q << 7) | d); }
+
 // Pick up to mc unexpanded pool entries, closest first, from index `hint`
 // on (entries before hint are expanded); marks them expanded.
 __device__ __forceinline__ int select_group(uint64_t *pool, int np, int mc, int &hint, uint32_t *sel)
@@ -212,7 +230,27 @@ __device__ __forceinline__ int expand_group(const KParams &P, const WaveLds &L, 
         if (r < nrows) {
             const int32_t id = ids[r];
             bool valid = id >= 0;
-            if (frozen) {
+            if (frozen && P.hnarrow) {
+                const uint16_t *F = (const uint16_t *)frozen;
+                uint32_t hp = 0u, q = 0u;
+                if (valid) nkey((uint32_t)id, P.hbits, P.hlog2, hp, q);
+                bool probing = valid;
+                uint32_t step = 0u;  // every probing lane is at the same step
+                while (__ballot(probing)) {
+                    if (probing) {
+                        const uint32_t v = F[hp];
+                        if (v == nentry(q, step)) {
+                            valid = false;
+                            probing = false;
+                        } else if (v == 0u || step == 127u) {
+                            probing = false;
+                        } else {
+                            hp = (hp + step + 1u) & hmask;
+                        }
+                    }
+                    step++;
+                }
+            } else if (frozen) {
                 const uint32_t want = (uint32_t)id + 1u;
                 uint32_t hp = valid ? vhash((uint32_t)id, P.hlog2) : 0u;
                 bool probing = valid;
@@ -305,19 +343,104 @@ __device__ __forceinline__ int expand_group(const KParams &P, const WaveLds &L, 
     return nl2;
 }
 
-// ---- exact visited set in LDS (small ef) -------------------------------
+// ---- exact visited set in LDS --------------------------------------------
 // Open addressing over 2^hlog2 slots of this wave's LDS (key = id + 1), so a
 // lookup-and-insert is one LDS compare-and-swap and collisions cost LDS
 // latency only.  Every table (LDS hash, HBM table) probes triangularly:
 // step d looks at home + d(d+1)/2, which reaches every slot of a
 // power-of-two table and avoids the long clusters of linear probing near
-// 3/4 load; all lanes probing together are at the same step.  LDS CAS serialises lanes, so an id met twice in one group
-// is evaluated once.  Used while the visited count stays below 3/4 of the
-// slots; a query that outgrows it is rerun with the HBM table.
+// 3/4 load; all lanes probing together are at the same step.  LDS CAS
+// serialises lanes, so an id met twice in one group is evaluated once.
+// Used while the visited count stays below 3/4 of the slots; a query that
+// outgrows it freezes it and goes on with the HBM table.
+
+// The same with 2-byte slots (ef above 128).  A lane's compare-and-swap guesses that the
+// slot's word is empty; a failed one returns the word, so a later step in
+// the same word compares against what is there (a lane that retries a slot
+// or steps within its word issues no atomic that round).  Returns -1 if a
+// probe sequence ran past step 127 (the query overflows).
+template <int MROWS>
+__device__ __forceinline__ int visit_lds_narrow(const KParams &P, const WaveLds &L, const int32_t (&ids)[MROWS],
+                                                int nrows)
+{
+    const uint32_t hmask = (1u << P.hlog2) - 1u;
+    uint32_t sl[MROWS], cur[MROWS], st[MROWS];
+    uint32_t pend = 0u, won = 0u;
+#pragma unroll
+    for (int r = 0; r < MROWS; r++) {
+        sl[r] = 0u;
+        cur[r] = 0u;
+        st[r] = 0u;
+        if (r < nrows && ids[r] >= 0) {
+            uint32_t q;
+            nkey((uint32_t)ids[r], P.hbits, P.hlog2, sl[r], q);
+            cur[r] = atomicCAS(&L.vh[sl[r] >> 1], 0u, nentry(q, 0u) << ((sl[r] & 1u) << 4));
+            if (cur[r] == 0u) won |= 1u << r;
+            else pend |= 1u << r;
+        }
+    }
+    bool fail = false;
+    while (__ballot(pend != 0u)) {
+#pragma unroll
+        for (int r = 0; r < MROWS; r++) {
+            if (pend & (1u << r)) {
+                uint32_t home, q;
+                nkey((uint32_t)ids[r], P.hbits, P.hlog2, home, q);
+                const uint32_t sh = (sl[r] & 1u) << 4;
+                const uint32_t h = (cur[r] >> sh) & 0xFFFFu;
+                const uint32_t want = nentry(q, st[r]);
+                if (h == want) {
+                    pend &= ~(1u << r);  // visited
+                } else if (h != 0u) {
+                    if (st[r] == 127u) {
+                        fail = true;
+                        pend &= ~(1u << r);
+                    } else {
+                        st[r]++;
+                        const uint32_t ns = (sl[r] + st[r]) & hmask;
+                        if ((ns >> 1) != (sl[r] >> 1)) cur[r] = 0u;
+                        sl[r] = ns;
+                    }
+                } else {
+                    const uint32_t old = atomicCAS(&L.vh[sl[r] >> 1], cur[r], cur[r] | (want << sh));
+                    if (old == cur[r]) {
+                        won |= 1u << r;
+                        pend &= ~(1u << r);
+                    } else {
+                        cur[r] = old;
+                    }
+                }
+            }
+        }
+    }
+    int nl = 0;
+#pragma unroll
+    for (int r = 0; r < MROWS; r++) {
+        if (r < nrows) {
+            const bool w = (won >> r) & 1u;
+            const uint64_t bal = __ballot(w);
+            if (w) L.lids[nl + mbcnt(bal)] = (uint32_t)ids[r];
+            nl += __popcll(bal);
+        }
+    }
+    return __ballot(fail) ? -1 : nl;
+}
+
 template <int NCH, int STORE, int METRIC, int MROWS>
 __device__ __forceinline__ int expand_lds(const KParams &P, const WaveLds &L, const float *qf,
                                           const int32_t (&ids)[MROWS], int nrows, int &evals EXP_PROF)
 {
+    if (P.hnarrow) {
+        const int nl = visit_lds_narrow<MROWS>(P, L, ids, nrows);
+        if (nl < 0) return -1;
+        evals += nl;
+        wave_sync();
+        PROF_MARK(2);
+        eval_rows<NCH, STORE, METRIC>(P, qf, L.lids, nl, L.lkeys);
+        wave_sync();
+        PROF_MARK(3);
+        return nl;
+    }
     const uint32_t hmask = (1u << P.hlog2) - 1u;
     // First probe of every row, issued back to back so the LDS atomics of
     // the rows overlap.  Collisions are then resolved for all rows together:
@@ -492,7 +615,8 @@ __device__ void search_query_req(const KParams &P, int64_t qi, int64_t slot, uns
     // Visited set: the LDS hash while it stays below 3/4 full (small ef);
     // a query that outgrows it freezes it and goes on with the HBM table for
     // the ids it meets from then on (frozen hash first, then the table).
-    // Large ef starts with the HBM table and the LDS Bloom filter.
+    // Large ef does the same with the 2-byte LDS hash where the graph's ids
+    // fit it, and otherwise starts with the HBM table and the LDS Bloom filter.
     bool lds_mode = P.hlog2 > 0;
     const uint32_t *frozen = nullptr;
     if (!lds_mode) T = P.vtab + ((size_t)slot << P.vlog2);
@@ -500,14 +624,19 @@ __device__ void search_query_req(const KParams &P, int64_t qi, int64_t slot, uns
     if (lds_mode) {
         uint4 *H4 = (uint4 *)L.vh;
         const uint4 z4 = make_uint4(0u, 0u, 0u, 0u);
-        for (uint32_t i = lane; i < ((1u << P.hlog2) >> 2); i += WAVE) H4[i] = z4;
+        const uint32_t n4 = (1u << P.hlog2) >> (P.hnarrow ? 3 : 2);
+        for (uint32_t i = lane; i < n4; i += WAVE) H4[i] = z4;
     } else {
         for (uint32_t w = lane; w < L.nbw; w += WAVE) L.bloom[w] = 0ull;
     }
     wave_sync();
     if (lane == 0) {
         pool[0] = mk_key(epd, ep);
-        if (lds_mode) {
+        if (lds_mode && P.hnarrow) {
+            uint32_t home, q;
+            nkey(ep, P.hbits, P.hlog2, home, q);
+            L.vh[home >> 1] = nentry(q, 0u) << ((home & 1u) << 4);
+        } else if (lds_mode) {
             L.vh[vhash(ep, P.hlog2)] = ep + 1u;
         } else {
             uint32_t w;
@@ -548,6 +677,10 @@ __device__ void search_query_req(const KParams &P, int64_t qi, int64_t slot, uns
         else
             nres = expand_group<NCH, STORE, METRIC, MROWS>(P, L, qf, T, idsP, nselP * rpc, evals,
                                                            frozen EXP_PROF_ARGS);
+        if (nres < 0) {  // probe sequence of the 2-byte hash too long
+            overflow = true;
+            break;
+        }
         haveQ = true;
         // the next group moves up
         uint32_t *ts = selP;

@@ -86,7 +86,9 @@ typedef struct gvs_params {
 /* Visited-set placement.  By default a query with ef_eff <= 128 keeps an exact
  * visited hash in LDS; a query that fills it to 3/4 freezes it and goes on
  * with an exact per-query hash in HBM for the ids it meets later.  Larger ef
- * use an LDS Bloom filter in front of the HBM hash from the start.
+ * keep an exact LDS hash of 2^13 2-byte slots (graphs of up to 2^21 nodes,
+ * one wave per query), frozen the same way; otherwise they use an LDS Bloom
+ * filter in front of the HBM hash from the start.
  * GVS_FLAG_HBM_VISITED forces the HBM form, GVS_FLAG_LDS_VISITED the LDS form
  * (visited_log2 in [10, 14] then sets the LDS hash size). */
 #define GVS_FLAG_HBM_VISITED 2

@@ -144,7 +144,30 @@ def test_visited_table_overflow_retry_is_exact():
     og = orc.OracleGraph(g.arrays())
     q = queries(100, 128, seed=12)
     # smallest table: ef=400 overflows it, the library retries with larger tables
-    check_equal(dev, og, q, 10, 400, visited_log2=11)
+    check_equal(dev, og, q, 10, 400, visited_log2=11, visited="hbm")
+
+
+@pytest.mark.parametrize("mc", [1, 4])
+def test_large_ef_lds_hash_matches_hbm_table_and_oracle(mc):
+    """ef above 128: the exact LDS hash of 2-byte slots (the default) gives
+    the oracle's results and the same as the HBM table behind a Bloom
+    filter; at ef=2500 some queries outgrow it (over ~6,000 evaluations),
+    freeze it and go on with the HBM table (frozen 2-byte lookups), still
+    exact."""
+    x, g, dev = graph(20000, 128, 16, 100)
+    og = orc.OracleGraph(g.arrays())
+    q = queries(100, 128, seed=18)
+    for ef in (200, 400):
+        print(f"case large ef mc={mc} ef={ef}", flush=True)
+        a = check_equal(dev, og, q, 10, ef, mc=mc)
+        b = dev.search(q, k=10, ef=ef, mc=mc, with_stats=True, visited="hbm")
+        for u, v in zip(a, b):
+            np.testing.assert_array_equal(u, v)
+    print(f"case large ef mc={mc} ef=2500 (freeze)", flush=True)
+    check_equal(dev, og, q, 10, 2500, mc=mc)
+    dev.search(q, k=10, ef=2500, mc=mc)
+    c = dev.counters()
+    assert c["spilled"] > 0 and c["overflowed"] == 0
 
 
 def test_recall_c1_scaled():
