@@ -295,7 +295,41 @@ extern "C" int gvs_index_device_bytes(const gvs_index *ix, int64_t *graph_bytes,
 // ---------------------------------------------------------------------------
 namespace gvs {
 
-int plan_search(const gvs_index *ix, const gvs_params *p, int max_ef, int max_k, LaunchPlan *lp)
+// One wave per query: LDS per wave, waves per block (the most resident waves
+// per CU: 160 KiB LDS, at most 16 workgroups per CU, at most 64 KiB per
+// block) and the resident wave count, for lp's tables.
+static int fit_waves(const gvs_index *ix, const void *fn, LaunchPlan *lp)
+{
+    int bytes = 16 * lp->ef_cap + 8 * lp->lcap + 8 * (1 << lp->blog2) + 4 * lp->lcap + 64 +
+                (lp->hlog2 > 0 ? (lp->hnarrow ? 2 : 4) * (1 << lp->hlog2) : 0);
+    lp->lds_per_wave = (bytes + 15) / 16 * 16;
+    lp->wpb = 1;
+    int best = 0;
+    for (int w = 1; w <= 4; w *= 2) {
+        if (w * lp->lds_per_wave > 65536) break;
+        int per_cu = std::min(16, (160 * 1024) / (w * lp->lds_per_wave)) * w;
+        if (per_cu >= best) {
+            best = per_cu;
+            lp->wpb = w;
+        }
+    }
+    if (lp->lds_per_wave > 65536) {
+        set_error("gvs_search: per-query LDS %d bytes exceeds 64 KiB (ef too large)", lp->lds_per_wave);
+        return GVS_ERR_UNSUPPORTED;
+    }
+    int blocks_per_cu = 0;
+    if (!hip_ok(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks_per_cu, fn, lp->wpb * WAVE,
+                                                             (size_t)lp->wpb * lp->lds_per_wave),
+                "hipOccupancyMaxActiveBlocksPerMultiprocessor"))
+        return GVS_ERR_HIP;
+    lp->resident_waves = std::max(1, blocks_per_cu) * ix->cu_count * lp->wpb;
+    return GVS_OK;
+}
+
+// max_waves > 0: the launch runs at most that many waves (a server's grid),
+// so resident waves beyond it do not count when the visited hash is sized.
+int plan_search(const gvs_index *ix, const gvs_params *p, int max_ef, int max_k, LaunchPlan *lp,
+                int64_t max_waves = 0)
 {
     if (!p) {
         set_error("gvs_search: params is NULL");
@@ -349,29 +383,31 @@ int plan_search(const gvs_index *ix, const gvs_params *p, int max_ef, int max_k,
     const bool lds_sized = (p->flags & GVS_FLAG_LDS_VISITED) && p->visited_log2 >= 10 && p->visited_log2 <= 14;
     lp->vlog2 = p->visited_log2 > 0 && !lds_sized ? std::max(p->visited_log2, vmin)
                                                    : std::max(vmin, std::min(20, ilog2_ceil((int64_t)efe * 128)));
-    // exact visited set in LDS for small ef (no HBM table), unless disabled
+    // exact visited set in LDS (no HBM table until it fills), unless disabled
     lp->hlog2 = 0;
-    // (2^12 slots hold the p99 query at ef 64-96 on degree-64 graphs; 2^13
-    // costs more in occupancy than the spills it saves)
-    if (!(p->flags & GVS_FLAG_HBM_VISITED) && ((p->flags & GVS_FLAG_LDS_VISITED) || efe <= 128)) {
+    // ef in (128, 512]: an exact LDS hash of 2-byte slots (a node's quotient
+    // and probe step), when a node's quotient fits 8 bits: 2^14 slots, or
+    // 2^13 where 2^14 would lower the resident wave count (below).  Few
+    // queries fill 2^13 slots to 3/4 at ef 512 on degree-64 graphs; those
+    // go on with the slower frozen lookups plus the HBM table.
+    const int nlog2 = lds_sized ? p->visited_log2 : 14;
+    const int nbits = std::max(nlog2, ilog2_ceil(std::max<int64_t>(ix->n, 2)));
+    const bool narrow = !(p->flags & GVS_FLAG_HBM_VISITED) && W == 1 && efe > 128 && nbits <= nlog2 + 8 &&
+                        ((p->flags & GVS_FLAG_LDS_VISITED) || efe <= 512);
+    if (narrow) {
+        lp->hlog2 = nlog2;
+        lp->hnarrow = 1;
+        lp->hbits = nbits;
+        lp->blog2 = p->bloom_log2 > 0 ? p->bloom_log2 : 1;
+    } else if (!(p->flags & GVS_FLAG_HBM_VISITED) && ((p->flags & GVS_FLAG_LDS_VISITED) || efe <= 128)) {
+        // 4-byte slots (2^12 hold the p99 query at ef 64-96 on degree-64
+        // graphs; 2^13 costs more in occupancy than the spills it saves)
         int want = ilog2_ceil(((int64_t)efe * 40 + lp->lcap) * 4 / 3);
         lp->hlog2 = lds_sized ? p->visited_log2 : std::max(11, std::min(12, want));
         // Bloom filter for the ids met after the hash is frozen; by default
         // none (2 words keep the hash 16-byte aligned): those ids are then
         // all gathered speculatively
         lp->blog2 = p->bloom_log2 > 0 ? p->bloom_log2 : 1;
-    } else if (!(p->flags & GVS_FLAG_HBM_VISITED) && W == 1) {
-        // larger ef: an exact LDS hash of 2^13 2-byte slots (a node's
-        // quotient and probe step) in the 16 KB a Bloom filter would take,
-        // when a node's quotient fits 8 bits; it holds ~6,000 evaluations
-        // before the query freezes it and goes on with the HBM table
-        const int nbits = std::max(13, ilog2_ceil(std::max<int64_t>(ix->n, 2)));
-        if (nbits <= 13 + 8) {
-            lp->hlog2 = 13;
-            lp->hnarrow = 1;
-            lp->hbits = nbits;
-            lp->blog2 = p->bloom_log2 > 0 ? p->bloom_log2 : 1;
-        }
     }
     if (lp->blog2 > 13 || lp->vlog2 > 26) {
         set_error("gvs_search: bloom_log2 <= 13 and visited_log2 <= 26 required");
@@ -396,32 +432,24 @@ int plan_search(const gvs_index *ix, const gvs_params *p, int max_ef, int max_k,
             return GVS_ERR_HIP;
         return GVS_OK;
     }
-    int bytes = 16 * lp->ef_cap + 8 * lp->lcap + 8 * (1 << lp->blog2) + 4 * lp->lcap + 64 +
-                (lp->hlog2 > 0 ? (lp->hnarrow ? 2 : 4) * (1 << lp->hlog2) : 0);
-    lp->lds_per_wave = (bytes + 15) / 16 * 16;
-    // waves per block: the most resident waves per CU (160 KiB LDS, at most
-    // 16 workgroups per CU, at most 64 KiB per block)
-    lp->wpb = 1;
-    int best = 0;
-    for (int w = 1; w <= 4; w *= 2) {
-        if (w * lp->lds_per_wave > 65536) break;
-        int per_cu = std::min(16, (160 * 1024) / (w * lp->lds_per_wave)) * w;
-        if (per_cu >= best) {
-            best = per_cu;
-            lp->wpb = w;
+    const int rc = fit_waves(ix, (const void *)fn, lp);
+    if (narrow && !lds_sized && lp->hlog2 == 14) {
+        // 2^13 slots instead where the larger table would cost resident
+        // waves or not fit: a chain-bound search gains more from waves
+        // than from the shorter probe sequences of a half-empty table
+        LaunchPlan alt = *lp;
+        alt.hlog2 = 13;
+        alt.hbits = std::max(13, ilog2_ceil(std::max<int64_t>(ix->n, 2)));
+        const auto waves = [&](const LaunchPlan &l) {
+            return max_waves > 0 ? std::min<int64_t>(l.resident_waves, max_waves) : (int64_t)l.resident_waves;
+        };
+        if (alt.hbits <= 13 + 8 && fit_waves(ix, (const void *)fn, &alt) == GVS_OK &&
+            (rc != GVS_OK || waves(alt) > waves(*lp))) {
+            *lp = alt;
+            return GVS_OK;
         }
     }
-    if (lp->lds_per_wave > 65536) {
-        set_error("gvs_search: per-query LDS %d bytes exceeds 64 KiB (ef too large)", lp->lds_per_wave);
-        return GVS_ERR_UNSUPPORTED;
-    }
-    int blocks_per_cu = 0;
-    if (!hip_ok(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks_per_cu, (const void *)fn, lp->wpb * WAVE,
-                                                             (size_t)lp->wpb * lp->lds_per_wave),
-                "hipOccupancyMaxActiveBlocksPerMultiprocessor"))
-        return GVS_ERR_HIP;
-    lp->resident_waves = std::max(1, blocks_per_cu) * ix->cu_count * lp->wpb;
-    return GVS_OK;
+    return rc;
 }
 
 int ensure_scratch(gvs_index *ix, int64_t slots, int vlog2, int64_t nstatus)

@@ -32,7 +32,8 @@
 using namespace gvsk;
 
 namespace gvs {
-int plan_search(const gvs_index *ix, const gvs_params *p, int max_ef, int max_k, LaunchPlan *lp);
+int plan_search(const gvs_index *ix, const gvs_params *p, int max_ef, int max_k, LaunchPlan *lp,
+                int64_t max_waves);
 int ensure_scratch(gvs_index *ix, int64_t slots, int vlog2, int64_t nstatus);
 void fill_params(const gvs_index *ix, const gvs_params *p, const LaunchPlan &lp, KParams &P);
 }  // namespace gvs
@@ -155,7 +156,10 @@ extern "C" int gvs_server_open(gvs_index *ix, const gvs_server_config *cfg, gvs_
     p.waves_per_query = 1;
     p.storage = GVS_STORE_F32;
     p.persistent = 1;
-    int rc = gvs::plan_search(ix, &p, cfg->max_ef, cfg->max_k, &s->lp);
+    // the grid holds at most grid_waves (or max_inflight) waves: the
+    // visited hash may use LDS that only waves beyond those would need
+    int rc = gvs::plan_search(ix, &p, cfg->max_ef, cfg->max_k, &s->lp,
+                              cfg->grid_waves > 0 ? cfg->grid_waves : cfg->max_inflight);
     if (rc) {
         delete s;
         return rc;

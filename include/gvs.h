@@ -85,12 +85,13 @@ typedef struct gvs_params {
 #define GVS_FLAG_ASYNC 1
 /* Visited-set placement.  By default a query with ef_eff <= 128 keeps an exact
  * visited hash in LDS; a query that fills it to 3/4 freezes it and goes on
- * with an exact per-query hash in HBM for the ids it meets later.  Larger ef
- * keep an exact LDS hash of 2^13 2-byte slots (graphs of up to 2^21 nodes,
- * one wave per query), frozen the same way; otherwise they use an LDS Bloom
- * filter in front of the HBM hash from the start.
- * GVS_FLAG_HBM_VISITED forces the HBM form, GVS_FLAG_LDS_VISITED the LDS form
- * (visited_log2 in [10, 14] then sets the LDS hash size). */
+ * with an exact per-query hash in HBM for the ids it meets later.  ef_eff in
+ * (128, 512] keeps an exact LDS hash of 2-byte slots (2^13 up to 256, 2^14
+ * above; graphs of up to 2^(slots' log2 + 8) nodes, one wave per query),
+ * frozen the same way; larger ef use an LDS Bloom filter in front of the HBM
+ * hash from the start.  GVS_FLAG_HBM_VISITED forces the HBM form,
+ * GVS_FLAG_LDS_VISITED the LDS form (visited_log2 in [10, 14] then sets the
+ * LDS hash size; its slots are 2 bytes when ef_eff > 128 and the graph fits). */
 #define GVS_FLAG_HBM_VISITED 2
 #define GVS_FLAG_LDS_VISITED 4
 /* Region order (persistent mode, no given entries, graphs with >= 2 upper
@@ -248,7 +249,9 @@ typedef struct gvs_server_config {
     int32_t max_k;             /* largest k accepted */
     int32_t max_ef;            /* largest ef accepted */
     int32_t queue_capacity;    /* reserved, 0 (requests wait in the caller until a slot frees) */
-    int32_t grid_waves;        /* persistent workers, 0 = auto */
+    int32_t grid_waves;        /* persistent workers, 0 = auto (resident
+                                * waves, at most max_inflight); heavy
+                                * mixes can saturate higher below that */
     int32_t mc;                /* DST candidates per step used for every request */
     int32_t storage;           /* vector copy traversed */
     int32_t reserved;

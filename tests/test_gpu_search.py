@@ -149,11 +149,11 @@ def test_visited_table_overflow_retry_is_exact():
 
 @pytest.mark.parametrize("mc", [1, 4])
 def test_large_ef_lds_hash_matches_hbm_table_and_oracle(mc):
-    """ef above 128: the exact LDS hash of 2-byte slots (the default) gives
-    the oracle's results and the same as the HBM table behind a Bloom
-    filter; at ef=2500 some queries outgrow it (over ~6,000 evaluations),
-    freeze it and go on with the HBM table (frozen 2-byte lookups), still
-    exact."""
+    """ef in (128, 512]: the exact LDS hash of 2-byte slots (the default)
+    gives the oracle's results and the same as the HBM table behind a Bloom
+    filter.  Forced down to 2^10 slots at ef=200, every query outgrows it,
+    freezes it and goes on with the HBM table (frozen 2-byte lookups), still
+    exact.  ef=1000 uses the Bloom filter and HBM table by default."""
     x, g, dev = graph(20000, 128, 16, 100)
     og = orc.OracleGraph(g.arrays())
     q = queries(100, 128, seed=18)
@@ -163,11 +163,13 @@ def test_large_ef_lds_hash_matches_hbm_table_and_oracle(mc):
         b = dev.search(q, k=10, ef=ef, mc=mc, with_stats=True, visited="hbm")
         for u, v in zip(a, b):
             np.testing.assert_array_equal(u, v)
-    print(f"case large ef mc={mc} ef=2500 (freeze)", flush=True)
-    check_equal(dev, og, q, 10, 2500, mc=mc)
-    dev.search(q, k=10, ef=2500, mc=mc)
+    print(f"case large ef mc={mc} ef=200 2-byte hash of 2^10 slots (freeze)", flush=True)
+    check_equal(dev, og, q, 10, 200, mc=mc, visited="lds", visited_log2=10)
+    dev.search(q, k=10, ef=200, mc=mc, visited="lds", visited_log2=10)
     c = dev.counters()
     assert c["spilled"] > 0 and c["overflowed"] == 0
+    print(f"case large ef mc={mc} ef=1000 (Bloom filter + HBM table)", flush=True)
+    check_equal(dev, og, q[:40], 10, 1000, mc=mc)
 
 
 def test_recall_c1_scaled():
