@@ -285,6 +285,15 @@ def test_low_precision_traversal_and_fp32_rerank(storage):
     ri, rd = orc.rerank_f32(og, q, cand, 10)
     np.testing.assert_array_equal(lab, ri)
     np.testing.assert_array_equal(dist, rd)
+    # ef above 128: pools of several 64-entry chunks, the 2-byte LDS visited
+    # hash, and a rerank over four chunks of candidates
+    print(f"case low precision storage={storage} ef=200", flush=True)
+    check_equal(dev, og, q, 10, 200, storage=storage)
+    lab2, dist2 = dev.search(q, k=10, ef=200, storage=storage, rerank=True)
+    cand2, _, _ = og.search(q, 200, 200, order=orc.WAVE)
+    ri2, rd2 = orc.rerank_f32(og, q, cand2, 10)
+    np.testing.assert_array_equal(lab2, ri2)
+    np.testing.assert_array_equal(dist2, rd2)
     gt, gd = orc.bruteforce_f64(x, q, 10)
     lab32, d32 = dev.search(q, k=10, ef=64)
     r32, rlp = orc.recall_at(lab32, gt, 10), orc.recall_at(lab, gt, 10)
