@@ -223,7 +223,9 @@ class HostGraph:
 
 def make_params(k=10, ef=64, mc=1, mg=1, waves_per_query=1, storage=F32, rerank=False, persistent=False,
                 visited_log2=0, bloom_log2=0, grid_waves=0, asynchronous=False, visited=None, region_order=False):
-    """visited: None (auto), "hbm" (LDS Bloom + HBM hash) or "lds" (exact LDS hash).
+    """visited: None (auto), "hbm" (LDS Bloom + HBM hash) or "lds" (exact LDS
+    hash; 2-byte slots when max(ef, k) > 128 and the graph's ids fit them;
+    visited_log2 in [10, 14] then sets its size).
     region_order: run the batch in region order (persistent mode; GVS_FLAG_REGION_ORDER)."""
     flags = FLAG_ASYNC if asynchronous else 0
     if region_order:
