@@ -202,9 +202,9 @@ def serve_child(a, gvs, g, q, ks=None, efs=None, metric_name="L2", max_inflight=
     (p99 at 600k requests/s: 0.32 ms with 256 slots, 0.70 ms with 4096)
     while the saturated rate stays above 1M/s.  Heavy requests (ef up to
     512 at D=384) need more waves in flight, but not every resident one:
-    that mix saturates at 491k requests/s on 768 waves (3 per CU) against
-    293k on all 1,024 resident ones (and 382k on 512).  grid_waves: the
-    serving grid (0 = resident waves, at most max_inflight)."""
+    over 30k requests that mix saturates at 620k requests/s on 768 waves
+    (3 per CU) against about 300k on 896 or all 1,024 resident ones.
+    grid_waves: the serving grid (0 = resident waves, at most max_inflight)."""
     dev = gvs.DeviceIndex(g)
     nreq = min(len(q), 20000)
     qq = q[:nreq]
