@@ -645,6 +645,13 @@ int launch_search_ix(gvs_index *ix, const gvs_params *p, const LaunchPlan &lp, c
     return launch_search(ix, p, lp, P, nq, stream, ix->d_vtab, ix->d_status, region);
 }
 
+// The most waves a batch of nq queries keeps busy: one per query, or the
+// persistent grid when one is given.
+static int64_t batch_waves(const gvs_params *p, int64_t nq)
+{
+    return p->persistent && p->grid_waves > 0 ? std::min<int64_t>(nq, p->grid_waves) : nq;
+}
+
 }  // namespace gvs
 
 static int search_impl(gvs_index *ix, const float *queries, int64_t nq, int64_t ldq, const int32_t *entries,
@@ -660,7 +667,7 @@ static int search_impl(gvs_index *ix, const float *queries, int64_t nq, int64_t 
         return GVS_ERR_ARG;
     }
     gvs::LaunchPlan lp;
-    int rc = gvs::plan_search(ix, p, p->ef, p->k, &lp);
+    int rc = gvs::plan_search(ix, p, p->ef, p->k, &lp, gvs::batch_waves(p, nq));
     if (rc) return rc;
     KParams P;
     gvs::fill_params(ix, p, lp, P);
@@ -925,7 +932,7 @@ extern "C" int gvs_search_graph_create(gvs_index *ix, const float *queries, int6
         return GVS_ERR_ARG;
     }
     gvs::LaunchPlan lp;
-    int rc = gvs::plan_search(ix, p, p->ef, p->k, &lp);
+    int rc = gvs::plan_search(ix, p, p->ef, p->k, &lp, gvs::batch_waves(p, nq));
     if (rc) return rc;
     KParams P;
     gvs::fill_params(ix, p, lp, P);
