@@ -199,7 +199,7 @@ def serve_child(a, gvs, g, q, ks=None, efs=None, metric_name="L2", max_inflight=
     requests (1M x 128, ef 60) are served best by one wave per CU (256):
     idle serving waves poll the host ring with system-scope acquires, which
     invalidate L2 under the running searches, so extra waves add latency
-    (p99 at 600k requests/s: 0.32 ms with 256 slots, 0.70 ms with 4096)
+    (p99 at 600k requests/s: 0.29 ms with 256 slots, 0.39 ms with 4096)
     while the saturated rate stays above 1M/s.  Heavy requests (ef up to
     512 at D=384) need more waves in flight, but not every resident one:
     over 30k requests that mix saturates at 620k requests/s on 768 waves

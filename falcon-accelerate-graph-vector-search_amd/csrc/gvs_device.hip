@@ -401,7 +401,8 @@ int plan_search(const gvs_index *ix, const gvs_params *p, int max_ef, int max_k,
         lp->blog2 = p->bloom_log2 > 0 ? p->bloom_log2 : 1;
     } else if (!(p->flags & GVS_FLAG_HBM_VISITED) && ((p->flags & GVS_FLAG_LDS_VISITED) || efe <= 128)) {
         // 4-byte slots (2^12 hold the p99 query at ef 64-96 on degree-64
-        // graphs; 2^13 costs more in occupancy than the spills it saves)
+        // graphs; in a full batch 2^13 costs more in occupancy than the
+        // spills it saves)
         int want = ilog2_ceil(((int64_t)efe * 40 + lp->lcap) * 4 / 3);
         lp->hlog2 = lds_sized ? p->visited_log2 : std::max(11, std::min(12, want));
         // Bloom filter for the ids met after the hash is frozen; by default
@@ -433,18 +434,27 @@ int plan_search(const gvs_index *ix, const gvs_params *p, int max_ef, int max_k,
         return GVS_OK;
     }
     const int rc = fit_waves(ix, (const void *)fn, lp);
+    // A default-sized LDS hash takes the larger of two sizes unless that
+    // costs waves the launch can use: the shorter probe sequences of an
+    // emptier table help, but a chain-bound search gains more from waves.
+    const auto waves = [&](const LaunchPlan &l) {
+        return max_waves > 0 ? std::min<int64_t>(l.resident_waves, max_waves) : (int64_t)l.resident_waves;
+    };
     if (narrow && !lds_sized && lp->hlog2 == 14) {
-        // 2^13 slots instead where the larger table would cost resident
-        // waves or not fit: a chain-bound search gains more from waves
-        // than from the shorter probe sequences of a half-empty table
+        // 2-byte slots: 2^14, else 2^13
         LaunchPlan alt = *lp;
         alt.hlog2 = 13;
         alt.hbits = std::max(13, ilog2_ceil(std::max<int64_t>(ix->n, 2)));
-        const auto waves = [&](const LaunchPlan &l) {
-            return max_waves > 0 ? std::min<int64_t>(l.resident_waves, max_waves) : (int64_t)l.resident_waves;
-        };
         if (alt.hbits <= 13 + 8 && fit_waves(ix, (const void *)fn, &alt) == GVS_OK &&
             (rc != GVS_OK || waves(alt) > waves(*lp))) {
+            *lp = alt;
+            return GVS_OK;
+        }
+    } else if (!narrow && !lds_sized && W == 1 && lp->hlog2 > 0 && lp->hlog2 < 13 && rc == GVS_OK) {
+        // 4-byte slots: the size ef asks for, or twice that
+        LaunchPlan alt = *lp;
+        alt.hlog2 = lp->hlog2 + 1;
+        if (fit_waves(ix, (const void *)fn, &alt) == GVS_OK && waves(alt) >= waves(*lp)) {
             *lp = alt;
             return GVS_OK;
         }
