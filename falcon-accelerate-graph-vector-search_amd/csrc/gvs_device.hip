@@ -387,7 +387,7 @@ int plan_search(const gvs_index *ix, const gvs_params *p, int max_ef, int max_k,
     lp->hlog2 = 0;
     // ef in (128, 512]: an exact LDS hash of 2-byte slots (a node's quotient
     // and probe step), when a node's quotient fits 8 bits: 2^14 slots, or
-    // 2^13 where 2^14 would lower the resident wave count (below).  Few
+    // 2^13 where 2^14 would cost waves the launch can use (below).  Few
     // queries fill 2^13 slots to 3/4 at ef 512 on degree-64 graphs; those
     // go on with the slower frozen lookups plus the HBM table.
     const int nlog2 = lds_sized ? p->visited_log2 : 14;
