@@ -872,6 +872,21 @@ extern "C" int gvs_autotune(gvs_index *ix, const float *queries, int64_t nq, int
                 cand.push_back(c);
             }
     }
+    // Above ef 128 the visited set's placement is timed too, unless the
+    // caller fixed it: the LDS hash costs fewer instructions and HBM atomics
+    // when many waves search at once, while the Bloom filter + HBM table
+    // overlaps each id's check with its gather, which a small batch's few
+    // waves feel more (on 1M x 128 at ef 192-512 the plain mc = 1 search
+    // is 4-18% faster with it up to 1,024 queries, slower for full batches)
+    if (std::max(p->ef, p->k) > 128 && !(p->flags & (GVS_FLAG_HBM_VISITED | GVS_FLAG_LDS_VISITED))) {
+        const size_t n0 = cand.size();
+        for (size_t i = 0; i < n0; i++)
+            if (cand[i].waves_per_query == 1) {
+                gvs_params h = cand[i];
+                h.flags |= GVS_FLAG_HBM_VISITED;
+                cand.push_back(h);
+            }
+    }
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (!hip_ok(hipEventCreate(&e0), "hipEventCreate") || !hip_ok(hipEventCreate(&e1), "hipEventCreate")) {
         if (e0) (void)hipEventDestroy(e0);

@@ -297,6 +297,8 @@ class DeviceIndex:
         cfg = dict(ef=best.ef, mc=best.mc, mg=best.mg, persistent=bool(best.persistent),
                    waves_per_query=best.waves_per_query, grid_waves=best.grid_waves,
                    region_order=bool(best.flags & FLAG_REGION_ORDER))
+        if best.flags & FLAG_HBM_VISITED:
+            cfg["visited"] = "hbm"
         return cfg, ms.value
 
     def capture(self, q_ptr, nq, ldq, labels_ptr, dists_ptr, k=10, ef=64, entries_ptr=None, stats_ptr=None, **kw):

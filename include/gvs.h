@@ -203,7 +203,9 @@ GVS_API int gvs_device_copy(void *dst, const void *src, int64_t bytes, void *str
  * and flags from *p; labels/dists are device result buffers it may overwrite)
  * over mc in {1,2,4,8} (as r0 allows) x mg in {1,2} x persistent in {0,1}
  * (persistent also at half the resident waves), and for nq <= 64 also
- * waves_per_query in {2,4,8}; writes the fastest configuration to *best and
+ * waves_per_query in {2,4,8}; above ef_eff 128, unless p->flags fixes the
+ * visited placement, each one-wave candidate also with GVS_FLAG_HBM_VISITED
+ * (best->flags says which won); writes the fastest configuration to *best and
  * its time per launch to *best_ms.  DST settings trade evaluations for steps
  * and can change recall: checking recall is the caller's part.  Replaces the
  * reference's exhaustive (mc, mg) timing, perf_test_scripts/perf_test_throughput.py:83-133. */

@@ -393,6 +393,12 @@ def test_autotuner_returns_a_working_configuration(nq):
     assert cfg["waves_per_query"] in ((1, 2, 4, 8) if nq <= 64 else (1,))
     kw = {kk: v for kk, v in cfg.items() if kk not in ("ef", "mc", "mg")}
     check_equal(dev, og, q, 10, 64, mc=cfg["mc"], mg=cfg["mg"], **kw)
+    # above ef 128 the visited-set placement is one of the timed choices
+    print(f"case autotune nq={nq} ef=200", flush=True)
+    cfg, ms = dev.autotune(dq.address, nq, 128, dl.address, dd.address, k=10, ef=200)
+    assert ms > 0 and cfg["ef"] == 200 and cfg.get("visited") in (None, "hbm")
+    kw = {kk: v for kk, v in cfg.items() if kk not in ("ef", "mc", "mg")}
+    check_equal(dev, og, q, 10, 200, mc=cfg["mc"], mg=cfg["mg"], **kw)
 
 
 @pytest.mark.parametrize("mc", [1, 2])
