@@ -387,9 +387,10 @@ int plan_search(const gvs_index *ix, const gvs_params *p, int max_ef, int max_k,
     lp->hlog2 = 0;
     // ef in (128, 512]: an exact LDS hash of 2-byte slots (a node's quotient
     // and probe step), when a node's quotient fits 8 bits: 2^14 slots, or
-    // 2^13 where 2^14 would cost waves the launch can use (below).  Few
-    // queries fill 2^13 slots to 3/4 at ef 512 on degree-64 graphs; those
-    // go on with the slower frozen lookups plus the HBM table.
+    // 2^13 where 2^14 would cost waves the launch can use, or 2^13 4-byte
+    // slots in the same 32 KiB where those cost none (below).  Few queries
+    // fill 2^13 slots to 3/4 at ef 512 on degree-64 graphs; those go on
+    // with the slower frozen lookups plus the HBM table.
     const int nlog2 = lds_sized ? p->visited_log2 : 14;
     const int nbits = std::max(nlog2, ilog2_ceil(std::max<int64_t>(ix->n, 2)));
     const bool narrow = !(p->flags & GVS_FLAG_HBM_VISITED) && W == 1 && efe > 128 && nbits <= nlog2 + 8 &&
@@ -442,14 +443,27 @@ int plan_search(const gvs_index *ix, const gvs_params *p, int max_ef, int max_k,
     };
     if (narrow && !lds_sized && lp->hlog2 == 14) {
         // 2-byte slots: 2^14, else 2^13
+        LaunchPlan best = *lp;
+        int best_rc = rc;
         LaunchPlan alt = *lp;
         alt.hlog2 = 13;
         alt.hbits = std::max(13, ilog2_ceil(std::max<int64_t>(ix->n, 2)));
         if (alt.hbits <= 13 + 8 && fit_waves(ix, (const void *)fn, &alt) == GVS_OK &&
-            (rc != GVS_OK || waves(alt) > waves(*lp))) {
-            *lp = alt;
-            return GVS_OK;
+            (best_rc != GVS_OK || waves(alt) > waves(best))) {
+            best = alt;
+            best_rc = GVS_OK;
         }
+        // 2^13 4-byte slots (32 KiB, as 2^14 2-byte ones) where they cost no
+        // waves the launch can use: a 4-byte slot is checked with fewer
+        // dependent instructions, which few waves feel most
+        LaunchPlan a4 = *lp;
+        a4.hlog2 = 13;
+        a4.hnarrow = 0;
+        a4.hbits = 0;
+        if (best_rc == GVS_OK && fit_waves(ix, (const void *)fn, &a4) == GVS_OK && waves(a4) >= waves(best))
+            best = a4;
+        *lp = best;
+        return best_rc;
     } else if (!narrow && !lds_sized && W == 1 && lp->hlog2 > 0 && lp->hlog2 < 13 && rc == GVS_OK) {
         // 4-byte slots: the size ef asks for, or twice that
         LaunchPlan alt = *lp;

@@ -86,10 +86,11 @@ typedef struct gvs_params {
 /* Visited-set placement.  By default a query with ef_eff <= 128 keeps an exact
  * visited hash in LDS; a query that fills it to 3/4 freezes it and goes on
  * with an exact per-query hash in HBM for the ids it meets later.  ef_eff in
- * (128, 512] keeps an exact LDS hash of 2-byte slots (2^13 up to 256, 2^14
- * above; graphs of up to 2^(slots' log2 + 8) nodes, one wave per query),
- * frozen the same way; larger ef use an LDS Bloom filter in front of the HBM
- * hash from the start.  GVS_FLAG_HBM_VISITED forces the HBM form,
+ * (128, 512] keeps an exact LDS hash too (one wave per query): 2^13 4-byte
+ * slots where their 32 KiB cost no waves the launch can use, else 2-byte
+ * slots (2^14 or 2^13; graphs of up to 2^(slots' log2 + 8) nodes), frozen
+ * the same way; larger ef use an LDS Bloom filter in front of the HBM hash
+ * from the start.  GVS_FLAG_HBM_VISITED forces the HBM form,
  * GVS_FLAG_LDS_VISITED the LDS form (visited_log2 in [10, 14] then sets the
  * LDS hash size; its slots are 2 bytes when ef_eff > 128 and the graph fits). */
 #define GVS_FLAG_HBM_VISITED 2
