@@ -890,8 +890,9 @@ extern "C" int gvs_autotune(gvs_index *ix, const float *queries, int64_t nq, int
     // caller fixed it: the LDS hash costs fewer instructions and HBM atomics
     // when many waves search at once, while the Bloom filter + HBM table
     // overlaps each id's check with its gather, which a small batch's few
-    // waves feel more (on 1M x 128 at ef 192-512 the plain mc = 1 search
-    // is 4-18% faster with it up to 1,024 queries, slower for full batches)
+    // waves feel more (on 1M x 128 at ef 192-512 the plain mc = 1 search of
+    // one query runs about as fast with it as with 4-byte LDS slots, and of
+    // up to 1,024 queries 4-18% faster than with 2-byte slots)
     if (std::max(p->ef, p->k) > 128 && !(p->flags & (GVS_FLAG_HBM_VISITED | GVS_FLAG_LDS_VISITED))) {
         const size_t n0 = cand.size();
         for (size_t i = 0; i < n0; i++)
