@@ -172,6 +172,25 @@ def test_large_ef_lds_hash_matches_hbm_table_and_oracle(mc):
     check_equal(dev, og, q[:40], 10, 1000, mc=mc)
 
 
+def test_large_ef_full_batch_two_byte_hash():
+    """A batch larger than the resident grid at ef=200 takes the 2-byte LDS
+    hash (2^13 slots: 4-byte slots in 32 KiB would cost resident waves):
+    its results equal the HBM table's for every query and the oracle's on a
+    sample."""
+    x, g, dev = graph(20000, 128, 16, 100)
+    og = orc.OracleGraph(g.arrays())
+    q = queries(4000, 128, seed=19)
+    print("case large ef full batch ef=200", flush=True)
+    a = dev.search(q, k=10, ef=200, with_stats=True)
+    b = dev.search(q, k=10, ef=200, with_stats=True, visited="hbm")
+    for u, v in zip(a, b):
+        np.testing.assert_array_equal(u, v)
+    ids, ds, ost = og.search(q[:300], 10, 200, order=orc.WAVE)
+    np.testing.assert_array_equal(a[0][:300], ids.astype(np.int64))
+    np.testing.assert_array_equal(a[1][:300], ds)
+    np.testing.assert_array_equal(a[2][:300].astype(np.int64), ost)
+
+
 def test_recall_c1_scaled():
     """100k x 128, M=16, efC=128, k=10, ef=64 (the reference's CPU case):
     GPU recall@10 equals the oracle's (same ids) and is reported."""
