@@ -60,7 +60,7 @@ struct KParams {
     uint32_t *vtab;
     int vlog2;
     int blog2;
-    int hlog2;                 // > 0: exact visited hash in LDS (2^hlog2 slots), no HBM table
+    int hlog2;                 // > 0: exact visited hash in LDS (2^hlog2 slots); the HBM table only once it is frozen
     int hnarrow;               // LDS hash slots of 2 bytes (quotient + probe step) instead of 4 (id + 1)
     int hbits;                 // id bits the 2-byte slots scramble (hlog2 <= hbits <= hlog2 + 8)
     uint32_t bloom_seed;
