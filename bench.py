@@ -479,8 +479,28 @@ def main():
                 stream.synchronize()
             res[name] = round((time.perf_counter() - t0) / reps * 1e6, 1)
         sg.close()
+        # the library's autotuner on the same small batch (mc, mg, persistent,
+        # waves per query), timed as the eager call above; DST can change the
+        # results, so its recall over the whole query set is reported with it
+        cfg_t, _ = dev.autotune(dq.address, nb, a.dim, dlab.address, ddist.address, k=a.k, ef=plain_ef,
+                                stream=stream.handle)
+
+        def tuned_call():
+            dev.search_device(dq.address, nb, a.dim, dlab.address, ddist.address, k=a.k, stream=stream.handle,
+                              asynchronous=True, **cfg_t)
+        for _ in range(10):
+            tuned_call()
+        stream.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            tuned_call()
+            stream.synchronize()
+        res["autotuned"] = {"us": round((time.perf_counter() - t0) / reps * 1e6, 1),
+                            "config": {kk: v for kk, v in cfg_t.items() if kk != "ef"},
+                            "recall_at_10": round(measure_recall(cfg_t), 4)}
         small.append(res)
-        log(f"[bench] batch {nb}: eager {res['eager_us']} us, graph replay {res['graph_us']} us per call")
+        log(f"[bench] batch {nb}: eager {res['eager_us']} us, graph replay {res['graph_us']} us per call; "
+            f"autotuned {res['autotuned']}")
 
     serving = None
     if not a.no_serving:
