@@ -149,7 +149,7 @@ def intra_latency(a, gvs, g, q):
     dq = H.DeviceArray((1, a.dim), np.float32)
     dl = H.DeviceArray((1, a.k), np.int64)
     dd = H.DeviceArray((1, a.k), np.float32)
-    ds = H.DeviceArray((1, 4), np.int32)
+    ds = H.DeviceArray((1, gvs.STATS_ROW), np.int32)
     dqa = H.DeviceArray.from_host(q[:nq])
     gti = H.DeviceArray((nq, a.k), np.int32)
     gtd = H.DeviceArray((nq, a.k), np.float64)
@@ -337,7 +337,7 @@ def main():
     dq = H.DeviceArray.from_host(q)
     dlab = H.DeviceArray((a.nq, a.k), np.int64)
     ddist = H.DeviceArray((a.nq, a.k), np.float32)
-    dstat = H.DeviceArray((a.nq, 4), np.int32)
+    dstat = H.DeviceArray((a.nq, gvs.STATS_ROW), np.int32)
     gti = H.DeviceArray((a.nq, a.k), np.int32)
     gtd = H.DeviceArray((a.nq, a.k), np.float64)
     t0 = time.time()

@@ -31,18 +31,21 @@ __global__ __launch_bounds__(256) void k_copy16(const uint4 *__restrict__ src, u
         dst[i] = src[i];
 }
 
-static dfn_t pick_descend(int storage, int metric, int nch)
+// seq: fp32 rows summed in hnswlib's scalar order (GVS_FLAG_HNSWLIB_ORDER)
+static dfn_t pick_descend(int storage, int metric, int nch, bool seq)
 {
     const bool l2 = metric == GVS_METRIC_L2;
+    if (seq) return storage == GVS_STORE_F32 ? (l2 ? gvs_descend_f32seq_l2(nch) : gvs_descend_f32seq_ip(nch)) : nullptr;
     if (storage == GVS_STORE_F32) return l2 ? gvs_descend_f32_l2(nch) : gvs_descend_f32_ip(nch);
     if (storage == GVS_STORE_BF16) return l2 ? gvs_descend_bf16_l2(nch) : gvs_descend_bf16_ip(nch);
     if (storage == GVS_STORE_FP8) return l2 ? gvs_descend_fp8_l2(nch) : gvs_descend_fp8_ip(nch);
     return nullptr;
 }
 
-static kfn_t pick_kernel(int storage, int metric, int nch, int rows)
+static kfn_t pick_kernel(int storage, int metric, int nch, int rows, bool seq)
 {
     const bool l2 = metric == GVS_METRIC_L2;
+    if (seq) return storage == GVS_STORE_F32 ? (l2 ? gvs_kernel_f32seq_l2(nch, rows) : gvs_kernel_f32seq_ip(nch, rows)) : nullptr;
     if (storage == GVS_STORE_F32) return l2 ? gvs_kernel_f32_l2(nch, rows) : gvs_kernel_f32_ip(nch, rows);
     if (storage == GVS_STORE_BF16) return l2 ? gvs_kernel_bf16_l2(nch, rows) : gvs_kernel_bf16_ip(nch, rows);
     if (storage == GVS_STORE_FP8) return l2 ? gvs_kernel_fp8_l2(nch, rows) : gvs_kernel_fp8_ip(nch, rows);
@@ -363,17 +366,24 @@ int plan_search(const gvs_index *ix, const gvs_params *p, int max_ef, int max_k,
         const int rpw = (p->mc * rpc + W - 1) / W;
         wrows = rpw <= 1 ? 1 : rpw <= 2 ? 2 : 4;
     }
+    const bool seq = (p->flags & GVS_FLAG_HNSWLIB_ORDER) != 0;
+    if (seq && p->storage != GVS_STORE_F32) {
+        set_error("gvs_search: GVS_FLAG_HNSWLIB_ORDER needs fp32 storage");
+        return GVS_ERR_UNSUPPORTED;
+    }
+    const bool l2 = ix->metric == GVS_METRIC_L2;
     kfn_t fn = nullptr;
     if (ix->dim_pad % (8 * E) == 0)
-        fn = W > 1 ? (ix->metric == GVS_METRIC_L2 ? gvs_kernel_mw_f32_l2(nch, wrows) : gvs_kernel_mw_f32_ip(nch, wrows))
-                   : pick_kernel(p->storage, ix->metric, nch, p->mc * rpc);
+        fn = W > 1 ? (seq ? (l2 ? gvs_kernel_mw_f32seq_l2(nch, wrows) : gvs_kernel_mw_f32seq_ip(nch, wrows))
+                          : (l2 ? gvs_kernel_mw_f32_l2(nch, wrows) : gvs_kernel_mw_f32_ip(nch, wrows)))
+                   : pick_kernel(p->storage, ix->metric, nch, p->mc * rpc, seq);
     if (!fn) {
         set_error("gvs_search: dim %d (padded %d) not compiled for storage %d", ix->dim, ix->dim_pad, p->storage);
         return GVS_ERR_UNSUPPORTED;
     }
     int efe = std::max(max_ef, max_k);
     lp->fn = (void *)fn;
-    lp->dfn = W == 1 ? (void *)pick_descend(p->storage, ix->metric, nch) : nullptr;
+    lp->dfn = W == 1 ? (void *)pick_descend(p->storage, ix->metric, nch, seq) : nullptr;
     lp->ef_cap = (efe + 63) / 64 * 64;
     lp->lcap = p->mc * rpc * WAVE;
     lp->blog2 = p->bloom_log2 > 0 ? p->bloom_log2 : std::min(12, std::max(8, ilog2_ceil((int64_t)efe * 8)));
@@ -711,13 +721,17 @@ static int search_impl(gvs_index *ix, const float *queries, int64_t nq, int64_t 
     if (!hip_ok(hipMemcpyAsync(&novf, ix->d_counters + 1, 4, hipMemcpyDeviceToHost, stream), "read overflow count") ||
         !hip_ok(hipStreamSynchronize(stream), "hipStreamSynchronize"))
         return GVS_ERR_HIP;
-    for (int attempt = 0; novf > 0 && attempt < 3; attempt++) {
+    // idx[i] = the query that retry row i re-runs; a retry launch writes its
+    // statuses by retry row, so each attempt keeps the rows that failed again
+    std::vector<int64_t> idx;
+    if (novf > 0) {
         std::vector<int32_t> st((size_t)nq);
         if (!hip_ok(hipMemcpy(st.data(), ix->d_status, sizeof(int32_t) * nq, hipMemcpyDeviceToHost), "read status"))
             return GVS_ERR_HIP;
-        std::vector<int64_t> idx;
         for (int64_t i = 0; i < nq; i++)
             if (st[(size_t)i]) idx.push_back(i);
+    }
+    for (int attempt = 0; !idx.empty() && attempt < 3; attempt++) {
         gvs_params p2 = *p;
         p2.visited_log2 = lp.vlog2 + 2 * (attempt + 1);
         p2.persistent = 0;
@@ -734,14 +748,15 @@ static int search_impl(gvs_index *ix, const float *queries, int64_t nq, int64_t 
             return GVS_ERR_HIP;
         }
         // rerun the failed rows, writing into temporaries, then scatter back
-        int64_t m = (int64_t)idx.size();
+        const int64_t m = (int64_t)idx.size();
         int64_t *t_lab = nullptr;
         float *t_d = nullptr;
         int32_t *t_st = nullptr;
         int32_t *t_ent = nullptr;
+        std::vector<int32_t> st2((size_t)m);
         bool ok = hip_ok(hipMalloc(&t_lab, sizeof(int64_t) * m * p->k), "hipMalloc") &&
                   hip_ok(hipMalloc(&t_d, sizeof(float) * m * p->k), "hipMalloc") &&
-                  hip_ok(hipMalloc(&t_st, sizeof(int32_t) * m * 4), "hipMalloc");
+                  hip_ok(hipMalloc(&t_st, sizeof(int32_t) * m * GVS_STATS_ROW), "hipMalloc");
         if (ok && entries) {
             std::vector<int32_t> he((size_t)m);
             for (int64_t i = 0; i < m; i++)
@@ -760,16 +775,20 @@ static int search_impl(gvs_index *ix, const float *queries, int64_t nq, int64_t 
             P2.out_dists = t_d;
             P2.out_stats = t_st;
             ok = gvs::launch_search_ix(ix, &p2, lp2, P2, m, stream) == GVS_OK &&
-                 hip_ok(hipStreamSynchronize(stream), "sync retry");
+                 hip_ok(hipStreamSynchronize(stream), "sync retry") &&
+                 hip_ok(hipMemcpy(st2.data(), ix->d_status, sizeof(int32_t) * m, hipMemcpyDeviceToHost),
+                        "read retry status");
             for (int64_t i = 0; ok && i < m; i++) {
-                int64_t q = idx[(size_t)i];
+                if (st2[(size_t)i]) continue;  // failed again: its rows are written by a later attempt
+                const int64_t q = idx[(size_t)i];
                 ok = hip_ok(hipMemcpy(labels + q * p->k, t_lab + i * p->k, sizeof(int64_t) * p->k,
                                       hipMemcpyDeviceToDevice), "scatter") &&
                      hip_ok(hipMemcpy(dists + q * p->k, t_d + i * p->k, sizeof(float) * p->k,
                                       hipMemcpyDeviceToDevice), "scatter") &&
-                     (!stats || hip_ok(hipMemcpy(stats + q * 4, t_st + i * 4, 16, hipMemcpyDeviceToDevice), "scatter"));
+                     (!stats || hip_ok(hipMemcpy(stats + q * GVS_STATS_ROW, t_st + i * GVS_STATS_ROW,
+                                                 sizeof(int32_t) * GVS_STATS_ROW, hipMemcpyDeviceToDevice),
+                                       "scatter"));
             }
-            if (ok) ok = hip_ok(hipMemcpy(&novf, ix->d_counters + 1, 4, hipMemcpyDeviceToHost), "read overflow");
         }
         (void)hipFree(d_idx);
         (void)hipFree(t_lab);
@@ -777,7 +796,12 @@ static int search_impl(gvs_index *ix, const float *queries, int64_t nq, int64_t 
         (void)hipFree(t_st);
         (void)hipFree(t_ent);
         if (!ok) return GVS_ERR_HIP;
+        std::vector<int64_t> again;
+        for (int64_t i = 0; i < m; i++)
+            if (st2[(size_t)i]) again.push_back(idx[(size_t)i]);
+        idx.swap(again);
     }
+    novf = (uint32_t)idx.size();
     if (novf > 0) {
         gvs::set_error("gvs_search: %u queries overflowed the visited table", novf);
         return GVS_ERR_OVERFLOW;
@@ -791,6 +815,8 @@ extern "C" int gvs_search_device(gvs_index *ix, const float *queries, int64_t nq
 {
     return search_impl(ix, queries, nq, ldq, entries, p, labels, dists, stats, (hipStream_t)stream);
 }
+
+extern "C" int gvs_stats_per_query(void) { return GVS_STATS_ROW; }
 
 extern "C" int gvs_device_copy(void *dst, const void *src, int64_t bytes, void *stream)
 {
@@ -930,7 +956,7 @@ extern "C" int gvs_autotune(gvs_index *ix, const float *queries, int64_t nq, int
         if (best_t < 0.0f || ms < best_t) {
             best_t = ms;
             *best = c;
-            best->flags = (p->flags & ~GVS_FLAG_REGION_ORDER) | (c.flags & GVS_FLAG_REGION_ORDER);
+            best->flags = (p->flags & ~GVS_FLAG_REGION_ORDER) | (c.flags & (GVS_FLAG_REGION_ORDER | GVS_FLAG_HBM_VISITED));
         }
     }
     (void)hipEventDestroy(e0);
@@ -1050,14 +1076,15 @@ extern "C" int gvs_search_host(gvs_index *ix, const float *queries, int64_t nq, 
     if (ok && entries)
         ok = hip_ok(hipMalloc(&de, sizeof(int32_t) * nq), "hipMalloc entries") &&
              hip_ok(hipMemcpy(de, entries, sizeof(int32_t) * nq, hipMemcpyHostToDevice), "copy entries");
-    if (ok && stats) ok = hip_ok(hipMalloc(&ds, sizeof(int32_t) * nq * 4), "hipMalloc stats");
+    if (ok && stats) ok = hip_ok(hipMalloc(&ds, sizeof(int32_t) * nq * GVS_STATS_ROW), "hipMalloc stats");
     if (ok) {
         rc = search_impl(ix, dq, nq, ldq, de, p, dl, dd, ds, nullptr);
         if (rc == GVS_OK) {
             ok = hip_ok(hipDeviceSynchronize(), "hipDeviceSynchronize") &&
                  hip_ok(hipMemcpy(labels, dl, sizeof(int64_t) * nq * k, hipMemcpyDeviceToHost), "copy labels") &&
                  hip_ok(hipMemcpy(dists, dd, sizeof(float) * nq * k, hipMemcpyDeviceToHost), "copy dists") &&
-                 (!stats || hip_ok(hipMemcpy(stats, ds, sizeof(int32_t) * nq * 4, hipMemcpyDeviceToHost), "copy stats"));
+                 (!stats || hip_ok(hipMemcpy(stats, ds, sizeof(int32_t) * nq * GVS_STATS_ROW, hipMemcpyDeviceToHost),
+                                    "copy stats"));
             if (!ok) rc = GVS_ERR_HIP;
         }
     }

@@ -203,7 +203,7 @@ extern "C" int gvs_server_open(gvs_index *ix, const gvs_server_config *cfg, gvs_
               gvs::hip_ok(hipHostMalloc((void **)&s->cring, 8 * (size_t)rcap, hipHostMallocCoherent), "hipHostMalloc") &&
               gvs::hip_ok(hipHostMalloc((void **)&s->labels, 8 * (size_t)R * cfg->max_k, hipHostMallocCoherent), "hipHostMalloc") &&
               gvs::hip_ok(hipHostMalloc((void **)&s->dists, 4 * (size_t)R * cfg->max_k, hipHostMallocCoherent), "hipHostMalloc") &&
-              gvs::hip_ok(hipHostMalloc((void **)&s->stats, 16 * (size_t)R, hipHostMallocCoherent), "hipHostMalloc") &&
+              gvs::hip_ok(hipHostMalloc((void **)&s->stats, 4 * GVS_STATS_ROW * (size_t)R, hipHostMallocCoherent), "hipHostMalloc") &&
               gvs::hip_ok(hipMalloc(&s->d_head, 8), "hipMalloc") && gvs::hip_ok(hipMalloc(&s->d_served, 8), "hipMalloc") &&
               gvs::hip_ok(hipMemset(s->d_head, 0, 8), "hipMemset") && gvs::hip_ok(hipMemset(s->d_served, 0, 8), "hipMemset") &&
               gvs::hip_ok(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "hipStreamCreateWithFlags");
@@ -333,7 +333,9 @@ static int poll_locked(gvs_server *s, int32_t max_done, int64_t *req_ids, int64_
             if (labels) std::memcpy(labels + (size_t)n * K, s->labels + (size_t)slot * K, 8 * (size_t)K);
             if (dists) std::memcpy(dists + (size_t)n * K, s->dists + (size_t)slot * K, 4 * (size_t)K);
             if (latency_us) latency_us[n] = t - sl.t_arrival;
-            if (stats_out) std::memcpy(stats_out + (size_t)n * 4, s->stats + (size_t)slot * 4, 16);
+            if (stats_out)
+                std::memcpy(stats_out + (size_t)n * GVS_STATS_ROW, s->stats + (size_t)slot * GVS_STATS_ROW,
+                            4 * GVS_STATS_ROW);
             n++;
             s->c_completed++;
         }

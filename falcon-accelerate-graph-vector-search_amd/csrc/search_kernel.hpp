@@ -33,7 +33,21 @@ constexpr int WAVE = 64;
 constexpr uint64_t EMPTY_KEY = ~0ull;
 constexpr int MAX_ROWS = 8;        // largest mc * ceil(r0/64) adjacency rows per step
 constexpr int STORE_F32 = 0, STORE_BF16 = 1, STORE_FP8 = 2;
+// fp32 rows summed in hnswlib's scalar order (one lane per row, res += t*t
+// for i = 0..D-1, HNSWLIB/space_l2.h:6-20, space_ip.h:7-19) instead of the
+// 8-lane order: the search then returns hnswlib's ids and distances bit for
+// bit.  A parity mode, not a storage format (it reads the fp32 copy).
+constexpr int STORE_F32_SEQ = 3;
 constexpr int METRIC_L2 = 0, METRIC_IP = 1;
+
+// int32 per query in a stats row: hops, evaluations, upper hops, upper
+// evaluations; the -DGVS_PROFILE build appends 8 phase timers and 4 spare
+// words.  Every stats buffer (device, host, server) is sized from this.
+#ifdef GVS_PROFILE
+#define GVS_STATS_ROW 16
+#else
+#define GVS_STATS_ROW 4
+#endif
 
 struct KParams {
     const void *vec;           // traversal copy (f32 / bf16 / e4m3)
@@ -196,6 +210,11 @@ struct StoreTraits<STORE_F32> {
     static constexpr int BYTES = 4;
 };
 template <>
+struct StoreTraits<STORE_F32_SEQ> {
+    static constexpr int E = 4;
+    static constexpr int BYTES = 4;
+};
+template <>
 struct StoreTraits<STORE_BF16> {
     static constexpr int E = 8;
     static constexpr int BYTES = 2;
@@ -210,7 +229,7 @@ struct StoreTraits<STORE_FP8> {
 template <int STORE>
 __device__ __forceinline__ void decode_chunk(const uint4 raw, float scale, float *x)
 {
-    if constexpr (STORE == STORE_F32) {
+    if constexpr (STORE == STORE_F32 || STORE == STORE_F32_SEQ) {
         x[0] = __uint_as_float(raw.x);
         x[1] = __uint_as_float(raw.y);
         x[2] = __uint_as_float(raw.z);
@@ -351,6 +370,45 @@ __device__ __forceinline__ void eval_block(const KParams &P, const float *qf, co
     }
 }
 
+// hnswlib's scalar order: lane r sums row lids[b0 + r] element by element,
+// res = res + t*t (two roundings, no fma) for L2, res = res + a*b then
+// 1 - res for IP.  Element j of the query sits in lane (j/4) % 8 of every
+// 8-lane group, register (j/4/8)*4 + j%4, and is broadcast by readlane.
+// Padding elements are zero in row and query and add +0.
+template <int NCH, int METRIC>
+__device__ __forceinline__ void eval_rows_seq(const KParams &P, const float *qf, const uint32_t *lids, int n,
+                                              uint64_t *keys)
+{
+    const int lane = lane_id();
+    for (int b0 = 0; b0 < n; b0 += WAVE) {
+        const int r = min(b0 + lane, n - 1);
+        const uint32_t id = lids[r];
+        const float *row = (const float *)P.vec + (uint64_t)id * (uint32_t)P.row_elems;
+        float res = 0.0f;
+#pragma unroll
+        for (int i = 0; i < NCH; i++)
+#pragma unroll
+            for (int p = 0; p < 8; p++) {
+                const float4 x = *(const float4 *)(row + (p + 8 * i) * 4);
+                const float xv[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+                for (int t = 0; t < 4; t++) {
+                    const float qv = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(qf[i * 4 + t]), p));
+                    if constexpr (METRIC == METRIC_L2) {
+                        const float d = __fsub_rn(xv[t], qv);
+                        res = __fadd_rn(res, __fmul_rn(d, d));
+                    } else {
+                        res = __fadd_rn(res, __fmul_rn(xv[t], qv));
+                    }
+                }
+            }
+        if (b0 + lane < n) {
+            const float d = (METRIC == METRIC_L2) ? res : __fsub_rn(1.0f, res);
+            keys[b0 + lane] = mk_key(d, id);
+        }
+    }
+}
+
 // Distances of rows lids[0..n) -> keys[0..n).  All 64 lanes take part;
 // rows go in blocks of up to 32 (passes of 8 rows, 8 lanes per row).
 template <int NCH, int STORE, int METRIC>
@@ -358,6 +416,10 @@ __device__ __forceinline__ void eval_rows(const KParams &P, const float *qf, con
                                           uint64_t *keys)
 {
     n = uni(n);
+    if constexpr (STORE == STORE_F32_SEQ) {
+        eval_rows_seq<NCH, METRIC>(P, qf, lids, n, keys);
+        return;
+    }
     for (int b0 = 0; b0 < n; b0 += 32) {
         const int nb = n - b0;
         if (nb > 16) eval_block<NCH, STORE, METRIC, 4>(P, qf, lids, b0, n, keys);

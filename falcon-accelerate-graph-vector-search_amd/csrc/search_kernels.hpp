@@ -93,13 +93,8 @@ __global__ __launch_bounds__(256) void k_descend(KParams P, int32_t *ep_out, uin
         key_out[qi] = ((uint64_t)n3 << P.key_bits) | (uint64_t)n2;
         perm_out[qi] = (int32_t)qi;
         if (P.out_stats) {
-#ifdef GVS_PROFILE
-            constexpr int SW = 16;  // the profile build's stats row
-#else
-            constexpr int SW = 4;
-#endif
-            P.out_stats[qi * SW + 2] = up_hops;
-            P.out_stats[qi * SW + 3] = up_evals;
+            P.out_stats[qi * GVS_STATS_ROW + 2] = up_hops;
+            P.out_stats[qi * GVS_STATS_ROW + 3] = up_evals;
         }
     }
 }
@@ -154,3 +149,10 @@ gvsk::dfn_t gvs_descend_fp8_ip(int nch);
 // several waves per query (fp32 storage)
 gvsk::kfn_t gvs_kernel_mw_f32_l2(int nch, int rows_per_wave);
 gvsk::kfn_t gvs_kernel_mw_f32_ip(int nch, int rows_per_wave);
+// fp32 rows in hnswlib's scalar summation order (parity mode)
+gvsk::kfn_t gvs_kernel_f32seq_l2(int nch, int rows);
+gvsk::kfn_t gvs_kernel_f32seq_ip(int nch, int rows);
+gvsk::kfn_t gvs_kernel_mw_f32seq_l2(int nch, int rows_per_wave);
+gvsk::kfn_t gvs_kernel_mw_f32seq_ip(int nch, int rows_per_wave);
+gvsk::dfn_t gvs_descend_f32seq_l2(int nch);
+gvsk::dfn_t gvs_descend_f32seq_ip(int nch);

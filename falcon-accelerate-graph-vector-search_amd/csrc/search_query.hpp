@@ -709,7 +709,7 @@ __device__ void search_query_req(const KParams &P, int64_t qi, int64_t slot, uns
     // Low-precision traversal: rerank the final pool with exact fp32
     // distances (fp32 copy, the kernels' fp32 summation order), then sort by
     // (dist, id).
-    if constexpr (STORE != STORE_F32) {
+    if constexpr (STORE == STORE_BF16 || STORE == STORE_FP8) {
         if (P.rerank && !overflow) {
             constexpr int NCH32 = NCH * E / 4;
             float q32[NCH32 * 4];
@@ -761,26 +761,18 @@ __device__ void search_query_req(const KParams &P, int64_t qi, int64_t slot, uns
     }
     PROF_MARK(7);
     if (lane == 0) {
+        if (P.out_stats) {
+            int32_t *row = P.out_stats + qi * GVS_STATS_ROW;
+            row[0] = hops;
+            row[1] = evals;
+            if (!P.keep_upper_stats) {
+                row[2] = up_hops;
+                row[3] = up_evals;
+            }
 #ifdef GVS_PROFILE
-        if (P.out_stats) {
-            P.out_stats[qi * 16 + 0] = hops;
-            P.out_stats[qi * 16 + 1] = evals;
-            if (!P.keep_upper_stats) {
-                P.out_stats[qi * 16 + 2] = up_hops;
-                P.out_stats[qi * 16 + 3] = up_evals;
-            }
-            for (int ph = 0; ph < 8; ph++) P.out_stats[qi * 16 + 4 + ph] = (int32_t)(prof_acc[ph] >> 6);
-        }
-#else
-        if (P.out_stats) {
-            P.out_stats[qi * 4 + 0] = hops;
-            P.out_stats[qi * 4 + 1] = evals;
-            if (!P.keep_upper_stats) {
-                P.out_stats[qi * 4 + 2] = up_hops;
-                P.out_stats[qi * 4 + 3] = up_evals;
-            }
-        }
+            for (int ph = 0; ph < 8; ph++) row[4 + ph] = (int32_t)(prof_acc[ph] >> 6);
 #endif
+        }
         if (P.status) P.status[qi] = overflow ? 1 : 0;
         if (overflow) atomicAdd(&P.counters[1], 1u);
     }
@@ -992,10 +984,10 @@ __device__ void search_query_mw(const KParams &P, int64_t qi, int64_t slot, unsi
         }
         if (lane == 0) {
             if (P.out_stats) {
-                P.out_stats[qi * 4 + 0] = hops;
-                P.out_stats[qi * 4 + 1] = (int)G.ctl[WG_EVALS];
-                P.out_stats[qi * 4 + 2] = up_hops;
-                P.out_stats[qi * 4 + 3] = up_evals;
+                P.out_stats[qi * GVS_STATS_ROW + 0] = hops;
+                P.out_stats[qi * GVS_STATS_ROW + 1] = (int)G.ctl[WG_EVALS];
+                P.out_stats[qi * GVS_STATS_ROW + 2] = up_hops;
+                P.out_stats[qi * GVS_STATS_ROW + 3] = up_evals;
             }
             if (P.status) P.status[qi] = overflow ? 1 : 0;
             if (overflow) atomicAdd(&P.counters[1], 1u);

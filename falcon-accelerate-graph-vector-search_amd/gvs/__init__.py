@@ -21,6 +21,7 @@ FLAG_ASYNC = 1
 FLAG_HBM_VISITED = 2
 FLAG_LDS_VISITED = 4
 FLAG_REGION_ORDER = 8
+FLAG_HNSWLIB_ORDER = 16
 ERRORS = {-1: "ARG", -2: "HIP", -3: "NOMEM", -4: "UNSUPPORTED", -5: "OVERFLOW", -6: "IO", -7: "STATE"}
 L2, IP = 0, 1
 F32, BF16, FP8 = 0, 1, 2
@@ -85,6 +86,7 @@ def _load():
         "gvs_search_graph_launch": (C.c_int, [P, P]),
         "gvs_search_graph_destroy": (C.c_int, [P]),
         "gvs_device_copy": (C.c_int, [P, P, I64, P]),
+        "gvs_stats_per_query": (C.c_int, []),
         "gvs_bruteforce_f64_device": (C.c_int, [P, P, I64, I64, I32, P, P, P]),
         "gvs_server_open": (C.c_int, [P, P, P]),
         "gvs_server_submit": (I64, [P, P, I32, I32]),
@@ -105,6 +107,8 @@ def _load():
 
 
 lib = _load()
+# stats row width of this build (4 counters; more in a phase-profiling build)
+STATS_ROW = lib.gvs_stats_per_query()
 
 
 def exported_symbols():
@@ -113,7 +117,7 @@ def exported_symbols():
              "gvs_hnswlib_save", "gvs_nsg_load", "gvs_fpga_load", "gvs_fpga_save", "gvs_host_graph_view", "gvs_host_graph_free",
              "gvs_gen_mixture", "gvs_index_create", "gvs_index_destroy", "gvs_index_device_bytes",
              "gvs_search_device", "gvs_search_host", "gvs_search_overflow_count", "gvs_search_counters", "gvs_autotune", "gvs_search_graph_create",
-             "gvs_search_graph_launch", "gvs_search_graph_destroy", "gvs_device_copy", "gvs_bruteforce_f64_device", "gvs_server_open",
+             "gvs_search_graph_launch", "gvs_search_graph_destroy", "gvs_device_copy", "gvs_stats_per_query", "gvs_bruteforce_f64_device", "gvs_server_open",
              "gvs_server_submit", "gvs_server_poll", "gvs_server_poll_stats", "gvs_server_cancel",
              "gvs_server_stats", "gvs_server_close", "gvs_server_loadgen"]
     return [n for n in names if hasattr(lib, n)]
@@ -222,14 +226,18 @@ class HostGraph:
 
 
 def make_params(k=10, ef=64, mc=1, mg=1, waves_per_query=1, storage=F32, rerank=False, persistent=False,
-                visited_log2=0, bloom_log2=0, grid_waves=0, asynchronous=False, visited=None, region_order=False):
+                visited_log2=0, bloom_log2=0, grid_waves=0, asynchronous=False, visited=None, region_order=False,
+                hnswlib_order=False):
     """visited: None (auto), "hbm" (LDS Bloom + HBM hash) or "lds" (exact LDS
     hash; 2-byte slots when max(ef, k) > 128 and the graph's ids fit them;
     visited_log2 in [10, 14] then sets its size).
-    region_order: run the batch in region order (persistent mode; GVS_FLAG_REGION_ORDER)."""
+    region_order: run the batch in region order (persistent mode; GVS_FLAG_REGION_ORDER).
+    hnswlib_order: sum distances in hnswlib's scalar order (GVS_FLAG_HNSWLIB_ORDER)."""
     flags = FLAG_ASYNC if asynchronous else 0
     if region_order:
         flags |= FLAG_REGION_ORDER
+    if hnswlib_order:
+        flags |= FLAG_HNSWLIB_ORDER
     if visited == "hbm":
         flags |= FLAG_HBM_VISITED
     elif visited == "lds":
@@ -263,12 +271,12 @@ class DeviceIndex:
         nq = q.shape[0]
         lab = np.empty((nq, k), dtype=np.int64)
         dist = np.empty((nq, k), dtype=np.float32)
-        st = np.empty((nq, STATS_PER_QUERY), dtype=np.int32) if with_stats else None
+        st = np.empty((nq, STATS_ROW), dtype=np.int32) if with_stats else None
         ent = None if entries is None else np.ascontiguousarray(entries, dtype=np.int32)
         p = make_params(k=k, ef=ef, **kw)
         _check(lib.gvs_search_host(self._h, _ptr(q), nq, q.shape[1], _ptr(ent), C.byref(p), _ptr(lab),
                                    _ptr(dist), _ptr(st)))
-        return (lab, dist, st) if with_stats else (lab, dist)
+        return (lab, dist, st[:, :STATS_PER_QUERY]) if with_stats else (lab, dist)
 
     def search_device(self, q_ptr, nq, ldq, labels_ptr, dists_ptr, k=10, ef=64, entries_ptr=None,
                       stats_ptr=None, stream=None, **kw):
@@ -388,7 +396,7 @@ class Server:
         dist = np.empty((max_done, self.max_k), dtype=np.float32)
         lat = np.empty(max_done, dtype=np.float64)
         if with_stats:
-            st = np.empty((max_done, STATS_PER_QUERY), dtype=np.int32)
+            st = np.empty((max_done, STATS_ROW), dtype=np.int32)
             n = lib.gvs_server_poll_stats(self._h, max_done, _ptr(rid), _ptr(lab), _ptr(dist), _ptr(lat), _ptr(st))
         else:
             st = None
@@ -396,7 +404,7 @@ class Server:
         if n < 0:
             raise GvsError(n, lib.gvs_last_error().decode())
         out = (rid[:n], lab[:n], dist[:n], lat[:n])
-        return out + (st[:n],) if with_stats else out
+        return out + (st[:n, :STATS_PER_QUERY],) if with_stats else out
 
     def cancel(self, req):
         return lib.gvs_server_cancel(self._h, req) == GVS_OK

@@ -104,12 +104,24 @@ typedef struct gvs_params {
  * counters are those of the unordered search.  Ignored by captured
  * searches (gvs_search_graph_create). */
 #define GVS_FLAG_REGION_ORDER 8
+/* hnswlib order (fp32 storage): every distance is summed as hnswlib's scalar
+ * L2Sqr / InnerProductDistance does, res += t*t for i = 0..D-1 with no fused
+ * multiply-add (HNSWLIB/space_l2.h:6-20, HNSWLIB/space_ip.h:7-19), one lane
+ * per row, so ids, distances and counters equal hnswlib's searchKnn bit for
+ * bit.  A parity mode: the default 8-lane order is faster and differs from
+ * it only in the last bits of each distance. */
+#define GVS_FLAG_HNSWLIB_ORDER 16
 
 /* Per-query debug counters written when stats != NULL (4 int32 per query):
  * layer-0 hops, layer-0 distance evaluations, upper-layer list scans,
  * upper-layer distance evaluations.  The reference's mem_debug carries
  * {hops, visited} per query (INTER/vadd.cpp:88-94, INTER/scheduler.hpp:98-166). */
 #define GVS_STATS_PER_QUERY 4
+/* int32 per query in every stats buffer this library reads or writes:
+ * GVS_STATS_PER_QUERY, or 16 in a build with -DGVS_PROFILE (the counters,
+ * then per-phase timers).  Size stats buffers from this.  No reference
+ * counterpart (mem_debug has a fixed debug_size, INTER/host.cpp:851-995). */
+GVS_API int gvs_stats_per_query(void);
 
 typedef struct gvs_index gvs_index;
 typedef struct gvs_host_graph gvs_host_graph;

@@ -303,10 +303,13 @@ static int drain_sorted(heap_t *top, int32_t *out_ids, float *out_dists)
  * complete in launch order and the pipeline is refilled after each
  * completion (FPGA_multi_DDR/FPGA_inter_query_v1.3_longer_FIFO_alt_PR/src/scheduler.hpp:108-154).
  * mc = mg = 1 is exactly searchBaseLayerST. */
-int orc_search_layer0(const orc_graph *g, const float *q, int32_t ep, int ef,
-                      int mc, int mg, int order,
-                      int32_t *out_ids, float *out_dists, orc_stats *st)
+static int layer0_impl(const orc_graph *g, const float *q, int32_t ep, int ef,
+                       int mc, int mg, int order,
+                       int32_t *out_ids, float *out_dists, orc_stats *st,
+                       int32_t *trace_ids, int32_t *trace_group, int trace_cap, int *ntrace)
 {
+    int64_t ngroups = 0;
+    int nt = 0;
     if (ef < 1) ef = 1;
     if (mc < 1) mc = 1;
     if (mg < 1) mg = 1;
@@ -340,6 +343,11 @@ int orc_search_layer0(const orc_graph *g, const float *q, int32_t ep, int ef,
                  * HNSWLIB/hnswalg.h:267 */
                 if (key_lt(top.v[0], c)) break;
                 heap_pop(&cand);
+                if (nt < trace_cap && trace_ids) {
+                    trace_ids[nt] = c.id;
+                    if (trace_group) trace_group[nt] = (int32_t)ngroups;
+                }
+                nt++;
                 taken++;
                 hops++;
                 const int32_t *nb = g->adj0 + (int64_t)c.id * g->r0;
@@ -354,6 +362,7 @@ int orc_search_layer0(const orc_graph *g, const float *q, int32_t ep, int ef,
                 }
             }
             if (taken == 0) break;
+            ngroups++;
             inflight++;
         }
         if (inflight == 0) break;
@@ -378,22 +387,33 @@ int orc_search_layer0(const orc_graph *g, const float *q, int32_t ep, int ef,
     if (st) {
         st->hops += hops;
         st->evals += evals;
+        st->groups += ngroups;
     }
+    if (ntrace) *ntrace = nt;
     return n;
+}
+
+int orc_search_layer0(const orc_graph *g, const float *q, int32_t ep, int ef,
+                      int mc, int mg, int order,
+                      int32_t *out_ids, float *out_dists, orc_stats *st)
+{
+    return layer0_impl(g, q, ep, ef, mc, mg, order, out_ids, out_dists, st, NULL, NULL, 0, NULL);
 }
 
 /* HNSWLIB/hnswalg.h:1114-1168 searchKnn, in closer-first order
  * (HNSWLIB/hnswlib.h:171-185). */
-int orc_search_knn(const orc_graph *g, const float *q, int k, int ef,
-                   int mc, int mg, int order, const int32_t *entry_or_null,
-                   int32_t *out_ids, float *out_dists, orc_stats *st)
+int orc_search_knn_trace(const orc_graph *g, const float *q, int k, int ef,
+                         int mc, int mg, int order, const int32_t *entry_or_null,
+                         int32_t *out_ids, float *out_dists, orc_stats *st,
+                         int32_t *trace_ids, int32_t *trace_group, int trace_cap, int *ntrace)
 {
+    if (ntrace) *ntrace = 0;
     if (g->n <= 0 || k <= 0) return 0;
     int32_t ep = entry_or_null ? *entry_or_null : orc_descend(g, q, order, NULL, st);
     int efe = ef > k ? ef : k;
     int32_t *ids = (int32_t *)malloc(sizeof(int32_t) * (size_t)efe);
     float *ds = (float *)malloc(sizeof(float) * (size_t)efe);
-    int n = orc_search_layer0(g, q, ep, efe, mc, mg, order, ids, ds, st);
+    int n = layer0_impl(g, q, ep, efe, mc, mg, order, ids, ds, st, trace_ids, trace_group, trace_cap, ntrace);
     int m = n < k ? n : k;
     memcpy(out_ids, ids, sizeof(int32_t) * (size_t)m);
     memcpy(out_dists, ds, sizeof(float) * (size_t)m);
@@ -403,27 +423,47 @@ int orc_search_knn(const orc_graph *g, const float *q, int k, int ef,
     return m;
 }
 
+int orc_search_knn(const orc_graph *g, const float *q, int k, int ef,
+                   int mc, int mg, int order, const int32_t *entry_or_null,
+                   int32_t *out_ids, float *out_dists, orc_stats *st)
+{
+    return orc_search_knn_trace(g, q, k, ef, mc, mg, order, entry_or_null, out_ids, out_dists, st,
+                                NULL, NULL, 0, NULL);
+}
+
+int orc_search_batch_ex(const orc_graph *g, const float *q, int64_t nq, int k, int ef,
+                        int mc, int mg, int order, const int32_t *entries_or_null,
+                        int32_t *out_ids, float *out_dists, int64_t *out_stats, int stats_width,
+                        int nthreads)
+{
+    if (nthreads < 1) nthreads = 1;
+    if (stats_width < 4) stats_width = 4;
+    int64_t i;
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 16)
+    for (i = 0; i < nq; i++) {
+        orc_stats st = {0, 0, 0, 0, 0};
+        orc_search_knn(g, q + i * (int64_t)g->dim_pad, k, ef, mc, mg, order,
+                       entries_or_null ? entries_or_null + i : NULL,
+                       out_ids + i * (int64_t)k, out_dists + i * (int64_t)k, &st);
+        if (out_stats) {
+            int64_t *row = out_stats + (int64_t)stats_width * i;
+            row[0] = st.hops;
+            row[1] = st.evals;
+            row[2] = st.up_hops;
+            row[3] = st.up_evals;
+            if (stats_width > 4) row[4] = st.groups;
+        }
+    }
+    return 0;
+}
+
 int orc_search_batch(const orc_graph *g, const float *q, int64_t nq, int k, int ef,
                      int mc, int mg, int order, const int32_t *entries_or_null,
                      int32_t *out_ids, float *out_dists, int64_t *out_stats4,
                      int nthreads)
 {
-    if (nthreads < 1) nthreads = 1;
-    int64_t i;
-#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 16)
-    for (i = 0; i < nq; i++) {
-        orc_stats st = {0, 0, 0, 0};
-        orc_search_knn(g, q + i * (int64_t)g->dim_pad, k, ef, mc, mg, order,
-                       entries_or_null ? entries_or_null + i : NULL,
-                       out_ids + i * (int64_t)k, out_dists + i * (int64_t)k, &st);
-        if (out_stats4) {
-            out_stats4[4 * i + 0] = st.hops;
-            out_stats4[4 * i + 1] = st.evals;
-            out_stats4[4 * i + 2] = st.up_hops;
-            out_stats4[4 * i + 3] = st.up_evals;
-        }
-    }
-    return 0;
+    return orc_search_batch_ex(g, q, nq, k, ef, mc, mg, order, entries_or_null, out_ids, out_dists,
+                               out_stats4, 4, nthreads);
 }
 
 /* ------------------------------------------------------------------ */

@@ -80,6 +80,7 @@ typedef struct {
     int64_t evals;       /* layer-0 distance evaluations (entry point included) */
     int64_t up_hops;     /* upper-layer neighbour-list scans */
     int64_t up_evals;    /* upper-layer distance evaluations (entry distance included) */
+    int64_t groups;      /* layer-0 DST groups launched (== hops when mc == 1) */
 } orc_stats;
 
 /* distance between a query (fp32, dim_pad elements, zero padded) and node id */
@@ -121,6 +122,22 @@ int orc_search_batch(const orc_graph *g, const float *q, int64_t nq, int k, int 
                      int mc, int mg, int order, const int32_t *entries_or_null,
                      int32_t *out_ids, float *out_dists, int64_t *out_stats4,
                      int nthreads);
+
+/* searchKnn of one query that also records the layer-0 expansion order:
+ * trace_ids[i] = the i-th node expanded (popped from the candidate set),
+ * trace_group[i] = the DST group (0, 1, ...) that expanded it; at most
+ * trace_cap entries are written, *ntrace = the number of expansions. */
+int orc_search_knn_trace(const orc_graph *g, const float *q, int k, int ef,
+                         int mc, int mg, int order, const int32_t *entry_or_null,
+                         int32_t *out_ids, float *out_dists, orc_stats *st,
+                         int32_t *trace_ids, int32_t *trace_group, int trace_cap, int *ntrace);
+
+/* orc_search_batch with stats_width int64 per query: 4 as above, or 5 with
+ * the DST group count appended */
+int orc_search_batch_ex(const orc_graph *g, const float *q, int64_t nq, int k, int ef,
+                        int mc, int mg, int order, const int32_t *entries_or_null,
+                        int32_t *out_ids, float *out_dists, int64_t *out_stats, int stats_width,
+                        int nthreads);
 
 /* FP64 exact kNN: out ids/dists (ascending, ties by id) for nq queries */
 void orc_bruteforce_f64(const float *base, int64_t n, int dim, int metric,

@@ -337,7 +337,7 @@ def test_captured_graph_replays_match_eager(persistent):
     dq = H.DeviceArray.from_host(batches[0])
     dl = H.DeviceArray((nq, k), np.int64)
     dd = H.DeviceArray((nq, k), np.float32)
-    ds = H.DeviceArray((nq, 4), np.int32)
+    ds = H.DeviceArray((nq, gvs.STATS_ROW), np.int32)
     st = H.Stream()
     sg = dev.capture(dq.address, nq, 128, dl.address, dd.address, k=k, ef=ef, mc=2, stats_ptr=ds.address,
                      persistent=persistent)
@@ -350,7 +350,7 @@ def test_captured_graph_replays_match_eager(persistent):
         lab, dist, cnt = dev.search(b, k=k, ef=ef, mc=2, with_stats=True)
         np.testing.assert_array_equal(dl.to_host(), lab)
         np.testing.assert_array_equal(dd.to_host(), dist)
-        np.testing.assert_array_equal(ds.to_host(), cnt)
+        np.testing.assert_array_equal(ds.to_host()[:, :gvs.STATS_PER_QUERY], cnt)
         if rep == 2:
             dev.search(queries(6000, 128, seed=40), k=10, ef=200)
     sg.close()
