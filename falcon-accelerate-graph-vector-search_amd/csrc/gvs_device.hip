@@ -24,6 +24,14 @@ __global__ void k_reset_counters(uint32_t *c)
     if (threadIdx.x < 12) c[threadIdx.x] = 0u;
 }
 
+// the Bloom filter's hash of n keys (known-answer checks)
+__global__ __launch_bounds__(256) void k_murmur2_k4(const uint32_t *__restrict__ keys, const uint32_t *__restrict__ seeds,
+                                                    uint32_t *__restrict__ out, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = murmur2_k4(keys[i], seeds[i]);
+}
+
 // plain streaming copy (16 bytes per lane per iteration)
 __global__ __launch_bounds__(256) void k_copy16(const uint4 *__restrict__ src, uint4 *__restrict__ dst, int64_t n16)
 {
@@ -817,6 +825,19 @@ extern "C" int gvs_search_device(gvs_index *ix, const float *queries, int64_t nq
 }
 
 extern "C" int gvs_stats_per_query(void) { return GVS_STATS_ROW; }
+
+extern "C" int gvs_hash_murmur2_keylen4_device(const uint32_t *keys, const uint32_t *seeds, uint32_t *out,
+                                               int64_t n, void *stream)
+{
+    if (n < 0 || (n > 0 && (!keys || !seeds || !out)) || n > ((int64_t)1 << 40)) {
+        gvs::set_error("gvs_hash_murmur2_keylen4_device: invalid argument");
+        return GVS_ERR_ARG;
+    }
+    if (n == 0) return GVS_OK;
+    hipLaunchKernelGGL(k_murmur2_k4, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, keys,
+                       seeds, out, n);
+    return hip_ok(hipGetLastError(), "launch k_murmur2_k4") ? GVS_OK : GVS_ERR_HIP;
+}
 
 extern "C" int gvs_device_copy(void *dst, const void *src, int64_t bytes, void *stream)
 {

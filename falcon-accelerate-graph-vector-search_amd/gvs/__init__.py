@@ -87,6 +87,7 @@ def _load():
         "gvs_search_graph_destroy": (C.c_int, [P]),
         "gvs_device_copy": (C.c_int, [P, P, I64, P]),
         "gvs_stats_per_query": (C.c_int, []),
+        "gvs_hash_murmur2_keylen4_device": (C.c_int, [P, P, P, I64, P]),
         "gvs_bruteforce_f64_device": (C.c_int, [P, P, I64, I64, I32, P, P, P]),
         "gvs_server_open": (C.c_int, [P, P, P]),
         "gvs_server_submit": (I64, [P, P, I32, I32]),
@@ -117,7 +118,7 @@ def exported_symbols():
              "gvs_hnswlib_save", "gvs_nsg_load", "gvs_fpga_load", "gvs_fpga_save", "gvs_host_graph_view", "gvs_host_graph_free",
              "gvs_gen_mixture", "gvs_index_create", "gvs_index_destroy", "gvs_index_device_bytes",
              "gvs_search_device", "gvs_search_host", "gvs_search_overflow_count", "gvs_search_counters", "gvs_autotune", "gvs_search_graph_create",
-             "gvs_search_graph_launch", "gvs_search_graph_destroy", "gvs_device_copy", "gvs_stats_per_query", "gvs_bruteforce_f64_device", "gvs_server_open",
+             "gvs_search_graph_launch", "gvs_search_graph_destroy", "gvs_device_copy", "gvs_stats_per_query", "gvs_hash_murmur2_keylen4_device", "gvs_bruteforce_f64_device", "gvs_server_open",
              "gvs_server_submit", "gvs_server_poll", "gvs_server_poll_stats", "gvs_server_cancel",
              "gvs_server_stats", "gvs_server_close", "gvs_server_loadgen"]
     return [n for n in names if hasattr(lib, n)]
@@ -135,6 +136,12 @@ def _ptr(a):
 def device_copy(dst_ptr, src_ptr, nbytes, stream=None):
     """Device-to-device copy through the library's streaming kernel."""
     _check(lib.gvs_device_copy(C.c_void_p(dst_ptr), C.c_void_p(src_ptr), nbytes, C.c_void_p(stream) if stream else None))
+
+
+def murmur2_keylen4_device(keys_ptr, seeds_ptr, out_ptr, n, stream=None):
+    """The device Bloom hash of n device words (MurmurHash2_KeyLen4)."""
+    _check(lib.gvs_hash_murmur2_keylen4_device(C.c_void_p(keys_ptr), C.c_void_p(seeds_ptr), C.c_void_p(out_ptr), n,
+                                               C.c_void_p(stream) if stream else None))
 
 
 def gen_mixture(n, dim, ncenters, seed_centers, seed_points, center_std=4.0, noise_std=1.0,

@@ -212,6 +212,13 @@ GVS_API int gvs_search_counters(gvs_index *ix, int64_t counters[3]);
  * pointers and size).  No reference counterpart; used to calibrate HBM byte
  * counters against a kernel of known traffic. */
 GVS_API int gvs_device_copy(void *dst, const void *src, int64_t bytes, void *stream);
+/* MurmurHash2_KeyLen4(keys[i], seeds[i]) -> out[i] for n DEVICE words, with
+ * the device function the LDS Bloom filter hashes ids with; the reference's
+ * Bloom hash (INTER/bloom_filter.hpp:43-66,
+ * unit_tests/bloom_and_hash/sw_murmur2/murmur2.cpp:61-90).  For checking the
+ * device hash against known answers. */
+GVS_API int gvs_hash_murmur2_keylen4_device(const uint32_t *keys, const uint32_t *seeds, uint32_t *out,
+                                            int64_t n, void *stream);
 /* Autotuner: times the search of nq DEVICE queries (k, ef, storage, rerank
  * and flags from *p; labels/dists are device result buffers it may overwrite)
  * over mc in {1,2,4,8} (as r0 allows) x mg in {1,2} x persistent in {0,1}

@@ -64,6 +64,10 @@ def _load():
     lib.orc_f32_to_bf16.argtypes = [C.c_float]
     lib.orc_f32_to_e4m3.restype = C.c_uint8
     lib.orc_f32_to_e4m3.argtypes = [C.c_float]
+    lib.orc_search_batch_ex.restype = I
+    lib.orc_search_batch_ex.argtypes = [P, P, I64, I, I, I, I, I, P, P, P, P, I, I]
+    lib.orc_search_knn_trace.restype = I
+    lib.orc_search_knn_trace.argtypes = [P, P, I, I, I, I, I, P, P, P, P, P, P, I, P]
     lib.orc_rerank_f32.restype = I
     lib.orc_rerank_f32.argtypes = [P, I, I, P, P, I, I, I, P, P]
     return lib
@@ -115,16 +119,38 @@ class OracleGraph:
         g.adj_up = _p(self.adj_up) if a["max_level"] > 0 else None
         self.g = g
 
-    def search(self, queries, k, ef, mc=1, mg=1, order=WAVE, entries=None, nthreads=8):
+    def search(self, queries, k, ef, mc=1, mg=1, order=WAVE, entries=None, nthreads=8, with_groups=False):
+        """-> ids, dists, stats (hops, evals, upper hops, upper evals[, DST groups])."""
         q = pad_rows(queries, self.dim_pad)
         nq = q.shape[0]
         ids = np.empty((nq, k), dtype=np.int32)
         ds = np.empty((nq, k), dtype=np.float32)
-        st = np.empty((nq, 4), dtype=np.int64)
+        w = 5 if with_groups else 4
+        st = np.empty((nq, w), dtype=np.int64)
         ent = None if entries is None else np.ascontiguousarray(entries, dtype=np.int32)
-        lib.orc_search_batch(C.byref(self.g), _p(q), nq, k, ef, mc, mg, order, _p(ent), _p(ids), _p(ds),
-                             _p(st), nthreads)
+        lib.orc_search_batch_ex(C.byref(self.g), _p(q), nq, k, ef, mc, mg, order, _p(ent), _p(ids), _p(ds),
+                                _p(st), w, nthreads)
         return ids, ds, st
+
+    def trace(self, query, k, ef, mc=1, mg=1, order=WAVE, entry=None, cap=4096):
+        """One query: ids, dists, stats (5 columns) and the layer-0 expansion
+        order as a list of DST groups, each the list of node ids it expanded."""
+        q = pad_rows(np.asarray(query, dtype=np.float32).reshape(1, -1), self.dim_pad)
+        ids = np.empty(k, dtype=np.int32)
+        ds = np.empty(k, dtype=np.float32)
+        st = np.zeros(5, dtype=np.int64)
+        tid = np.empty(cap, dtype=np.int32)
+        tgr = np.empty(cap, dtype=np.int32)
+        nt = C.c_int(0)
+        ent = None if entry is None else np.array([entry], dtype=np.int32)
+        lib.orc_search_knn_trace(C.byref(self.g), _p(q), k, ef, mc, mg, order, _p(ent), _p(ids), _p(ds), _p(st),
+                                 _p(tid), _p(tgr), cap, C.byref(nt))
+        groups = []
+        for i in range(min(nt.value, cap)):
+            if tgr[i] >= len(groups):
+                groups.append([])
+            groups[tgr[i]].append(int(tid[i]))
+        return ids, ds, st, groups
 
 
 def bruteforce_f64(base, queries, k, metric=L2, nthreads=8):

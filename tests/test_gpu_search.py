@@ -443,3 +443,55 @@ def test_c2_shape_d96_degree64_matches_oracle(ef):
     print(f"case c2 shape ef={ef}", flush=True)
     check_equal(dev, og, q, 10, ef)
     check_equal(dev, og, q, 10, ef, mc=2, persistent=True, region_order=True)
+
+
+class ArrayGraph:
+    """A gvs_graph view over numpy arrays (flat graph), kept alive here."""
+
+    def __init__(self, a):
+        import ctypes as C
+        self.a = {kk: (np.ascontiguousarray(v) if isinstance(v, np.ndarray) else v) for kk, v in a.items()}
+        v = gvs.Graph()
+        v.n, v.dim, v.metric = a["n"], a["dim"], a["metric"]
+        v.vectors = self.a["vectors"].ctypes.data
+        v.ld_vectors = a["dim"]
+        v.r0 = a["r0"]
+        v.adj0 = self.a["adj0"].ctypes.data
+        v.max_level, v.entry, v.ru, v.up_rows = 0, a["entry"], 0, 0
+        self.view = v
+
+
+@pytest.mark.parametrize("mc,mg", [(1, 1), (2, 1), (1, 2), (2, 2)])
+def test_dst_hand_traced_fixture_on_gpu(mc, mg):
+    """The hand-traced DST fixture (tests/test_oracle.py): the kernel gives
+    the derived results, hops and evaluations for each (mc, mg), one wave
+    per query and (mg = 1) two waves per query."""
+    import test_oracle as T
+    dev = gvs.DeviceIndex(ArrayGraph(T.dst_fixture_arrays()))
+    q = np.zeros((1, 1), dtype=np.float32)
+    want_ids, want_groups, want_evals = T.DST_EXPECT[(mc, mg)]
+    x = np.float32(T.DST_X)
+    runs = [dict()] + ([dict(waves_per_query=2)] if mg == 1 else [])
+    for kw in runs:
+        print(f"case hand-traced DST mc={mc} mg={mg} {kw}", flush=True)
+        lab, dist, st = dev.search(q, k=2, ef=2, mc=mc, mg=mg, with_stats=True, **kw)
+        assert lab[0].tolist() == want_ids
+        np.testing.assert_array_equal(dist[0], x[want_ids] * x[want_ids])
+        assert st[0, 0] == sum(len(gr) for gr in want_groups)
+        assert st[0, 1] == want_evals
+
+
+def test_device_bloom_hash_matches_reference_known_answers():
+    """The kernel's MurmurHash2_KeyLen4 (the LDS Bloom filter's hash) equals
+    the reference's on the committed known-answer vectors."""
+    import os
+    from gvs import hip as H
+    z = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "murmur2_kat.npz"))
+    n = z["keys"].size
+    dk = H.DeviceArray.from_host(np.ascontiguousarray(z["keys"], dtype=np.uint32))
+    ds = H.DeviceArray.from_host(np.ascontiguousarray(z["seeds"], dtype=np.uint32))
+    do = H.DeviceArray((n,), np.uint32)
+    print("case device murmur2 known answers", flush=True)
+    gvs.murmur2_keylen4_device(dk.address, ds.address, do.address, n)
+    H.synchronize()
+    np.testing.assert_array_equal(do.to_host(), z["h_keylen4"])

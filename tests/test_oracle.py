@@ -168,14 +168,129 @@ def test_dst_mc1_mg1_equals_plain_and_invariants():
         assert np.all(np.diff(d, axis=1) >= 0)
 
 
-def test_murmur2_keylen4_matches_generic():
-    rng = np.random.default_rng(2)
-    for key, seed in zip(rng.integers(0, 2**32, 500, dtype=np.uint64), rng.integers(0, 2**32, 500, dtype=np.uint64)):
+def test_murmur2_matches_reference_known_answers():
+    """orc_murmur2 / orc_murmur2_keylen4 equal the reference's MurmurHash2 and
+    MurmurHash2_KeyLen4 (unit_tests/bloom_and_hash/sw_murmur2/murmur2.cpp:8-90,
+    compiled from the reference tree into oracle/_ref/ref_murmur2, which
+    wrote tests/golden/murmur2_kat.npz) on 1,000 (key, seed) pairs and on
+    byte strings of every tail length 0..11."""
+    z = np.load(os.path.join(GOLD, "murmur2_kat.npz"))
+    for key, seed, hg, h4 in zip(z["keys"], z["seeds"], z["h_generic4"], z["h_keylen4"]):
         b = np.array([key], dtype=np.uint32)
-        assert orc.lib.orc_murmur2(orc._p(b), 4, int(seed)) == orc.lib.orc_murmur2_keylen4(int(key), int(seed))
-    # empty input with seed 0 hashes to 0 (h = seed ^ len = 0 through the final mix)
-    e = np.zeros(1, dtype=np.uint8)
-    assert orc.lib.orc_murmur2(orc._p(e), 0, 0) == 0
+        assert orc.lib.orc_murmur2(orc._p(b), 4, int(seed)) == int(hg)
+        assert orc.lib.orc_murmur2_keylen4(int(key), int(seed)) == int(h4)
+    assert np.array_equal(z["h_generic4"], z["h_keylen4"])  # the reference's own invariant
+    for n, blob, seed, h in zip(z["lens"], z["blobs"], z["blob_seeds"], z["h_blobs"]):
+        b = np.ascontiguousarray(blob[:max(int(n), 1)])
+        assert orc.lib.orc_murmur2(orc._p(b), int(n), int(seed)) == int(h)
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(os.path.dirname(orc.REF_BIN), "ref_murmur2")),
+                    reason="oracle/_ref not built (no reference tree)")
+def test_murmur2_known_answers_match_live_reference():
+    """The committed vectors are what the reference source computes now."""
+    z = np.load(os.path.join(GOLD, "murmur2_kat.npz"))
+    lines = [f"4 {int(s)} {np.array([k], dtype='<u4').tobytes().hex()}" for k, s in zip(z["keys"][:50], z["seeds"][:50])]
+    out = subprocess.run([os.path.join(os.path.dirname(orc.REF_BIN), "ref_murmur2")], input="\n".join(lines) + "\n",
+                         capture_output=True, text=True, check=True).stdout.split()
+    assert [int(v) for v in out[0::2]] == z["h_generic4"][:50].tolist()
+
+
+# A hand-traced DST fixture.  Eight nodes on a line, query at x = 0, L2
+# (d = x^2), flat graph, entry E, ef = k = 2:
+#   id: 0 E x=10 (d=100)  1 A x=6 (36)  2 B x=7 (49)  3 C x=9 (81)
+#       4 D x=5.5 (30.25)  5 F x=1 (1)  6 H x=0.5 (0.25)  7 I x=5.2 (27.04)
+#   links: E->A,B,C  A->E,D  B->E,F  C->E  D->A,I  F->B,H  H->F  I->D
+# F (and H behind it) hide behind B, which plain search never expands.
+# Semantics: a group pops up to mc candidates from the candidate set while
+# the closest is not beyond the worst of the ef results, evaluates their
+# unvisited neighbours at once; up to mg groups are in flight, and the
+# oldest completes (its results are merged) before the pipeline is refilled
+# (vector_search_baselines/unused/hnswalg.h:343-485, INTER/scheduler.hpp:100-157).
+DST_X = [10.0, 6.0, 7.0, 9.0, 5.5, 1.0, 0.5, 5.2]
+DST_ADJ = [[1, 2, 3], [0, 4], [0, 5], [0], [1, 7], [2, 6], [5], [4]]
+E, A, B, C, D, F, H, I = range(8)
+DST_EXPECT = {
+    # plain: expand E (A, B, C evaluated; top {A 36, B 49}), A (D; top
+    # {D, A}), D (30.25 <= 36; I; top {I, D}), I (D visited); B = 49 is
+    # beyond the worst result 30.25: stop.  6 evaluations, 4 hops.
+    (1, 1): ([I, D], [[E], [A], [D], [I]], 6),
+    # mc = 2: E; then A and B together (B = 49 is not beyond the worst, 49),
+    # D and F merged (top {F 1, D 30.25}); then F and D together (H, I; top
+    # {H, F}); then H.  8 evaluations, 6 hops, 4 groups.
+    (2, 1): ([H, F], [[E], [A, B], [F, D], [H]], 8),
+    # mg = 2: E; A is popped, and B is popped before A's group merges (the
+    # delayed threshold is still 49); A's group merges (top {D, A}); D is
+    # popped before B's group merges, then B's (top {F, D}); F popped, D's
+    # group merges I (top {F, I}); I popped, F's group merges H (top
+    # {H, F}); H popped.  8 evaluations, 7 hops, 7 groups.
+    (1, 2): ([H, F], [[E], [A], [B], [D], [F], [I], [H]], 8),
+    # mc = 2, mg = 2: after each merge the candidate set holds at most two
+    # entries, so the second group in flight is always empty and the trace
+    # is that of (2, 1).
+    (2, 2): ([H, F], [[E], [A, B], [F, D], [H]], 8),
+}
+
+
+def dst_fixture_arrays():
+    adj = -np.ones((8, 4), dtype=np.int32)
+    for v, row in enumerate(DST_ADJ):
+        adj[v, :len(row)] = row
+    x = np.zeros((8, 1), dtype=np.float32)
+    x[:, 0] = DST_X
+    return {"n": 8, "dim": 1, "metric": orc.L2, "r0": 4, "max_level": 0, "entry": E, "ru": 0, "vectors": x,
+            "adj0": adj, "upper_off": np.zeros(0, np.int64), "adj_up": np.zeros((0, 1), np.int32)}
+
+
+@pytest.mark.parametrize("mc,mg", sorted(DST_EXPECT))
+def test_dst_hand_traced_expansion_order(mc, mg):
+    og = orc.OracleGraph(dst_fixture_arrays())
+    q = np.zeros(1, dtype=np.float32)
+    ids, ds, st, groups = og.trace(q, 2, 2, mc=mc, mg=mg)
+    want_ids, want_groups, want_evals = DST_EXPECT[(mc, mg)]
+    assert ids.tolist() == want_ids
+    x = np.float32(DST_X)
+    np.testing.assert_array_equal(ds, x[want_ids] * x[want_ids])
+    assert groups == want_groups
+    assert st[0] == sum(len(gr) for gr in want_groups)        # hops
+    assert st[1] == want_evals                                 # evaluations (entry included)
+    assert st[4] == len(want_groups)                           # groups
+
+
+def test_dst_group_count_below_hops_for_multi_candidate_groups():
+    x = gvs.gen_mixture(6000, 32, 32, 21, 22)
+    q = gvs.gen_mixture(300, 32, 32, 21, 23)
+    g = gvs.HostGraph.build_hnsw(x, M=12, ef_construction=80, nthreads=8)
+    og = orc.OracleGraph(g.arrays())
+    _, _, s1 = og.search(q, 10, 48, with_groups=True)
+    assert np.array_equal(s1[:, 4], s1[:, 0])                 # mc = 1: one hop per group
+    for mc in (2, 4, 8):
+        _, _, s = og.search(q, 10, 48, mc=mc, with_groups=True)
+        assert np.all(s[:, 4] <= s[:, 0]) and np.all(s[:, 0] <= mc * s[:, 4])
+        assert s[:, 4].mean() < 0.8 * s[:, 0].mean()
+
+
+def test_rerank_matches_numpy_sort():
+    """orc_rerank_f32: distances of the candidates (orc_dist_wave, pinned
+    above) sorted by (distance, id) with numpy, first k."""
+    rng = np.random.default_rng(8)
+    base = rng.normal(0, 2, size=(500, 64)).astype(np.float32)
+    base[7] = base[3]                                           # a distance tie, broken by id
+    q = (base[3] + rng.normal(0, 0.05, size=64)).astype(np.float32)  # 3 and 7 tie at the top
+    cand = np.concatenate([[7, 3], rng.choice(500, 60, replace=False)]).astype(np.int32)
+    cand = np.unique(cand)[rng.permutation(np.unique(cand).size)].astype(np.int32)
+    for metric in (orc.L2, orc.IP):
+        d = np.array([orc.lib.orc_dist_wave(orc._p(base[c]), orc._p(q), 64, metric) for c in cand], np.float32)
+        order = np.lexsort((cand, d))[:12]
+        ids = np.empty(12, np.int32)
+        ds = np.empty(12, np.float32)
+        n = orc.lib.orc_rerank_f32(orc._p(base), 64, metric, orc._p(q), orc._p(cand), cand.size, 12, orc.WAVE,
+                                   orc._p(ids), orc._p(ds))
+        assert n == 12
+        np.testing.assert_array_equal(ids, cand[order])
+        np.testing.assert_array_equal(ds, d[order])
+        if metric == orc.L2:
+            assert ids[:2].tolist() == [3, 7]
 
 
 def test_bloom_no_false_negatives_and_fp_rate():
