@@ -40,6 +40,21 @@ __global__ __launch_bounds__(256) void k_copy16(const uint4 *__restrict__ src, u
 }
 
 // seq: fp32 rows summed in hnswlib's scalar order (GVS_FLAG_HNSWLIB_ORDER)
+// Device memory of the no-allocation search (gvs_search_device_noalloc):
+// the per-query status words, then the per-wave visited tables.  One copy
+// per GPU, held until the process ends; host code never passes `scratch`
+// itself to a kernel, only the address hipGetSymbolAddress returns for the
+// current device (noalloc_scratch below).
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+__global__ __launch_bounds__(256) void k_zero16(uint4 *__restrict__ dst, int64_t n16)
+{
+    const uint4 z = make_uint4(0u, 0u, 0u, 0u);
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = z;
+}
+
 static dfn_t pick_descend(int storage, int metric, int nch, bool seq)
 {
     const bool l2 = metric == GVS_METRIC_L2;
@@ -112,6 +127,30 @@ static uint8_t f32_to_e4m3(float f)
 }  // namespace gvs
 
 using gvs::hip_ok;
+
+namespace gvs {
+// Address of this device's `scratch` array, resolved once per device with
+// that device current (gvs_index_create is the set-up call).
+static std::mutex g_scratch_mu;
+static unsigned char *g_scratch_dev[64] = {};
+
+static int noalloc_scratch(int device, unsigned char **out)
+{
+    if (device < 0 || device >= 64) {
+        set_error("gvs: device %d out of range for the no-allocation scratch", device);
+        return GVS_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[device]) {
+        void *ptr = nullptr;
+        if (!hip_ok(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)") || !ptr)
+            return GVS_ERR_HIP;
+        g_scratch_dev[device] = (unsigned char *)ptr;
+    }
+    *out = g_scratch_dev[device];
+    return GVS_OK;
+}
+}  // namespace gvs
 
 static void free_index(gvs_index *ix)
 {
@@ -213,6 +252,14 @@ extern "C" int gvs_index_create(const gvs_graph *g, uint32_t storage_mask, gvs_i
     int gran = (ix->storage_mask & (1u << GVS_STORE_FP8)) ? 128 : (ix->storage_mask & (1u << GVS_STORE_BF16)) ? 64 : 32;
     ix->dim_pad = (g->dim + gran - 1) / gran * gran;
     (void)hipGetDevice(&ix->device);
+    {
+        unsigned char *scr = nullptr;  // set-up of the no-allocation search on this device
+        int rc = gvs::noalloc_scratch(ix->device, &scr);
+        if (rc) {
+            delete ix;
+            return rc;
+        }
+    }
     hipDeviceProp_t prop;
     if (!hip_ok(hipGetDeviceProperties(&prop, ix->device), "hipGetDeviceProperties")) {
         delete ix;
@@ -822,6 +869,75 @@ extern "C" int gvs_search_device(gvs_index *ix, const float *queries, int64_t nq
                                  int32_t *stats, void *stream)
 {
     return search_impl(ix, queries, nq, ldq, entries, p, labels, dists, stats, (hipStream_t)stream);
+}
+
+// gvs_search_device without any allocation: the persistent search with its
+// visited tables and status words in the device's `scratch` array.  The grid
+// is capped to the tables that fit; no retries (as GVS_FLAG_ASYNC).
+extern "C" int gvs_search_device_noalloc(gvs_index *ix, const float *queries, int64_t nq, int64_t ldq,
+                                         const int32_t *entries, const gvs_params *p, int64_t *labels,
+                                         float *dists, int32_t *stats, void *stream)
+{
+    if (!ix || !p || nq < 0 || (nq > 0 && (!queries || !labels || !dists)) || ldq < ix->dim) {
+        gvs::set_error("gvs_search_device_noalloc: invalid argument");
+        return GVS_ERR_ARG;
+    }
+    if (nq == 0) return GVS_OK;
+    if (p->waves_per_query != 1) {
+        gvs::set_error("gvs_search_device_noalloc: one wave per query only");
+        return GVS_ERR_UNSUPPORTED;
+    }
+    int dev = -1;
+    if (!hip_ok(hipGetDevice(&dev), "hipGetDevice")) return GVS_ERR_HIP;
+    if (dev != ix->device) {
+        gvs::set_error("gvs_search_device_noalloc: index is on device %d, current device is %d", ix->device, dev);
+        return GVS_ERR_ARG;
+    }
+    unsigned char *base = nullptr;
+    int rc = gvs::noalloc_scratch(dev, &base);
+    if (rc) return rc;
+    gvs_params q = *p;
+    q.persistent = 1;
+    q.flags = (p->flags | GVS_FLAG_ASYNC) & ~GVS_FLAG_REGION_ORDER;
+    gvs::LaunchPlan lp;
+    if ((rc = gvs::plan_search(ix, &q, q.ef, q.k, &lp, gvs::batch_waves(&q, nq)))) return rc;
+    const size_t table_bytes = (size_t)4 << lp.vlog2;
+    const size_t status_bytes = ((size_t)4 * (size_t)nq + 255) / 256 * 256;
+    const size_t room = status_bytes < NOALLOC_SCRATCH_BYTES ? NOALLOC_SCRATCH_BYTES - status_bytes : 0;
+    const int64_t fit = (int64_t)(room / table_bytes) / lp.wpb * lp.wpb;
+    if (fit < lp.wpb) {
+        gvs::set_error("gvs_search_device_noalloc: %lld queries with %zu-byte visited tables need more than the "
+                       "%zu-byte scratch (at most %zu status bytes and %d tables)", (long long)nq, table_bytes,
+                       NOALLOC_SCRATCH_BYTES, NOALLOC_SCRATCH_BYTES - (size_t)lp.wpb * table_bytes, lp.wpb);
+        return GVS_ERR_ARG;
+    }
+    const int64_t want = q.grid_waves > 0 ? q.grid_waves : lp.resident_waves;
+    q.grid_waves = (int)std::min<int64_t>(want, fit);
+    // re-plan for the capped grid (the LDS hash size depends on the waves in use)
+    if ((rc = gvs::plan_search(ix, &q, q.ef, q.k, &lp, gvs::batch_waves(&q, nq)))) return rc;
+    const int64_t slots = gvs::scratch_slots(&q, lp, nq);
+    if ((size_t)slots * ((size_t)4 << lp.vlog2) > room) {
+        gvs::set_error("gvs_search_device_noalloc: visited tables do not fit the scratch");
+        return GVS_ERR_ARG;
+    }
+    int32_t *status = (int32_t *)base;
+    uint32_t *vtab = (uint32_t *)(base + status_bytes);
+    KParams P;
+    gvs::fill_params(ix, &q, lp, P);
+    P.queries = queries;
+    P.ldq = ldq;
+    P.entries = entries;
+    P.k = q.k;
+    P.ef = std::max(q.ef, q.k);
+    P.out_labels = labels;
+    P.out_dists = dists;
+    P.out_stats = stats;
+    // the tables start empty (a fill kernel, so the sequence can be captured)
+    const int64_t n16 = (int64_t)(((size_t)slots << lp.vlog2) / 4);
+    hipLaunchKernelGGL(k_zero16, dim3((unsigned)std::min<int64_t>((n16 + 255) / 256, 4096)), dim3(256), 0,
+                       (hipStream_t)stream, (uint4 *)vtab, n16);
+    if (!hip_ok(hipGetLastError(), "launch k_zero16")) return GVS_ERR_HIP;
+    return gvs::launch_search(ix, &q, lp, P, nq, (hipStream_t)stream, vtab, status, false);
 }
 
 extern "C" int gvs_stats_per_query(void) { return GVS_STATS_ROW; }

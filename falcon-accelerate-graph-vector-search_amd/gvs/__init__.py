@@ -79,6 +79,7 @@ def _load():
         "gvs_index_device_bytes": (C.c_int, [P, P, P]),
         "gvs_search_device": (C.c_int, [P, P, I64, I64, P, P, P, P, P, P]),
         "gvs_search_host": (C.c_int, [P, P, I64, I64, P, P, P, P, P]),
+        "gvs_search_device_noalloc": (C.c_int, [P, P, I64, I64, P, P, P, P, P, P]),
         "gvs_search_overflow_count": (C.c_int, [P, P]),
         "gvs_search_counters": (C.c_int, [P, P]),
         "gvs_autotune": (C.c_int, [P, P, I64, I64, P, P, P, P, P, P]),
@@ -117,7 +118,7 @@ def exported_symbols():
     names = ["gvs_last_error", "gvs_version", "gvs_hnsw_build", "gvs_nsg_from_hnsw", "gvs_hnswlib_load",
              "gvs_hnswlib_save", "gvs_nsg_load", "gvs_fpga_load", "gvs_fpga_save", "gvs_host_graph_view", "gvs_host_graph_free",
              "gvs_gen_mixture", "gvs_index_create", "gvs_index_destroy", "gvs_index_device_bytes",
-             "gvs_search_device", "gvs_search_host", "gvs_search_overflow_count", "gvs_search_counters", "gvs_autotune", "gvs_search_graph_create",
+             "gvs_search_device", "gvs_search_host", "gvs_search_device_noalloc", "gvs_search_overflow_count", "gvs_search_counters", "gvs_autotune", "gvs_search_graph_create",
              "gvs_search_graph_launch", "gvs_search_graph_destroy", "gvs_device_copy", "gvs_stats_per_query", "gvs_hash_murmur2_keylen4_device", "gvs_bruteforce_f64_device", "gvs_server_open",
              "gvs_server_submit", "gvs_server_poll", "gvs_server_poll_stats", "gvs_server_cancel",
              "gvs_server_stats", "gvs_server_close", "gvs_server_loadgen"]
@@ -294,6 +295,16 @@ class DeviceIndex:
                                      C.c_void_p(labels_ptr), C.c_void_p(dists_ptr),
                                      C.c_void_p(stats_ptr) if stats_ptr else None,
                                      C.c_void_p(stream) if stream else None))
+
+    def search_device_noalloc(self, q_ptr, nq, ldq, labels_ptr, dists_ptr, k=10, ef=64, entries_ptr=None,
+                              stats_ptr=None, stream=None, **kw):
+        """gvs_search_device_noalloc: the same search with no allocation at call time."""
+        p = make_params(k=k, ef=ef, **kw)
+        _check(lib.gvs_search_device_noalloc(self._h, C.c_void_p(q_ptr), nq, ldq,
+                                             C.c_void_p(entries_ptr) if entries_ptr else None, C.byref(p),
+                                             C.c_void_p(labels_ptr), C.c_void_p(dists_ptr),
+                                             C.c_void_p(stats_ptr) if stats_ptr else None,
+                                             C.c_void_p(stream) if stream else None))
 
     def device_bytes(self):
         """Bytes resident on the device: {"graph": ..., "vectors": ...}."""

@@ -200,6 +200,22 @@ GVS_API int gvs_index_device_bytes(const gvs_index *ix, int64_t *graph_bytes, in
 GVS_API int gvs_search_device(gvs_index *ix, const float *queries, int64_t nq, int64_t ldq,
                               const int32_t *entries, const gvs_params *p,
                               int64_t *labels, float *dists, int32_t *stats, void *stream);
+/* gvs_search_device with no allocation at call time, for callers that cannot
+ * call hipMalloc (code under HIP graph capture, tight serving loops).  No
+ * counterpart in the reference.  The search runs as the persistent kernel
+ * (persistent is implied, region order and waves_per_query > 1 are not
+ * available); its per-query status words and per-wave visited tables live in
+ * a fixed 64 MiB device array of the library, one per GPU, held until the
+ * process ends; the grid is capped to the visited tables that fit beside the
+ * status words, and a batch whose status words leave no room for one
+ * workgroup's tables returns GVS_ERR_ARG.  Like GVS_FLAG_ASYNC, queries whose
+ * visited table overflows return label -1 and are counted by
+ * gvs_search_overflow_count(); there is no retry.  All calls on one GPU
+ * share the array: calls that overlap on one GPU (two streams, two threads)
+ * race and must be serialised by the caller. */
+GVS_API int gvs_search_device_noalloc(gvs_index *ix, const float *queries, int64_t nq, int64_t ldq,
+                                      const int32_t *entries, const gvs_params *p,
+                                      int64_t *labels, float *dists, int32_t *stats, void *stream);
 /* Number of queries of the last search on this index whose visited table
  * overflowed (only non-zero after a GVS_FLAG_ASYNC search; synchronises). */
 GVS_API int gvs_search_overflow_count(gvs_index *ix, int64_t *count);
