@@ -458,7 +458,8 @@ int plan_search(const gvs_index *ix, const gvs_params *p, int max_ef, int max_k,
     // with the slower frozen lookups plus the HBM table.
     const int nlog2 = lds_sized ? p->visited_log2 : 14;
     const int nbits = std::max(nlog2, ilog2_ceil(std::max<int64_t>(ix->n, 2)));
-    const bool narrow = !(p->flags & GVS_FLAG_HBM_VISITED) && W == 1 && efe > 128 && nbits <= nlog2 + 8 &&
+    const bool wide = (p->flags & GVS_FLAG_LDS_VISITED) && (p->flags & GVS_FLAG_LDS_WIDE);
+    const bool narrow = !(p->flags & GVS_FLAG_HBM_VISITED) && !wide && W == 1 && efe > 128 && nbits <= nlog2 + 8 &&
                         ((p->flags & GVS_FLAG_LDS_VISITED) || efe <= 512);
     if (narrow) {
         lp->hlog2 = nlog2;
@@ -941,6 +942,26 @@ extern "C" int gvs_search_device_noalloc(gvs_index *ix, const float *queries, in
 }
 
 extern "C" int gvs_stats_per_query(void) { return GVS_STATS_ROW; }
+
+extern "C" int gvs_search_plan(gvs_index *ix, int64_t nq, const gvs_params *p, int32_t plan[8])
+{
+    if (!ix || !p || !plan || nq < 1) {
+        gvs::set_error("gvs_search_plan: invalid argument");
+        return GVS_ERR_ARG;
+    }
+    gvs::LaunchPlan lp;
+    const int rc = gvs::plan_search(ix, p, p->ef, p->k, &lp, gvs::batch_waves(p, nq));
+    if (rc) return rc;
+    plan[0] = lp.hlog2;
+    plan[1] = lp.hnarrow;
+    plan[2] = lp.vlog2;
+    plan[3] = lp.blog2;
+    plan[4] = lp.lds_per_wave;
+    plan[5] = lp.wpb;
+    plan[6] = (int32_t)std::min<int64_t>(lp.resident_waves, INT32_MAX);
+    plan[7] = lp.lcap;
+    return GVS_OK;
+}
 
 extern "C" int gvs_hash_murmur2_keylen4_device(const uint32_t *keys, const uint32_t *seeds, uint32_t *out,
                                                int64_t n, void *stream)

@@ -20,6 +20,7 @@ GVS_OK = 0
 FLAG_ASYNC = 1
 FLAG_HBM_VISITED = 2
 FLAG_LDS_VISITED = 4
+FLAG_LDS_WIDE = 32
 FLAG_REGION_ORDER = 8
 FLAG_HNSWLIB_ORDER = 16
 ERRORS = {-1: "ARG", -2: "HIP", -3: "NOMEM", -4: "UNSUPPORTED", -5: "OVERFLOW", -6: "IO", -7: "STATE"}
@@ -88,6 +89,7 @@ def _load():
         "gvs_search_graph_destroy": (C.c_int, [P]),
         "gvs_device_copy": (C.c_int, [P, P, I64, P]),
         "gvs_stats_per_query": (C.c_int, []),
+        "gvs_search_plan": (C.c_int, [P, I64, P, P]),
         "gvs_hash_murmur2_keylen4_device": (C.c_int, [P, P, P, I64, P]),
         "gvs_bruteforce_f64_device": (C.c_int, [P, P, I64, I64, I32, P, P, P]),
         "gvs_server_open": (C.c_int, [P, P, P]),
@@ -119,7 +121,7 @@ def exported_symbols():
              "gvs_hnswlib_save", "gvs_nsg_load", "gvs_fpga_load", "gvs_fpga_save", "gvs_host_graph_view", "gvs_host_graph_free",
              "gvs_gen_mixture", "gvs_index_create", "gvs_index_destroy", "gvs_index_device_bytes",
              "gvs_search_device", "gvs_search_host", "gvs_search_device_noalloc", "gvs_search_overflow_count", "gvs_search_counters", "gvs_autotune", "gvs_search_graph_create",
-             "gvs_search_graph_launch", "gvs_search_graph_destroy", "gvs_device_copy", "gvs_stats_per_query", "gvs_hash_murmur2_keylen4_device", "gvs_bruteforce_f64_device", "gvs_server_open",
+             "gvs_search_graph_launch", "gvs_search_graph_destroy", "gvs_device_copy", "gvs_stats_per_query", "gvs_search_plan", "gvs_hash_murmur2_keylen4_device", "gvs_bruteforce_f64_device", "gvs_server_open",
              "gvs_server_submit", "gvs_server_poll", "gvs_server_poll_stats", "gvs_server_cancel",
              "gvs_server_stats", "gvs_server_close", "gvs_server_loadgen"]
     return [n for n in names if hasattr(lib, n)]
@@ -236,9 +238,10 @@ class HostGraph:
 def make_params(k=10, ef=64, mc=1, mg=1, waves_per_query=1, storage=F32, rerank=False, persistent=False,
                 visited_log2=0, bloom_log2=0, grid_waves=0, asynchronous=False, visited=None, region_order=False,
                 hnswlib_order=False):
-    """visited: None (auto), "hbm" (LDS Bloom + HBM hash) or "lds" (exact LDS
+    """visited: None (auto), "hbm" (LDS Bloom + HBM hash), "lds" (exact LDS
     hash; 2-byte slots when max(ef, k) > 128 and the graph's ids fit them;
-    visited_log2 in [10, 14] then sets its size).
+    visited_log2 in [10, 14] then sets its size) or "lds4" (the same with
+    4-byte slots at every ef).
     region_order: run the batch in region order (persistent mode; GVS_FLAG_REGION_ORDER).
     hnswlib_order: sum distances in hnswlib's scalar order (GVS_FLAG_HNSWLIB_ORDER)."""
     flags = FLAG_ASYNC if asynchronous else 0
@@ -250,6 +253,8 @@ def make_params(k=10, ef=64, mc=1, mg=1, waves_per_query=1, storage=F32, rerank=
         flags |= FLAG_HBM_VISITED
     elif visited == "lds":
         flags |= FLAG_LDS_VISITED
+    elif visited == "lds4":
+        flags |= FLAG_LDS_VISITED | FLAG_LDS_WIDE
     return Params(k, ef, mc, mg, waves_per_query, storage, int(bool(rerank)), int(bool(persistent)),
                   visited_log2, bloom_log2, grid_waves, flags)
 
@@ -305,6 +310,14 @@ class DeviceIndex:
                                              C.c_void_p(labels_ptr), C.c_void_p(dists_ptr),
                                              C.c_void_p(stats_ptr) if stats_ptr else None,
                                              C.c_void_p(stream) if stream else None))
+
+    def plan(self, nq, k=10, ef=64, **kw):
+        """The launch plan of a search of nq queries (gvs_search_plan)."""
+        p = make_params(k=k, ef=ef, **kw)
+        out = (C.c_int32 * 8)()
+        _check(lib.gvs_search_plan(self._h, nq, C.byref(p), out))
+        return dict(zip(("lds_hash_log2", "lds_hash_2byte", "hbm_table_log2", "bloom_log2", "lds_bytes",
+                         "waves_per_block", "resident_waves", "list_capacity"), list(out)))
 
     def device_bytes(self):
         """Bytes resident on the device: {"graph": ..., "vectors": ...}."""

@@ -95,6 +95,9 @@ typedef struct gvs_params {
  * LDS hash size; its slots are 2 bytes when ef_eff > 128 and the graph fits). */
 #define GVS_FLAG_HBM_VISITED 2
 #define GVS_FLAG_LDS_VISITED 4
+/* With GVS_FLAG_LDS_VISITED: 4-byte LDS hash slots (id + 1) at every ef,
+ * instead of 2-byte ones above ef 128. */
+#define GVS_FLAG_LDS_WIDE 32
 /* Region order (persistent mode, no given entries, graphs with >= 2 upper
  * levels): an upper-layer descent kernel first finds every query's entry
  * and the nodes it settled on at levels 3 and 2; the batch is sorted by
@@ -228,6 +231,14 @@ GVS_API int gvs_search_counters(gvs_index *ix, int64_t counters[3]);
  * pointers and size).  No reference counterpart; used to calibrate HBM byte
  * counters against a kernel of known traffic. */
 GVS_API int gvs_device_copy(void *dst, const void *src, int64_t bytes, void *stream);
+/* The launch plan gvs_search_device would use for nq queries with *p:
+ * plan[0] log2 LDS visited-hash slots (0 = none: Bloom filter + HBM table
+ * from the start), [1] 1 if those slots are 2 bytes, [2] log2 HBM
+ * visited-table words per wave, [3] log2 LDS Bloom words, [4] LDS bytes per
+ * wave (per workgroup with several waves per query), [5] waves per
+ * workgroup, [6] resident waves, [7] candidate-list capacity.  No reference
+ * counterpart (diagnostics). */
+GVS_API int gvs_search_plan(gvs_index *ix, int64_t nq, const gvs_params *p, int32_t plan[8]);
 /* MurmurHash2_KeyLen4(keys[i], seeds[i]) -> out[i] for n DEVICE words, with
  * the device function the LDS Bloom filter hashes ids with; the reference's
  * Bloom hash (INTER/bloom_filter.hpp:43-66,

@@ -1,0 +1,148 @@
+"""GPU parity at the benchmark configurations (through the C ABI, against the
+oracle on the same seeded inputs and the same graph):
+
+* 1M x 96, M=32 (layer-0 degree 64), efC=200, 10k queries, ef 32/64/128/256
+  (the plain inter-query configuration), every query;
+* 1M x 128, M=32, efC=200, 10k queries at the tuned point of the benchmark
+  (ef 58, mc 4, persistent, region order), every query;
+* batch-1 intra-query DST on a degree-64 graph: 1/2/4/8 waves per query x
+  mc 1/2/4/8 x ef 64/256/512, eager and as a captured graph;
+* an NSG-style flat graph of out-degree 64 traversed in BF16 and FP8-e4m3
+  (per-vector scale) at ef 128, with and without the exact fp32 rerank.
+
+Bar: ids, distances and the four per-query counters equal (the kernel's
+summation order; the oracle's hnswlib order is pinned separately)."""
+import numpy as np
+import pytest
+
+import gvs
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+_CACHE = {}
+
+
+def bench_graph(n, dim, M=32, efc=200, centres=256):
+    key = (n, dim, M, efc, centres)
+    if key not in _CACHE:
+        x = gvs.gen_mixture(n, dim, centres, 1234, 1235, nthreads=16)
+        q = gvs.gen_mixture(10000, dim, centres, 1234, 98765, nthreads=16)
+        g = gvs.HostGraph.build_hnsw(x, M=M, ef_construction=efc, nthreads=16)
+        _CACHE.clear()  # one 1M graph at a time
+        _CACHE[key] = dict(x=x, q=q, g=g, dev=gvs.DeviceIndex(g), og=orc.OracleGraph(g.arrays()))
+    return _CACHE[key]
+
+
+def check(dev, og, q, k, ef, mc=1, mg=1, nthreads=16, **kw):
+    lab, dist, st = dev.search(q, k=k, ef=ef, mc=mc, mg=mg, with_stats=True, **kw)
+    ids, ds, ost = og.search(q, k, ef, mc=mc, mg=mg, order=orc.WAVE, nthreads=nthreads)
+    np.testing.assert_array_equal(lab, ids.astype(np.int64))
+    np.testing.assert_array_equal(dist, ds)
+    np.testing.assert_array_equal(st.astype(np.int64), ost)
+    return lab, st
+
+
+@pytest.mark.parametrize("ef", [32, 64, 128, 256])
+def test_c2_full_size_every_query(ef):
+    c = bench_graph(1_000_000, 96)
+    print(f"case 1M x 96 ef={ef}: 10k queries", flush=True)
+    lab, st = check(c["dev"], c["og"], c["q"], 10, ef)
+    print(f"  mean hops {st[:, 0].mean():.1f} evals {st[:, 1].mean():.1f}", flush=True)
+
+
+def test_headline_tuned_config_every_query():
+    c = bench_graph(1_000_000, 128)
+    assert c["g"].view.max_level >= 2
+    print("case 1M x 128 ef=58 mc=4 persistent region order: 10k queries", flush=True)
+    lab, st = check(c["dev"], c["og"], c["q"], 10, 58, mc=4, persistent=True, region_order=True)
+    gt, _ = orc.bruteforce_f64(c["x"], c["q"][:1000], 10, nthreads=16)
+    rec = orc.recall_at(lab[:1000], gt, 10)
+    print(f"  recall@10 on 1k queries {rec:.4f}", flush=True)
+    assert rec >= 0.94
+
+
+_C3 = {}
+
+
+def c3_graph():
+    if not _C3:
+        x = gvs.gen_mixture(50000, 128, 64, 61, 62)
+        g = gvs.HostGraph.build_hnsw(x, M=32, ef_construction=200, nthreads=16)
+        assert g.view.r0 == 64
+        _C3.update(g=g, dev=gvs.DeviceIndex(g), og=orc.OracleGraph(g.arrays()),
+                   q=gvs.gen_mixture(12, 128, 64, 61, 63))
+    return _C3
+
+
+@pytest.mark.parametrize("ef", [64, 256, 512])
+@pytest.mark.parametrize("W", [1, 2, 4, 8])
+def test_c3_batch1_intra_query_dst(W, ef):
+    """One query per call, eager and replayed from a captured graph, equals
+    the oracle's DST(mc, 1) for every mc."""
+    from gvs import hip as H
+    c = c3_graph()
+    dev, og, q = c["dev"], c["og"], c["q"]
+    dq = H.DeviceArray((1, 128), np.float32)
+    dl = H.DeviceArray((1, 10), np.int64)
+    dd = H.DeviceArray((1, 10), np.float32)
+    ds = H.DeviceArray((1, gvs.STATS_ROW), np.int32)
+    st = H.Stream()
+    for mc in (1, 2, 4, 8):
+        print(f"case C3 W={W} mc={mc} ef={ef}", flush=True)
+        ids, dis, ost = og.search(q, 10, ef, mc=mc, order=orc.WAVE)
+        sg = dev.capture(dq.address, 1, 128, dl.address, dd.address, k=10, ef=ef, mc=mc, waves_per_query=W,
+                         stats_ptr=ds.address)
+        for i in range(q.shape[0]):
+            lab, dist, cnt = dev.search(q[i:i + 1], k=10, ef=ef, mc=mc, waves_per_query=W, with_stats=True)
+            np.testing.assert_array_equal(lab[0], ids[i])
+            np.testing.assert_array_equal(dist[0], dis[i])
+            np.testing.assert_array_equal(cnt[0].astype(np.int64), ost[i])
+            dq.upload(q[i:i + 1])
+            sg.launch(st.handle)
+            st.synchronize()
+            np.testing.assert_array_equal(dl.to_host()[0], ids[i])
+            np.testing.assert_array_equal(dd.to_host()[0], dis[i])
+            np.testing.assert_array_equal(ds.to_host()[0, :gvs.STATS_PER_QUERY].astype(np.int64), ost[i])
+        sg.close()
+
+
+_C4 = {}
+
+
+@pytest.mark.parametrize("storage", [gvs.BF16, gvs.FP8])
+def test_c4_nsg_low_precision_with_and_without_rerank(storage):
+    """NSG-style flat graph (out-degree 64, navigating-node entry), ef=128,
+    k=10: the low-precision traversal equals the oracle on the same
+    quantised rows; the fp32 rerank of the final 128 candidates equals the
+    oracle's rerank."""
+    import lowprec
+    if not _C4:
+        x = gvs.gen_mixture(50000, 128, 64, 81, 82)
+        g = gvs.HostGraph.build_hnsw(x, M=32, ef_construction=200, nthreads=16)
+        flat = g.to_nsg(64)
+        a = flat.arrays()
+        assert a["max_level"] == 0 and a["r0"] == 64
+        codes, scale = lowprec.quantize_fp8_rows(a["vectors"])
+        _C4.update(x=x, flat=flat, a=a, dev=gvs.DeviceIndex(flat, storages=(gvs.F32, gvs.BF16, gvs.FP8)),
+                   bf16=lowprec.to_bf16(a["vectors"]), codes=codes, scale=scale,
+                   q=gvs.gen_mixture(1000, 128, 64, 81, 83))
+    c = _C4
+    a = c["a"]
+    if storage == gvs.BF16:
+        og = orc.OracleGraph(a, dim_pad=128, storage=orc.BF16, bf16=c["bf16"])
+    else:
+        og = orc.OracleGraph(a, dim_pad=128, storage=orc.FP8, fp8=c["codes"], scale=c["scale"])
+    q = c["q"]
+    print(f"case C4 NSG storage={storage} ef=128", flush=True)
+    check(c["dev"], og, q, 10, 128, storage=storage)
+    lab, dist = c["dev"].search(q, k=10, ef=128, storage=storage, rerank=True)
+    cand, _, _ = og.search(q, 128, 128, order=orc.WAVE)
+    ri, rd = orc.rerank_f32(og, q, cand, 10)
+    np.testing.assert_array_equal(lab, ri)
+    np.testing.assert_array_equal(dist, rd)
+    gt, gd = orc.bruteforce_f64(c["x"], q, 10)
+    l_nr, d_nr = c["dev"].search(q, k=10, ef=128, storage=storage)
+    rel = np.abs(d_nr[:, 0] - gd[:, 0]) / gd[:, 0]
+    print(f"  recall@10 traversal {orc.recall_at(l_nr, gt, 10):.4f} reranked {orc.recall_at(lab, gt, 10):.4f}; "
+          f"top-1 distance rel. error mean {rel.mean():.2e} max {rel.max():.2e}", flush=True)

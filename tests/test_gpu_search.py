@@ -149,27 +149,51 @@ def test_visited_table_overflow_retry_is_exact():
 
 @pytest.mark.parametrize("mc", [1, 4])
 def test_large_ef_lds_hash_matches_hbm_table_and_oracle(mc):
-    """ef in (128, 512]: the exact LDS hash of 2-byte slots (the default)
-    gives the oracle's results and the same as the HBM table behind a Bloom
-    filter.  Forced down to 2^10 slots at ef=200, every query outgrows it,
-    freezes it and goes on with the HBM table (frozen 2-byte lookups), still
-    exact.  ef=1000 uses the Bloom filter and HBM table by default."""
+    """ef in (128, 512]: the default exact LDS hash (for a batch of 100 queries
+    far below the resident grid: 2^13 4-byte slots, whose 32 KiB cost no
+    usable waves) gives the oracle's results and the same as the HBM table
+    behind a Bloom filter.  Forced to 2-byte slots of 2^10 at ef=200, every
+    query outgrows the hash, freezes it and goes on with the HBM table
+    (frozen 2-byte lookups), still exact.  ef=1000 uses the Bloom filter and
+    HBM table by default."""
     x, g, dev = graph(20000, 128, 16, 100)
     og = orc.OracleGraph(g.arrays())
     q = queries(100, 128, seed=18)
     for ef in (200, 400):
+        plan = dev.plan(100, k=10, ef=ef, mc=mc)
+        assert (plan["lds_hash_log2"], plan["lds_hash_2byte"]) == (13, 0), plan
         print(f"case large ef mc={mc} ef={ef}", flush=True)
         a = check_equal(dev, og, q, 10, ef, mc=mc)
         b = dev.search(q, k=10, ef=ef, mc=mc, with_stats=True, visited="hbm")
         for u, v in zip(a, b):
             np.testing.assert_array_equal(u, v)
     print(f"case large ef mc={mc} ef=200 2-byte hash of 2^10 slots (freeze)", flush=True)
+    assert dev.plan(100, k=10, ef=200, mc=mc, visited="lds", visited_log2=10)["lds_hash_2byte"] == 1
     check_equal(dev, og, q, 10, 200, mc=mc, visited="lds", visited_log2=10)
     dev.search(q, k=10, ef=200, mc=mc, visited="lds", visited_log2=10)
     c = dev.counters()
     assert c["spilled"] > 0 and c["overflowed"] == 0
     print(f"case large ef mc={mc} ef=1000 (Bloom filter + HBM table)", flush=True)
+    assert dev.plan(40, k=10, ef=1000, mc=mc)["lds_hash_log2"] == 0
     check_equal(dev, og, q[:40], 10, 1000, mc=mc)
+
+
+@pytest.mark.parametrize("ef", [400, 512])
+def test_large_ef_four_byte_hash_freezes_and_spills(ef):
+    """4-byte LDS slots above ef 128 (the plan's choice where 32 KiB costs no
+    waves), forced down to 2^11 slots: every query outgrows the hash,
+    freezes it and goes on with the HBM table; results, distances and
+    counters equal the oracle's, nothing overflows."""
+    x, g, dev = graph(20000, 128, 16, 100)
+    og = orc.OracleGraph(g.arrays())
+    q = queries(100, 128, seed=17)
+    plan = dev.plan(100, k=10, ef=ef, visited="lds4", visited_log2=11)
+    assert (plan["lds_hash_log2"], plan["lds_hash_2byte"]) == (11, 0), plan
+    print(f"case 4-byte LDS hash 2^11 ef={ef} (freeze)", flush=True)
+    check_equal(dev, og, q, 10, ef, visited="lds4", visited_log2=11)
+    dev.search(q, k=10, ef=ef, visited="lds4", visited_log2=11)
+    c = dev.counters()
+    assert c["spilled"] > 0 and c["overflowed"] == 0, c
 
 
 def test_large_ef_full_batch_two_byte_hash():
@@ -180,6 +204,8 @@ def test_large_ef_full_batch_two_byte_hash():
     x, g, dev = graph(20000, 128, 16, 100)
     og = orc.OracleGraph(g.arrays())
     q = queries(4000, 128, seed=19)
+    plan = dev.plan(4000, k=10, ef=200)
+    assert (plan["lds_hash_log2"], plan["lds_hash_2byte"]) == (13, 1), plan
     print("case large ef full batch ef=200", flush=True)
     a = dev.search(q, k=10, ef=200, with_stats=True)
     b = dev.search(q, k=10, ef=200, with_stats=True, visited="hbm")
