@@ -191,17 +191,6 @@ __device__ __forceinline__ uint32_t murmur2_k4(uint32_t key, uint32_t seed)
     return h;
 }
 
-// e4m3 (OCP) -> f32, exact
-__device__ __forceinline__ float e4m3_to_f32(uint32_t v)
-{
-    uint32_t s = (v & 0x80u) << 24;
-    uint32_t e = (v >> 3) & 15u, m = v & 7u;
-    float sub = (float)m * 0x1p-9f;
-    float nor = __uint_as_float(s | ((e + 120u) << 23) | (m << 20));
-    float r = e ? nor : (s ? -sub : sub);
-    return r;
-}
-
 template <int STORE>
 struct StoreTraits;
 template <>
@@ -242,40 +231,44 @@ __device__ __forceinline__ void decode_chunk(const uint4 raw, float scale, float
             x[2 * i + 1] = __uint_as_float(w[i] & 0xFFFF0000u);
         }
     } else {
+        // gfx950 converts two OCP e4m3 codes (one 16-bit half of a word) to
+        // two fp32 per instruction, exactly.  The per-vector scale is not
+        // applied here: elem() folds it into the distance arithmetic.
         const uint32_t w[4] = {raw.x, raw.y, raw.z, raw.w};
 #pragma unroll
-        for (int i = 0; i < 4; i++)
-#pragma unroll
-            for (int b = 0; b < 4; b++) x[4 * i + b] = __fmul_rn(e4m3_to_f32((w[i] >> (8 * b)) & 0xFFu), scale);  // rounded before use (no fma contraction)
+        for (int i = 0; i < 4; i++) {
+            const auto lo = __builtin_amdgcn_cvt_pk_f32_fp8((int)w[i], false);
+            const auto hi = __builtin_amdgcn_cvt_pk_f32_fp8((int)w[i], true);
+            x[4 * i + 0] = lo[0];
+            x[4 * i + 1] = lo[1];
+            x[4 * i + 2] = hi[0];
+            x[4 * i + 3] = hi[1];
+        }
+        (void)scale;
     }
 }
 
-// partial distance of this lane's chunks of row `id`
-template <int NCH, int STORE, int METRIC>
-__device__ __forceinline__ float row_partial(const KParams &P, const float *qf, uint32_t id, int p)
+// One element's term of a lane's partial.  L2: (x - q)^2, accumulated with
+// one fma; for FP8 rows x = code * scale and the difference is formed in one
+// rounding, fma(code, scale, -q).  IP: x * q accumulated; FP8 rows accumulate
+// code * q and the lane's partial is multiplied by the scale once
+// (finish_partial).
+template <int STORE, int METRIC>
+__device__ __forceinline__ float elem(float x, float q, float scale, float acc)
 {
-    constexpr int E = StoreTraits<STORE>::E;
-    constexpr int B = StoreTraits<STORE>::BYTES;
-    // row stride in bytes fits 32 bits: one 32x32+64 multiply-add per row
-    const unsigned char *row = (const unsigned char *)P.vec + (uint64_t)id * (uint32_t)(P.row_elems * B);
-    float scale = 1.0f;
-    if constexpr (STORE == STORE_FP8) scale = P.vscale[id];
-    float acc = 0.0f;
-#pragma unroll
-    for (int i = 0; i < NCH; i++) {
-        uint4 raw = *(const uint4 *)(row + (size_t)(p + 8 * i) * 16);
-        float x[E];
-        decode_chunk<STORE>(raw, scale, x);
-#pragma unroll
-        for (int t = 0; t < E; t++) {
-            if constexpr (METRIC == METRIC_L2) {
-                float d = x[t] - qf[i * E + t];
-                acc = __builtin_fmaf(d, d, acc);
-            } else {
-                acc = __builtin_fmaf(x[t], qf[i * E + t], acc);
-            }
-        }
+    if constexpr (METRIC == METRIC_L2) {
+        const float d = (STORE == STORE_FP8) ? __builtin_fmaf(x, scale, -q) : x - q;
+        return __builtin_fmaf(d, d, acc);
+    } else {
+        (void)scale;
+        return __builtin_fmaf(x, q, acc);
     }
+}
+template <int STORE, int METRIC>
+__device__ __forceinline__ float finish_partial(float acc, float scale)
+{
+    if constexpr (STORE == STORE_FP8 && METRIC == METRIC_IP) return __fmul_rn(acc, scale);
+    (void)scale;
     return acc;
 }
 
@@ -351,16 +344,9 @@ __device__ __forceinline__ void eval_block(const KParams &P, const float *qf, co
             float x[E];
             decode_chunk<STORE>(raw[t][i], scale[t], x);
 #pragma unroll
-            for (int e = 0; e < E; e++) {
-                if constexpr (METRIC == METRIC_L2) {
-                    const float d = x[e] - qf[i * E + e];
-                    a = __builtin_fmaf(d, d, a);
-                } else {
-                    a = __builtin_fmaf(x[e], qf[i * E + e], a);
-                }
-            }
+            for (int e = 0; e < E; e++) a = elem<STORE, METRIC>(x[e], qf[i * E + e], scale[t], a);
         }
-        acc[t] = a;  // rows past n only meet lanes of the same row, and are dropped below
+        acc[t] = finish_partial<STORE, METRIC>(a, scale[t]);  // rows past n only meet lanes of the same row, and are dropped below
     }
     const float v = reduce8x4(acc, p);
     const int r = b0 + (p & 3) * 8 + g;

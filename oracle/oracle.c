@@ -152,8 +152,37 @@ static inline float stored_elem(const orc_graph *g, int64_t id, int i)
     }
 }
 
+/* FP8 rows in the kernels' order: as orc_dist_wave_e with E = 16, where the
+ * element term of L2 is d = fma(code, scale, -q) (one rounding), then
+ * acc = fma(d, d, acc), and for IP acc = fma(code, q, acc) with each lane's
+ * partial multiplied by the scale once before the butterfly. */
+static float dist_wave_fp8(const uint8_t *codes, float scale, const float *q, int dim_pad, int metric)
+{
+    float s[8];
+    int nchunk = dim_pad / 16;
+    for (int p = 0; p < 8; p++) {
+        float acc = 0.0f;
+        for (int c = p; c < nchunk; c += 8)
+            for (int t = 0; t < 16; t++) {
+                int i = 16 * c + t;
+                float x = orc_e4m3_to_f32(codes[i]);
+                if (metric == ORC_L2) {
+                    float d = fmaf(x, scale, -q[i]);
+                    acc = fmaf(d, d, acc);
+                } else {
+                    acc = fmaf(x, q[i], acc);
+                }
+            }
+        s[p] = metric == ORC_L2 ? acc : acc * scale;
+    }
+    float r = wave_combine(s);
+    return metric == ORC_L2 ? r : 1.0f - r;
+}
+
 float orc_dist(const orc_graph *g, const float *q, int64_t id, int order)
 {
+    if (g->storage == ORC_STORE_FP8 && order == ORC_ORDER_WAVE)
+        return dist_wave_fp8(g->vec_fp8 + id * (int64_t)g->dim_pad, g->vec_scale[id], q, g->dim_pad, g->metric);
     if (g->storage == ORC_STORE_F32) {
         const float *x = g->vec_f32 + id * (int64_t)g->dim_pad;
         return order == ORC_ORDER_SEQ ? orc_dist_seq(q, x, g->dim, g->metric)

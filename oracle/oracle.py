@@ -46,6 +46,8 @@ def _load():
     lib.orc_dist_wave_e.argtypes = [P, P, I, I, I]
     lib.orc_dist_f64.restype = C.c_double
     lib.orc_dist_f64.argtypes = [P, P, I, I]
+    lib.orc_dist.restype = C.c_float
+    lib.orc_dist.argtypes = [P, P, I64, I]
     lib.orc_search_batch.restype = I
     lib.orc_search_batch.argtypes = [P, P, I64, I, I, I, I, I, P, P, P, P, I]
     lib.orc_bruteforce_f64.restype = None

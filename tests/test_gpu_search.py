@@ -521,3 +521,25 @@ def test_device_bloom_hash_matches_reference_known_answers():
     gvs.murmur2_keylen4_device(dk.address, ds.address, do.address, n)
     H.synchronize()
     np.testing.assert_array_equal(do.to_host(), z["h_keylen4"])
+
+
+def test_fp8_decode_every_code():
+    """Two rows holding every non-NaN OCP e4m3 code once (positive codes in
+    row 0, negative in row 1, each row padded with 448 so its scale is 1 and
+    every value is its own code): the FP8 traversal's distances to a random
+    query equal the oracle's decode (orc_e4m3_to_f32 x scale) bit for bit."""
+    import lowprec
+    grid = lowprec.GRID  # codes 0..0x7E
+    x = np.full((2, 128), 448.0, dtype=np.float32)
+    x[0, :127] = grid
+    x[1, :127] = -grid
+    a = {"n": 2, "dim": 128, "metric": gvs.L2, "r0": 1, "max_level": 0, "entry": 0, "ru": 0, "vectors": x,
+         "adj0": np.array([[1], [0]], dtype=np.int32), "upper_off": np.zeros(0, np.int64),
+         "adj_up": np.zeros((0, 1), np.int32)}
+    codes, scale = lowprec.quantize_fp8_rows(x)
+    assert np.all(scale == 1.0) and len(set(codes.ravel().tolist())) == 254
+    dev = gvs.DeviceIndex(ArrayGraph(a), storages=(gvs.F32, gvs.FP8))
+    og = orc.OracleGraph(a, dim_pad=128, storage=orc.FP8, fp8=codes, scale=scale)
+    q = np.random.default_rng(4).normal(0, 30, size=(8, 128)).astype(np.float32)
+    print("case fp8 every code", flush=True)
+    check_equal(dev, og, q, 2, 2, storage=gvs.FP8)

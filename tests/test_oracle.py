@@ -94,6 +94,34 @@ def test_wave_order_distance_close_to_f64():
                 assert abs(s - ref) <= 1e-5 * scale + 1e-6
 
 
+@pytest.mark.parametrize("metric", [orc.L2, orc.IP])
+def test_fp8_wave_distance_close_to_f64_closed_form(metric):
+    """FP8 rows (code * per-vector scale): the kernels'-order distance is
+    within 1e-5 relative of sum((e4m3(c) * s - q)^2) (L2) or 1 - sum(e4m3(c)
+    * s * q) (IP) in FP64; ignoring the scale or a code fails by far."""
+    import lowprec
+    rng = np.random.default_rng(12)
+    x = rng.normal(0, 3, size=(64, 256)).astype(np.float32)
+    codes, scale = lowprec.quantize_fp8_rows(x)
+    a = {"n": 64, "dim": 256, "metric": metric, "r0": 1, "max_level": 0, "entry": 0, "ru": 0, "vectors": x,
+         "adj0": np.zeros((64, 1), np.int32), "upper_off": np.zeros(0, np.int64), "adj_up": np.zeros((0, 1), np.int32)}
+    og = orc.OracleGraph(a, dim_pad=256, storage=orc.FP8, fp8=codes, scale=scale)
+    dec = np.array([orc.lib.orc_e4m3_to_f32(int(c)) for c in range(256)], dtype=np.float64)
+    for j in range(8):
+        q = rng.normal(0, 3, size=256).astype(np.float32)
+        for i in range(64):
+            xv = dec[codes[i]] * np.float64(scale[i])
+            ref = np.sum((xv - q) ** 2) if metric == orc.L2 else 1.0 - np.dot(xv, q.astype(np.float64))
+            mag = abs(ref) if metric == orc.L2 else np.sum(np.abs(xv * q))
+            got = orc.lib.orc_dist(C_byref(og.g), orc._p(q), i, orc.WAVE)
+            assert abs(got - ref) <= 1e-5 * max(mag, 1e-6), (i, got, ref)
+
+
+def C_byref(s):
+    import ctypes
+    return ctypes.byref(s)
+
+
 def test_wave_order_is_the_stated_butterfly():
     """Closed form of the summation order on a vector where order matters."""
     dim = 32
