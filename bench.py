@@ -191,34 +191,32 @@ def intra_latency(a, gvs, g, q):
                       "best_per_ef": best, "rows": rows}), flush=True)
 
 
-def serve_child(a, gvs, g, q, ks=None, efs=None, metric_name="L2", max_inflight=256, grid_waves=0):
+def serve_child(a, gvs, g, q, ks=None, efs=None, metric_name="L2", max_inflight=4096, grid_waves=0, nreq=100000,
+                waves_per_request=1, mc=1):
     """Open-loop Poisson load at fractions of the serving loop's saturated
     throughput; latency runs from a request's arrival to the poll that sees
     its result.  ks / efs: per-request k and ef (default: a.k, a.serve_ef).
-    max_inflight: device slots, which also bound the serving grid.  Light
-    requests (1M x 128, ef 60) are served best by one wave per CU (256):
-    idle serving waves poll the host ring with system-scope acquires, which
-    invalidate L2 under the running searches, so extra waves add latency
-    (p99 at 600k requests/s: 0.29 ms with 256 slots, 0.39 ms with 4096)
-    while the saturated rate stays above 1M/s.  Heavy requests (ef up to
-    512 at D=384) need more waves in flight, but not every resident one:
-    over 30k requests that mix saturates at 620k requests/s on 768 waves
-    (3 per CU) against about 300k on 896 or all 1,024 resident ones.
-    grid_waves: the serving grid (0 = resident waves, at most max_inflight)."""
+    max_inflight: device slots, which also bound the serving grid.
+    grid_waves: the serving grid (0 = resident waves, at most max_inflight).
+    waves_per_request / mc: one wave per request, or one workgroup of that
+    many waves running DST(mc, 1) (lower latency for heavy requests).  The
+    query set is cycled to nreq requests; the host-side time split of the
+    saturated run is reported with the latencies."""
     dev = gvs.DeviceIndex(g)
-    nreq = min(len(q), 20000)
-    qq = q[:nreq]
-    ks = np.full(nreq, a.k, dtype=np.int32) if ks is None else ks[:nreq]
-    efs = np.full(nreq, a.serve_ef, dtype=np.int32) if efs is None else efs[:nreq]
+    qq = np.ascontiguousarray(np.tile(q, (nreq // len(q) + 1, 1))[:nreq])
+    ks = np.full(nreq, a.k, dtype=np.int32) if ks is None else np.resize(ks, nreq).astype(np.int32)
+    efs = np.full(nreq, a.serve_ef, dtype=np.int32) if efs is None else np.resize(efs, nreq).astype(np.int32)
     max_k, max_ef = int(ks.max()), int(max(efs.max(), ks.max()))
-    srv = gvs.Server(dev, max_inflight=max_inflight, max_k=max_k, max_ef=max_ef, grid_waves=grid_waves)
+    srv = gvs.Server(dev, max_inflight=max_inflight, max_k=max_k, max_ef=max_ef, grid_waves=grid_waves, mc=mc,
+                     waves_per_request=waves_per_request)
     # untimed warm-up: the first requests pay the serving kernel's first
     # launch and the first touches of the pinned rings
-    nw = min(nreq, 2000)
+    nw = min(nreq, 5000)
     srv.loadgen(qq[:nw], 1e9, ks[:nw], efs[:nw], seed=0)
     _, _, lat, wall = srv.loadgen(qq, 1e9, ks, efs, seed=1)
     peak = nreq / wall
-    log(f"[serve] saturated {peak:.0f} QPS, p50 {np.percentile(lat, 50):.0f} us")
+    host = srv.loadgen_breakdown()
+    log(f"[serve] saturated {peak:.0f} QPS; host side {host}")
     loads = []
     for frac in (0.25, 0.5, 0.75, 0.9):
         _, _, lat, wall = srv.loadgen(qq, peak * frac, ks, efs, seed=2)
@@ -230,8 +228,8 @@ def serve_child(a, gvs, g, q, ks=None, efs=None, metric_name="L2", max_inflight=
     kdesc = a.k if len(set(ks.tolist())) == 1 else sorted(set(ks.tolist()))
     edesc = a.serve_ef if len(set(efs.tolist())) == 1 else [int(efs.min()), int(efs.max())]
     print(json.dumps({"requests": nreq, "k": kdesc, "ef": edesc, "peak_qps": round(peak, 1),
-                      "max_inflight": max_inflight, "grid_waves": grid_waves,
-                      "workload": f"{a.n}x{a.dim} {metric_name}",
+                      "max_inflight": max_inflight, "grid_waves": grid_waves, "waves_per_request": waves_per_request,
+                      "mc": mc, "workload": f"{a.n}x{a.dim} {metric_name}", "host_breakdown_saturated": host,
                       "loads": loads}), flush=True)
 
 
@@ -320,8 +318,12 @@ def main():
         rng = np.random.default_rng(5)
         ks = rng.choice([1, 10, 100], size=len(q)).astype(np.int32)
         efs = rng.integers(32, 513, size=len(q)).astype(np.int32)
+        # one wave per request (throughput), then one workgroup of 8 waves
+        # running DST(8, 1) per request (latency of the heavy requests)
         serve_child(a, gvs, g, q, ks=ks, efs=efs, metric_name="inner product, unit-norm", max_inflight=4096,
-                    grid_waves=768)
+                    grid_waves=768, nreq=30000)
+        serve_child(a, gvs, g, q, ks=ks, efs=efs, metric_name="inner product, unit-norm", max_inflight=4096,
+                    grid_waves=1024, nreq=30000, waves_per_request=8, mc=8)
         return
     if a.intra:
         intra_latency(a, gvs, g, q)

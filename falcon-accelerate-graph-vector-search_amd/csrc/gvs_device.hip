@@ -466,6 +466,13 @@ int plan_search(const gvs_index *ix, const gvs_params *p, int max_ef, int max_k,
         lp->hnarrow = 1;
         lp->hbits = nbits;
         lp->blog2 = p->bloom_log2 > 0 ? p->bloom_log2 : 1;
+    } else if (!(p->flags & GVS_FLAG_HBM_VISITED) && W > 1 && efe > 128 && efe <= 512 && !lds_sized) {
+        // several waves per query share one workgroup's LDS (up to 160 KiB):
+        // an exact 4-byte hash of 2^13 slots (2^14 above ef 256), which holds
+        // the evaluations of such queries, frozen and spilled to the HBM
+        // table as the one-wave hashes are
+        lp->hlog2 = efe > 256 ? 14 : 13;
+        lp->blog2 = p->bloom_log2 > 0 ? p->bloom_log2 : 1;
     } else if (!(p->flags & GVS_FLAG_HBM_VISITED) && ((p->flags & GVS_FLAG_LDS_VISITED) || efe <= 128)) {
         // 4-byte slots (2^12 hold the p99 query at ef 64-96 on degree-64
         // graphs; in a full batch 2^13 costs more in occupancy than the

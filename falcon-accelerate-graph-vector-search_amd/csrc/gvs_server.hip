@@ -23,6 +23,7 @@
 #include <cstring>
 #include <mutex>
 #include <random>
+#include <string>
 #include <thread>
 #include <vector>
 
@@ -43,6 +44,7 @@ struct Slot {
     int64_t req = -1;
     uint64_t ring_idx = 0;
     double t_arrival = 0.0;  // microseconds, steady clock
+    int32_t k = 0;           // the request's k (its results are k entries)
     bool busy = false;
     bool cancelled = false;
 };
@@ -61,6 +63,7 @@ struct gvs_server {
     sfn_t fn = nullptr;
     hipStream_t stream = nullptr;
     int64_t grid_waves = 0;
+    size_t shm = 0;          // dynamic LDS per workgroup
     bool launched = false;
     // pinned host memory
     ServeCtl *ctl = nullptr;
@@ -72,21 +75,35 @@ struct gvs_server {
     float *dists = nullptr;
     int32_t *stats = nullptr;
     // device
-    uint64_t *d_head = nullptr, *d_served = nullptr;
+    uint64_t *d_head = nullptr, *d_served = nullptr, *d_mirror = nullptr;
     uint32_t *d_vtab = nullptr;  // the server's own per-wave HBM visited tables
     int32_t *d_status = nullptr;
-    // host bookkeeping
-    std::mutex mu;
+    // host bookkeeping.  Three locks: submit_mu serialises submitters (ring
+    // tails, request writes, publication), poll_mu serialises pollers
+    // (completion ring head, result copies), and mu guards what both touch
+    // (free slots, slot records, the in-flight list, ring-entry reuse,
+    // counters, the device loop's state).  mu is held for bookkeeping only;
+    // the per-request copies run outside it.
+    std::mutex mu, submit_mu, poll_mu;
     std::vector<Slot> slots;
     std::vector<int> free_slots, inflight;
     std::vector<int> inflight_pos;   // slot -> index in inflight
     uint64_t *cring = nullptr;       // pinned completion ring (written by the device)
     uint64_t chead = 0;              // completions consumed
     double last_query_us = -1e30;    // last hipStreamQuery
-    std::vector<uint8_t> ring_busy;  // ring entry still names an incomplete request
-    uint64_t next_seq = 1, tail = 0;
+    std::vector<uint8_t> ring_busy;  // ring entry (q * rcap + i) still names an incomplete request
+    struct Admitted { int slot; size_t rpos; int64_t m; };
+    std::vector<Admitted> adm;       // submit_mu: the batch being admitted
+    struct Done { int slot; bool ovf, cancelled; int64_t req; double t_arrival; int k; };
+    std::vector<Done> comp;          // poll_mu: the completions being delivered
+    uint64_t tails[SERVE_QUEUES] = {};      // requests appended to each ring
+    uint64_t published[SERVE_QUEUES] = {};  // ... and published to the device
+    int rr = 0;                              // ring the next request tries first
+    uint64_t next_seq = 1;
     int64_t next_req = 0;
-    int64_t c_submitted = 0, c_completed = 0, c_cancelled = 0, c_rejected = 0, c_loops = 0;
+    int64_t c_submitted = 0, c_completed = 0, c_cancelled = 0, c_rejected = 0, c_loops = 0, c_overflowed = 0;
+    // host-side breakdown of the last gvs_server_loadgen (see gvs.h)
+    double lg_breakdown[6] = {0, 0, 0, 0, 0, 0};
 };
 
 static void serve_free(gvs_server *s)
@@ -105,6 +122,7 @@ static void serve_free(gvs_server *s)
     (void)hipHostFree(s->dists);
     (void)hipHostFree(s->stats);
     (void)hipFree(s->d_head);
+    (void)hipFree(s->d_mirror);
     (void)hipFree(s->d_served);
     (void)hipFree(s->d_vtab);
     (void)hipFree(s->d_status);
@@ -128,8 +146,7 @@ static int serve_ensure_running(gvs_server *s)
         }
         s->launched = false;
     }
-    const size_t shm = (size_t)s->lp.wpb * s->lp.lds_per_wave;
-    hipLaunchKernelGGL(s->fn, dim3((unsigned)(s->grid_waves / s->lp.wpb)), dim3(s->lp.wpb * WAVE), shm, s->stream,
+    hipLaunchKernelGGL(s->fn, dim3((unsigned)(s->grid_waves / s->lp.wpb)), dim3(s->lp.wpb * WAVE), s->shm, s->stream,
                        s->args);
     if (!gvs::hip_ok(hipGetLastError(), "launch k_serve")) return GVS_ERR_HIP;
     s->launched = true;
@@ -139,9 +156,10 @@ static int serve_ensure_running(gvs_server *s)
 
 extern "C" int gvs_server_open(gvs_index *ix, const gvs_server_config *cfg, gvs_server **out)
 {
+    const int W = cfg ? (cfg->waves_per_request > 0 ? cfg->waves_per_request : 1) : 1;
     if (!ix || !cfg || !out || cfg->max_inflight < 1 || cfg->max_inflight > (1 << 20) || cfg->max_k < 1 ||
-        cfg->max_ef < 1 || cfg->mc < 0 || cfg->storage != GVS_STORE_F32) {
-        gvs::set_error("gvs_server_open: invalid configuration (fp32 storage only)");
+        cfg->max_ef < 1 || cfg->mc < 0 || cfg->storage != GVS_STORE_F32 || (W != 1 && W != 2 && W != 4 && W != 8)) {
+        gvs::set_error("gvs_server_open: invalid configuration (fp32 storage; 1, 2, 4 or 8 waves per request)");
         return GVS_ERR_ARG;
     }
     gvs_server *s = new gvs_server();
@@ -153,20 +171,36 @@ extern "C" int gvs_server_open(gvs_index *ix, const gvs_server_config *cfg, gvs_
     p.ef = cfg->max_ef;
     p.mc = cfg->mc > 0 ? cfg->mc : 1;
     p.mg = 1;
-    p.waves_per_query = 1;
+    p.waves_per_query = W;
     p.storage = GVS_STORE_F32;
     p.persistent = 1;
     // the grid holds at most grid_waves (or max_inflight) waves: the
     // visited hash may use LDS that only waves beyond those would need
     int rc = gvs::plan_search(ix, &p, cfg->max_ef, cfg->max_k, &s->lp,
-                              cfg->grid_waves > 0 ? cfg->grid_waves : cfg->max_inflight);
+                              cfg->grid_waves > 0 ? cfg->grid_waves : (int64_t)cfg->max_inflight * W);
     if (rc) {
         delete s;
         return rc;
     }
     const int nch = ix->dim_pad / 32;
     const int rows = p.mc * ((ix->r0 + WAVE - 1) / WAVE);
-    s->fn = ix->metric == GVS_METRIC_L2 ? gvs_serve_kernel_f32_l2(nch, rows) : gvs_serve_kernel_f32_ip(nch, rows);
+    const bool l2 = ix->metric == GVS_METRIC_L2;
+    if (W == 1) {
+        s->fn = l2 ? gvs_serve_kernel_f32_l2(nch, rows) : gvs_serve_kernel_f32_ip(nch, rows);
+        s->shm = (size_t)s->lp.wpb * s->lp.lds_per_wave;
+    } else {
+        // one workgroup of W waves per request (its LDS: lp.lds_per_wave)
+        const int rpw = (rows + W - 1) / W;
+        s->fn = l2 ? gvs_serve_kernel_mw_f32_l2(nch, rpw) : gvs_serve_kernel_mw_f32_ip(nch, rpw);
+        s->shm = (size_t)s->lp.lds_per_wave;
+        if (s->fn && s->shm > 65536 &&
+            !gvs::hip_ok(hipFuncSetAttribute((const void *)s->fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                             (int)s->shm),
+                         "hipFuncSetAttribute")) {
+            delete s;
+            return GVS_ERR_HIP;
+        }
+    }
     if (!s->fn) {
         delete s;
         gvs::set_error("gvs_server_open: dim %d not compiled for serving", ix->dim);
@@ -174,38 +208,45 @@ extern "C" int gvs_server_open(gvs_index *ix, const gvs_server_config *cfg, gvs_
     }
     int blocks_per_cu = 0;
     if (!gvs::hip_ok(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks_per_cu, (const void *)s->fn,
-                                                                  s->lp.wpb * WAVE,
-                                                                  (size_t)s->lp.wpb * s->lp.lds_per_wave),
+                                                                  s->lp.wpb * WAVE, s->shm),
                      "occupancy")) {
         delete s;
         return GVS_ERR_HIP;
     }
     s->grid_waves = cfg->grid_waves > 0 ? cfg->grid_waves
                                         : (int64_t)std::max(1, blocks_per_cu) * ix->cu_count * s->lp.wpb;
-    // a request occupies one wave, so waves beyond max_inflight could never
-    // hold work; they would only add pollers
-    if (cfg->grid_waves <= 0) s->grid_waves = std::min<int64_t>(s->grid_waves, (int64_t)cfg->max_inflight);
+    // a request occupies one wave (one workgroup of W), so waves beyond
+    // max_inflight requests could never hold work; they would only add pollers
+    if (cfg->grid_waves <= 0) s->grid_waves = std::min<int64_t>(s->grid_waves, (int64_t)cfg->max_inflight * W);
     s->grid_waves = (s->grid_waves + s->lp.wpb - 1) / s->lp.wpb * s->lp.wpb;
     const int64_t R = cfg->max_inflight;
-    // ring of twice the slot count: an entry is reused only after the request
-    // that held it is complete (checked at submit), so a holder never reads
-    // an overwritten entry
-    uint32_t rcap = 2;
-    while (rcap < 2u * (uint32_t)R) rcap <<= 1;
+    // Each ring holds up to rcap entries; an entry is reused only after the
+    // request that held it is complete (checked at admission, which then
+    // tries the next ring), so a holder never reads an overwritten entry.
+    // Requests go to the rings round robin.
+    uint32_t rcap = 16;
+    while (rcap < 4u * (uint32_t)((R + SERVE_QUEUES - 1) / SERVE_QUEUES)) rcap <<= 1;
+    uint32_t ccap = 2;  // completion ring: at most R entries are unconsumed
+    while (ccap < 2u * (uint32_t)R) ccap <<= 1;
     const int dim = ix->dim;
     bool ok = gvs::hip_ok(hipHostMalloc((void **)&s->ctl, sizeof(ServeCtl), hipHostMallocCoherent), "hipHostMalloc") &&
-              gvs::hip_ok(hipHostMalloc((void **)&s->ring, 4 * (size_t)rcap, hipHostMallocCoherent), "hipHostMalloc") &&
+              gvs::hip_ok(hipHostMalloc((void **)&s->ring, 4 * (size_t)rcap * SERVE_QUEUES, hipHostMallocCoherent),
+                          "hipHostMalloc") &&
               gvs::hip_ok(hipHostMalloc((void **)&s->qbuf, 4 * (size_t)R * dim, hipHostMallocCoherent), "hipHostMalloc") &&
               gvs::hip_ok(hipHostMalloc((void **)&s->qk, 4 * (size_t)R, hipHostMallocCoherent), "hipHostMalloc") &&
               gvs::hip_ok(hipHostMalloc((void **)&s->qef, 4 * (size_t)R, hipHostMallocCoherent), "hipHostMalloc") &&
               gvs::hip_ok(hipHostMalloc((void **)&s->qseq, 8 * (size_t)R, hipHostMallocCoherent), "hipHostMalloc") &&
               gvs::hip_ok(hipHostMalloc((void **)&s->done, 8 * (size_t)R, hipHostMallocCoherent), "hipHostMalloc") &&
-              gvs::hip_ok(hipHostMalloc((void **)&s->cring, 8 * (size_t)rcap, hipHostMallocCoherent), "hipHostMalloc") &&
+              gvs::hip_ok(hipHostMalloc((void **)&s->cring, 8 * (size_t)ccap, hipHostMallocCoherent), "hipHostMalloc") &&
               gvs::hip_ok(hipHostMalloc((void **)&s->labels, 8 * (size_t)R * cfg->max_k, hipHostMallocCoherent), "hipHostMalloc") &&
               gvs::hip_ok(hipHostMalloc((void **)&s->dists, 4 * (size_t)R * cfg->max_k, hipHostMallocCoherent), "hipHostMalloc") &&
               gvs::hip_ok(hipHostMalloc((void **)&s->stats, 4 * GVS_STATS_ROW * (size_t)R, hipHostMallocCoherent), "hipHostMalloc") &&
-              gvs::hip_ok(hipMalloc(&s->d_head, 8), "hipMalloc") && gvs::hip_ok(hipMalloc(&s->d_served, 8), "hipMalloc") &&
-              gvs::hip_ok(hipMemset(s->d_head, 0, 8), "hipMemset") && gvs::hip_ok(hipMemset(s->d_served, 0, 8), "hipMemset") &&
+              gvs::hip_ok(hipMalloc(&s->d_head, 8 * SERVE_QUEUES * SERVE_HEAD_STRIDE), "hipMalloc") &&
+              gvs::hip_ok(hipMalloc(&s->d_served, 8), "hipMalloc") &&
+              gvs::hip_ok(hipMalloc(&s->d_mirror, 8 * SERVE_MIRROR_WORDS), "hipMalloc") &&
+              gvs::hip_ok(hipMemset(s->d_mirror, 0, 8 * SERVE_MIRROR_WORDS), "hipMemset") &&
+              gvs::hip_ok(hipMemset(s->d_head, 0, 8 * SERVE_QUEUES * SERVE_HEAD_STRIDE), "hipMemset") &&
+              gvs::hip_ok(hipMemset(s->d_served, 0, 8), "hipMemset") &&
               gvs::hip_ok(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "hipStreamCreateWithFlags");
     const size_t vwords = (size_t)s->grid_waves << s->lp.vlog2;
     if (ok)
@@ -218,11 +259,11 @@ extern "C" int gvs_server_open(gvs_index *ix, const gvs_server_config *cfg, gvs_
     }
     std::memset(s->ctl, 0, sizeof(ServeCtl));
     std::memset(s->done, 0, 8 * (size_t)R);
-    std::memset(s->cring, 0, 8 * (size_t)rcap);
+    std::memset(s->cring, 0, 8 * (size_t)ccap);
     s->inflight_pos.assign((size_t)R, -1);
     std::memset(s->qseq, 0, 8 * (size_t)R);
     s->slots.assign((size_t)R, Slot());
-    s->ring_busy.assign(rcap, 0);
+    s->ring_busy.assign((size_t)rcap * SERVE_QUEUES, 0);
     for (int64_t i = R - 1; i >= 0; i--) s->free_slots.push_back((int)i);
     KParams P;
     gvs::fill_params(ix, &p, s->lp, P);
@@ -232,6 +273,7 @@ extern "C" int gvs_server_open(gvs_index *ix, const gvs_server_config *cfg, gvs_
     P.out_labels = s->labels;
     P.out_dists = s->dists;
     P.out_stats = s->stats;
+    P.out_k_only = 1;  // only a request's own k results cross the bus
     P.persistent = 1;
     P.vtab = s->d_vtab;
     P.status = s->d_status;
@@ -246,8 +288,10 @@ extern "C" int gvs_server_open(gvs_index *ix, const gvs_server_config *cfg, gvs_
     s->args.qseq = s->qseq;
     s->args.done = s->done;
     s->args.cring = s->cring;
-    s->args.ccap = rcap;
+    s->args.ccap = ccap;
+    s->args.nslots = (uint32_t)R;
     s->args.head = s->d_head;
+    s->args.mirror = s->d_mirror;
     s->args.served = s->d_served;
     s->args.idle_ticks = 20 * 1000 * 1000;  // 200 ms at 100 MHz
     std::lock_guard<std::mutex> lk(s->mu);
@@ -260,43 +304,88 @@ extern "C" int gvs_server_open(gvs_index *ix, const gvs_server_config *cfg, gvs_
     return GVS_OK;
 }
 
-static int64_t submit_locked(gvs_server *s, const float *query, int32_t k, int32_t ef, double t_arrival)
+// Admit up to n requests in order, stopping at the first that is outside
+// the admitted range (GVS_ERR_ARG, counted as rejected) or finds no free
+// slot or ring entry (GVS_ERR_STATE), write them into their slots and ring
+// entries, and publish them together (one release of each ring's tail).
+// Caller holds s->submit_mu.  Returns the number admitted, or the error of
+// the first request when none was.
+static int64_t submit_batch_impl(gvs_server *s, const float *queries, int64_t n, int64_t ldq, const int32_t *ks,
+                                 const int32_t *efs, int64_t *req_ids, const double *t_arrival)
 {
-    if (k < 1 || k > s->cfg.max_k || ef < 1 || ef > s->cfg.max_ef) {
-        s->c_rejected++;
-        gvs::set_error("gvs_server_submit: k=%d ef=%d outside the admitted range (max_k %d, max_ef %d)", k, ef,
-                       s->cfg.max_k, s->cfg.max_ef);
-        return GVS_ERR_ARG;
+    const double t = now_us();
+    const uint64_t rmask = (uint64_t)s->args.rcap - 1u;
+    int64_t err = 0;
+    s->adm.clear();
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        for (int64_t m = 0; m < n; m++) {
+            const int32_t k = ks[m], ef = efs[m];
+            if (k < 1 || k > s->cfg.max_k || ef < 1 || ef > s->cfg.max_ef) {
+                s->c_rejected++;
+                gvs::set_error("gvs_server_submit: k=%d ef=%d outside the admitted range (max_k %d, max_ef %d)", k,
+                               ef, s->cfg.max_k, s->cfg.max_ef);
+                err = GVS_ERR_ARG;
+                break;
+            }
+            // a ring whose next entry is free (an entry is reused only after
+            // the request that held it is complete), round robin
+            int q = -1;
+            for (int j = 0; j < SERVE_QUEUES && !s->free_slots.empty(); j++) {
+                const int c = (s->rr + j) % SERVE_QUEUES;
+                if (!s->ring_busy[(size_t)c * s->args.rcap + (s->tails[c] & rmask)]) {
+                    q = c;
+                    break;
+                }
+            }
+            if (q < 0) {
+                gvs::set_error("gvs_server_submit: all %d device slots busy", s->cfg.max_inflight);
+                err = GVS_ERR_STATE;
+                break;
+            }
+            s->rr = (q + 1) % SERVE_QUEUES;
+            const int slot = s->free_slots.back();
+            s->free_slots.pop_back();
+            const size_t rpos = (size_t)q * s->args.rcap + (size_t)(s->tails[q] & rmask);
+            s->tails[q]++;
+            Slot &sl = s->slots[(size_t)slot];
+            sl.req = s->next_req++;
+            sl.t_arrival = t_arrival ? t_arrival[m] : t;
+            sl.k = k;
+            sl.busy = true;
+            sl.cancelled = false;
+            sl.ring_idx = rpos;
+            s->ring_busy[rpos] = 1;
+            s->inflight_pos[(size_t)slot] = (int)s->inflight.size();
+            s->inflight.push_back(slot);
+            s->c_submitted++;
+            if (req_ids) req_ids[m] = sl.req;
+            s->adm.push_back({slot, rpos, m});
+        }
     }
-    if (s->free_slots.empty() || s->ring_busy[s->tail & (uint64_t)(s->args.rcap - 1u)]) {
-        gvs::set_error("gvs_server_submit: all %d device slots busy", s->cfg.max_inflight);
-        return GVS_ERR_STATE;
+    // the admitted slots and ring entries belong to this batch until published
+    const int dim = s->ix->dim;
+    for (const auto &a : s->adm) {
+        std::memcpy(s->qbuf + (size_t)a.slot * dim, queries + a.m * ldq, sizeof(float) * dim);
+        s->qk[a.slot] = ks[a.m];
+        s->qef[a.slot] = efs[a.m];
+        s->qseq[a.slot] = s->next_seq++;
+        s->ring[a.rpos] = (uint32_t)a.slot;
     }
-    const int slot = s->free_slots.back();
-    s->free_slots.pop_back();
-    std::memcpy(s->qbuf + (size_t)slot * s->ix->dim, query, sizeof(float) * s->ix->dim);
-    s->qk[slot] = k;
-    s->qef[slot] = ef;
-    const uint64_t seq = s->next_seq++;
-    s->qseq[slot] = seq;
-    Slot &sl = s->slots[(size_t)slot];
-    sl.req = s->next_req++;
-    sl.t_arrival = t_arrival;
-    sl.busy = true;
-    sl.cancelled = false;
-    s->inflight_pos[(size_t)slot] = (int)s->inflight.size();
-    s->inflight.push_back(slot);
-    sl.ring_idx = s->tail;
-    s->ring_busy[s->tail & (uint64_t)(s->args.rcap - 1u)] = 1;
-    s->ring[s->tail & (uint64_t)(s->args.rcap - 1u)] = (uint32_t)slot;
-    s->tail++;
-    // request fields and the ring entry are globally visible before the tail
-    std::atomic_thread_fence(std::memory_order_seq_cst);
-    __atomic_store_n(&s->ctl->tail, s->tail, __ATOMIC_RELEASE);
-    s->c_submitted++;
-    int rc = serve_ensure_running(s);
-    if (rc) return rc;
-    return sl.req;
+    if (!s->adm.empty()) {
+        // request fields and ring entries are globally visible before the tails
+        std::atomic_thread_fence(std::memory_order_seq_cst);
+        for (int q = 0; q < SERVE_QUEUES; q++)
+            if (s->tails[q] != s->published[q]) {
+                __atomic_store_n(&s->ctl->tail[q], s->tails[q], __ATOMIC_RELEASE);
+                s->published[q] = s->tails[q];
+            }
+        std::lock_guard<std::mutex> lk(s->mu);
+        const int rc = serve_ensure_running(s);
+        if (rc) return rc;
+    }
+    if (s->adm.empty() && err == GVS_ERR_ARG) return err;
+    return (int64_t)s->adm.size();  // 0 when no slot was free (the message says so)
 }
 
 extern "C" int64_t gvs_server_submit(gvs_server *s, const float *query, int32_t k, int32_t ef)
@@ -305,52 +394,120 @@ extern "C" int64_t gvs_server_submit(gvs_server *s, const float *query, int32_t 
         gvs::set_error("gvs_server_submit: invalid argument");
         return GVS_ERR_ARG;
     }
-    std::lock_guard<std::mutex> lk(s->mu);
-    return submit_locked(s, query, k, ef, now_us());
+    std::lock_guard<std::mutex> lk(s->submit_mu);
+    int64_t id = -1;
+    const int64_t r = submit_batch_impl(s, query, 1, s->ix->dim, &k, &ef, &id, nullptr);
+    return r == 1 ? id : r == 0 ? GVS_ERR_STATE : r;
 }
 
-static int poll_locked(gvs_server *s, int32_t max_done, int64_t *req_ids, int64_t *labels, float *dists,
-                       double *latency_us, int32_t *stats_out)
+extern "C" int64_t gvs_server_submit_batch(gvs_server *s, const float *queries, int64_t n, int64_t ldq,
+                                           const int32_t *ks, const int32_t *efs, int64_t *req_ids)
+{
+    if (!s || n < 0 || (n > 0 && (!queries || !ks || !efs)) || ldq < s->ix->dim) {
+        gvs::set_error("gvs_server_submit_batch: invalid argument");
+        return GVS_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lk(s->submit_mu);
+    return submit_batch_impl(s, queries, n, ldq, ks, efs, req_ids, nullptr);
+}
+
+// Deliver up to max_done completions, in the order the device finished
+// them; caller holds s->poll_mu.  The device wrote each request's own k
+// results into its slot's rows; rows are delivered max_k wide, the rest
+// label -1 / dist +inf.
+static int poll_impl(gvs_server *s, int32_t max_done, int64_t *req_ids, int64_t *labels, float *dists,
+                     double *latency_us, int32_t *stats_out, int32_t *status_out)
 {
     const int K = s->cfg.max_k;
     const uint64_t cmask = (uint64_t)s->args.ccap - 1u;
-    int n = 0;
-    // completions in the order the device finished them: entry i carries
-    // (i + 1) in its high word once it is written
-    while (n < max_done) {
+    const uint32_t err = __atomic_load_n(&s->ctl->err, __ATOMIC_ACQUIRE);
+    if (err) {
+        gvs::set_error("gvs_server_poll: the device read an out-of-range slot index from ring entry %u", err - 1u);
+        return GVS_ERR_STATE;
+    }
+    // completion ring entry i carries (i + 1) in its high word once it is
+    // written, then the overflow bit and the slot
+    s->comp.clear();
+    while ((int32_t)s->comp.size() < max_done) {
         const uint64_t v = __atomic_load_n(&s->cring[s->chead & cmask], __ATOMIC_ACQUIRE);
         if ((uint32_t)(v >> 32) != (uint32_t)(s->chead + 1u)) break;
-        const int slot = (int)(uint32_t)v;
-        if (slot < 0 || slot >= s->cfg.max_inflight || !s->slots[(size_t)slot].busy) {
-            gvs::set_error("gvs_server_poll: completion ring names slot %d, which is not in flight", slot);
+        const int slot = (int)((uint32_t)v & 0x7FFFFFFFu);
+        if (slot >= s->cfg.max_inflight) {
+            gvs::set_error("gvs_server_poll: completion ring names slot %d of %d", slot, s->cfg.max_inflight);
             return GVS_ERR_STATE;
         }
         s->chead++;
-        const double t = now_us();
-        Slot &sl = s->slots[(size_t)slot];
-        if (!sl.cancelled) {
-            if (req_ids) req_ids[n] = sl.req;
-            if (labels) std::memcpy(labels + (size_t)n * K, s->labels + (size_t)slot * K, 8 * (size_t)K);
-            if (dists) std::memcpy(dists + (size_t)n * K, s->dists + (size_t)slot * K, 4 * (size_t)K);
-            if (latency_us) latency_us[n] = t - sl.t_arrival;
-            if (stats_out)
-                std::memcpy(stats_out + (size_t)n * GVS_STATS_ROW, s->stats + (size_t)slot * GVS_STATS_ROW,
-                            4 * GVS_STATS_ROW);
-            n++;
-            s->c_completed++;
+        s->comp.push_back({slot, ((uint32_t)v >> 31) != 0u, false, 0, 0.0, 0});
+    }
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        for (auto &c : s->comp) {
+            const Slot &sl = s->slots[(size_t)c.slot];
+            if (!sl.busy) {
+                gvs::set_error("gvs_server_poll: completion ring names slot %d, which is not in flight", c.slot);
+                return GVS_ERR_STATE;
+            }
+            c.req = sl.req;
+            c.t_arrival = sl.t_arrival;
+            c.cancelled = sl.cancelled;
+            c.k = sl.k;
         }
+        if (s->comp.empty()) {
+            if (!s->inflight.empty()) {
+                const int rc = serve_ensure_running(s);  // a launch that went idle leaves work behind
+                if (rc) return rc;
+            }
+            return 0;
+        }
+    }
+    // copies, outside mu: a slot is freed only below
+    const double t = now_us();
+    int n = 0;
+    const size_t nc = s->comp.size();
+    for (size_t j = 0; j < nc; j++) {
+        // the device wrote these rows across the bus: the host reads them from
+        // memory, so the rows of a few completions ahead are requested early
+        if (j + 6 < nc) {
+            const int ps = s->comp[j + 6].slot;
+            __builtin_prefetch(s->labels + (size_t)ps * K);
+            __builtin_prefetch(s->dists + (size_t)ps * K);
+        }
+        const auto &c = s->comp[j];
+        if (c.cancelled) continue;
+        const int k = std::min(K, std::max(1, c.k));
+        if (req_ids) req_ids[n] = c.req;
+        if (labels) {
+            std::memcpy(labels + (size_t)n * K, s->labels + (size_t)c.slot * K, 8 * (size_t)k);
+            std::fill(labels + (size_t)n * K + k, labels + (size_t)(n + 1) * K, (int64_t)-1);
+        }
+        if (dists) {
+            std::memcpy(dists + (size_t)n * K, s->dists + (size_t)c.slot * K, 4 * (size_t)k);
+            std::fill(dists + (size_t)n * K + k, dists + (size_t)(n + 1) * K, INFINITY);
+        }
+        if (latency_us) latency_us[n] = t - c.t_arrival;
+        if (stats_out)
+            std::memcpy(stats_out + (size_t)n * GVS_STATS_ROW, s->stats + (size_t)c.slot * GVS_STATS_ROW,
+                        4 * GVS_STATS_ROW);
+        if (status_out) status_out[n] = c.ovf ? GVS_ERR_OVERFLOW : GVS_OK;
+        n++;
+    }
+    std::lock_guard<std::mutex> lk(s->mu);
+    for (const auto &c : s->comp) {
+        Slot &sl = s->slots[(size_t)c.slot];
+        if (!c.cancelled) s->c_completed++;
+        if (c.ovf) s->c_overflowed++;
         sl.busy = false;
-        s->ring_busy[sl.ring_idx & (uint64_t)(s->args.rcap - 1u)] = 0;
-        s->free_slots.push_back(slot);
-        const int ip = s->inflight_pos[(size_t)slot];
+        s->ring_busy[sl.ring_idx] = 0;
+        s->free_slots.push_back(c.slot);
+        const int ip = s->inflight_pos[(size_t)c.slot];
         const int last = s->inflight.back();
         s->inflight[(size_t)ip] = last;
         s->inflight_pos[(size_t)last] = ip;
         s->inflight.pop_back();
-        s->inflight_pos[(size_t)slot] = -1;
+        s->inflight_pos[(size_t)c.slot] = -1;
     }
     if (!s->inflight.empty()) {
-        int rc = serve_ensure_running(s);  // a launch that went idle leaves work behind
+        const int rc = serve_ensure_running(s);
         if (rc) return rc;
     }
     return n;
@@ -363,8 +520,8 @@ extern "C" int gvs_server_poll(gvs_server *s, int32_t max_done, int64_t *req_ids
         gvs::set_error("gvs_server_poll: invalid argument");
         return GVS_ERR_ARG;
     }
-    std::lock_guard<std::mutex> lk(s->mu);
-    return poll_locked(s, max_done, req_ids, labels, dists, latency_us, nullptr);
+    std::lock_guard<std::mutex> lk(s->poll_mu);
+    return poll_impl(s, max_done, req_ids, labels, dists, latency_us, nullptr, nullptr);
 }
 
 extern "C" int gvs_server_poll_stats(gvs_server *s, int32_t max_done, int64_t *req_ids, int64_t *labels,
@@ -374,8 +531,19 @@ extern "C" int gvs_server_poll_stats(gvs_server *s, int32_t max_done, int64_t *r
         gvs::set_error("gvs_server_poll_stats: invalid argument");
         return GVS_ERR_ARG;
     }
-    std::lock_guard<std::mutex> lk(s->mu);
-    return poll_locked(s, max_done, req_ids, labels, dists, latency_us, stats);
+    std::lock_guard<std::mutex> lk(s->poll_mu);
+    return poll_impl(s, max_done, req_ids, labels, dists, latency_us, stats, nullptr);
+}
+
+extern "C" int gvs_server_poll_status(gvs_server *s, int32_t max_done, int64_t *req_ids, int64_t *labels,
+                                      float *dists, double *latency_us, int32_t *stats, int32_t *status)
+{
+    if (!s || max_done < 0) {
+        gvs::set_error("gvs_server_poll_status: invalid argument");
+        return GVS_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lk(s->poll_mu);
+    return poll_impl(s, max_done, req_ids, labels, dists, latency_us, stats, status);
 }
 
 extern "C" int gvs_server_cancel(gvs_server *s, int64_t req_id)
@@ -408,12 +576,23 @@ extern "C" int gvs_server_stats(gvs_server *s, int64_t counters[6])
     return GVS_OK;
 }
 
+extern "C" int gvs_server_counters(gvs_server *s, int64_t *counters, int32_t n)
+{
+    if (!s || !counters || n < 0) return GVS_ERR_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    const int64_t c[7] = {s->c_submitted, s->c_submitted, s->c_completed, s->c_cancelled,
+                          s->c_rejected,  s->c_loops,     s->c_overflowed};
+    for (int i = 0; i < n && i < 7; i++) counters[i] = c[i];
+    return std::min<int32_t>(n, 7);
+}
+
 extern "C" int gvs_server_close(gvs_server *s)
 {
     if (!s) return GVS_ERR_ARG;
     {
-        std::lock_guard<std::mutex> lk(s->mu);
-        __atomic_store_n(&s->ctl->stop, 1u, __ATOMIC_RELEASE);
+        std::lock_guard<std::mutex> lk(s->submit_mu);
+        for (int q = 0; q < SERVE_QUEUES; q++)
+            __atomic_store_n(&s->ctl->tail[q], s->published[q] | SERVE_STOP, __ATOMIC_RELEASE);
     }
     // the loop leaves within microseconds of seeing stop; a loop that does
     // not is reported and its buffers are left allocated (it may still use them)
@@ -431,8 +610,13 @@ extern "C" int gvs_server_close(gvs_server *s)
 }
 
 // Open-loop load generator: request i arrives at a seeded Poisson time
-// (rate_qps) and is submitted as soon as it has arrived and a slot is free;
-// its latency runs from its arrival to the poll that sees it completed.
+// (rate_qps).  As the reference client does with its send and receive
+// threads (networked_FPGA/CPU_programs/CPU_client.cpp:490-620), one thread
+// submits every request that has arrived, in one batch per pass (one
+// publication to the device), as slots allow, and the calling thread
+// collects completions; a request's latency runs from its arrival to the
+// poll that sees it completed.  No other thread may submit to the server
+// meanwhile (request ids map back to query indices by submission order).
 extern "C" int gvs_server_loadgen(gvs_server *s, const float *queries, int64_t nq, int64_t ldq, double rate_qps,
                                   uint64_t seed, const int32_t *ks, const int32_t *efs, int64_t *out_labels,
                                   float *out_dists, double *out_latency_us, double *out_wall_s)
@@ -450,46 +634,125 @@ extern "C" int gvs_server_loadgen(gvs_server *s, const float *queries, int64_t n
         t += gap(rng);
         arrival[(size_t)i] = t;
     }
-    std::vector<int64_t> req_of((size_t)nq, -1);
-    std::vector<int64_t> index_of_req;
-    index_of_req.reserve((size_t)nq);
-    const int batch = 256;
+    int64_t req_base;
+    {
+        std::lock_guard<std::mutex> lk(s->submit_mu);
+        std::lock_guard<std::mutex> lk2(s->mu);
+        req_base = s->next_req;
+    }
+    const double t0 = now_us();
+    std::vector<double> t_arr((size_t)nq);
+    for (int64_t i = 0; i < nq; i++) t_arr[(size_t)i] = t0 + arrival[(size_t)i];
+    std::atomic<int> sub_err{0};
+    std::atomic<bool> stop{false};
+    std::atomic<int64_t> submitted{0};
+    std::string sub_msg;
+    double sub_us = 0.0, poll_us = 0.0;  // time inside the submit and poll calls (lock included)
+    int64_t sub_batches = 0, poll_calls = 0;
+    // submit thread: everything that has arrived, in one batch per pass
+    std::thread submitter([&] {
+        std::vector<int64_t> ids(4096);
+        int64_t next = 0;
+        while (next < nq && !stop.load(std::memory_order_relaxed)) {
+            const double now = now_us() - t0;
+            int64_t m = 0;
+            while (next + m < nq && m < (int64_t)ids.size() && arrival[(size_t)(next + m)] <= now) m++;
+            if (m == 0) {
+                std::this_thread::yield();
+                continue;
+            }
+            int64_t got;
+            const double a0 = now_us();
+            {
+                std::lock_guard<std::mutex> lk(s->submit_mu);
+                got = submit_batch_impl(s, queries + next * ldq, m, ldq, ks + next, efs + next, ids.data(),
+                                        t_arr.data() + next);
+                if (got < 0) sub_msg = gvs_last_error();
+            }
+            sub_us += now_us() - a0;
+            if (got > 0) sub_batches++;
+            if (got < 0) {
+                sub_err.store((int)got);
+                return;
+            }
+            next += got;
+            submitted.store(next, std::memory_order_relaxed);
+            if (got < m) std::this_thread::yield();  // no free slot: wait for completions
+        }
+    });
+    const int batch = 1024;
     std::vector<int64_t> rid(batch), lab((size_t)batch * K);
     std::vector<float> dd((size_t)batch * K);
     std::vector<double> lat(batch);
-    const double t0 = now_us();
-    int64_t next = 0, done_n = 0;
-    const int64_t req_base = s->next_req;
-    double last_progress = 0.0;
+    int64_t done_n = 0;
+    double last_progress = now_us();
+    int rc = GVS_OK;
     while (done_n < nq) {
-        const double now = now_us() - t0;
-        if (now - last_progress > 30e6) {
-            gvs::set_error("gvs_server_loadgen: no request completed for 30 s (%lld of %lld done)",
-                           (long long)done_n, (long long)nq);
-            return GVS_ERR_STATE;
+        if (sub_err.load()) {
+            rc = sub_err.load();
+            break;
         }
+        int n;
+        const double a0 = now_us();
         {
-            std::lock_guard<std::mutex> lk(s->mu);
-            while (next < nq && arrival[(size_t)next] <= now) {
-                int64_t r = submit_locked(s, queries + next * ldq, ks[next], efs[next], t0 + arrival[(size_t)next]);
-                if (r == GVS_ERR_STATE) break;  // no free slot: retry after completions
-                if (r < 0) return (int)r;
-                req_of[(size_t)next] = r;
-                next++;
+            std::lock_guard<std::mutex> lk(s->poll_mu);
+            n = poll_impl(s, batch, rid.data(), lab.data(), dd.data(), lat.data(), nullptr, nullptr);
+        }
+        if (n > 0) {
+            poll_us += now_us() - a0;
+            poll_calls++;
+        }
+        if (n < 0) {
+            rc = n;
+            break;
+        }
+        for (int j = 0; j < n; j++) {
+            const int64_t i = rid[(size_t)j] - req_base;  // requests are numbered in submission order
+            if (i < 0 || i >= nq) continue;
+            if (out_labels) std::memcpy(out_labels + i * K, lab.data() + (size_t)j * K, 8 * (size_t)K);
+            if (out_dists) std::memcpy(out_dists + i * K, dd.data() + (size_t)j * K, 4 * (size_t)K);
+            if (out_latency_us) out_latency_us[i] = lat[(size_t)j];
+            done_n++;
+        }
+        const double now = now_us();
+        if (n > 0) {
+            last_progress = now;
+        } else {
+            if (now - last_progress > 30e6) {
+                gvs::set_error("gvs_server_loadgen: no request completed for 30 s (%lld of %lld done, %lld submitted)",
+                               (long long)done_n, (long long)nq, (long long)submitted.load());
+                rc = GVS_ERR_STATE;
+                break;
             }
-            int n = poll_locked(s, batch, rid.data(), lab.data(), dd.data(), lat.data(), nullptr);
-            if (n < 0) return n;
-            for (int j = 0; j < n; j++) {
-                const int64_t i = rid[(size_t)j] - req_base;  // requests are numbered in submission order
-                if (i < 0 || i >= nq) continue;
-                if (out_labels) std::memcpy(out_labels + i * K, lab.data() + (size_t)j * K, 8 * (size_t)K);
-                if (out_dists) std::memcpy(out_dists + i * K, dd.data() + (size_t)j * K, 4 * (size_t)K);
-                if (out_latency_us) out_latency_us[i] = lat[(size_t)j];
-                done_n++;
-            }
-            if (n > 0) last_progress = now;
+            std::this_thread::yield();
         }
     }
-    if (out_wall_s) *out_wall_s = (now_us() - t0) / 1e6;
+    stop.store(true);
+    submitter.join();
+    if (rc == GVS_OK && sub_err.load()) rc = sub_err.load();
+    if (rc != GVS_OK) {
+        if (!sub_msg.empty() && sub_err.load() == rc) gvs::set_error("%s", sub_msg.c_str());
+        return rc;
+    }
+    const double wall_us = now_us() - t0;
+    if (out_wall_s) *out_wall_s = wall_us / 1e6;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        const double n = (double)std::max<int64_t>(nq, 1);
+        s->lg_breakdown[0] = 1e3 * sub_us / n;
+        s->lg_breakdown[1] = 1e3 * poll_us / n;
+        s->lg_breakdown[2] = sub_us / wall_us;
+        s->lg_breakdown[3] = poll_us / wall_us;
+        s->lg_breakdown[4] = (double)nq / (double)std::max<int64_t>(sub_batches, 1);
+        s->lg_breakdown[5] = (double)nq / (double)std::max<int64_t>(poll_calls, 1);
+    }
+    return GVS_OK;
+}
+
+extern "C" int gvs_server_loadgen_breakdown(gvs_server *s, double out[6])
+{
+    if (!s || !out) return GVS_ERR_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    for (int i = 0; i < 6; i++) out[i] = s->lg_breakdown[i];
     return GVS_OK;
 }

@@ -98,6 +98,7 @@ struct KParams {
     int keep_upper_stats;      // leave out_stats[.., 2..3] as the descent wrote them
     int key_bits;              // region key: bits per node id (level-3 node above level-2 node)
     int xcd_queues;            // 8 work queues (counters[4..11]), one per XCD segment of the order
+    int out_k_only;            // write only the query's own k results (rows keep stride k above)
 };
 
 __device__ __forceinline__ uint32_t ord_f(float f)

@@ -570,8 +570,10 @@ __device__ __forceinline__ void find_entry(const KParams &P, int64_t qi, const W
 }
 
 
+// Returns true if the query's visited table overflowed (its results are
+// then label -1 / +inf and its status word is 1).
 template <int NCH, int STORE, int METRIC, int MROWS>
-__device__ void search_query_req(const KParams &P, int64_t qi, int64_t slot, unsigned char *lds, int k, int ef,
+__device__ bool search_query_req(const KParams &P, int64_t qi, int64_t slot, unsigned char *lds, int k, int ef,
                                  const float *Q, bool sys_query)
 {
     constexpr int E = StoreTraits<STORE>::E;
@@ -747,7 +749,8 @@ __device__ void search_query_req(const KParams &P, int64_t qi, int64_t slot, uns
     }
 
     // results, closer first (HNSWLIB/hnswlib.h:171-185)
-    for (int i = lane; i < P.k; i += WAVE) {
+    const int nout = P.out_k_only ? min(k, P.k) : P.k;
+    for (int i = lane; i < nout; i += WAVE) {
         const uint64_t key = (!overflow && i < np && i < k) ? pool[i] : EMPTY_KEY;
         int64_t lab = -1;
         float d = __builtin_inff();
@@ -786,6 +789,7 @@ __device__ void search_query_req(const KParams &P, int64_t qi, int64_t slot, uns
         __builtin_amdgcn_s_waitcnt(0);
     }
     wave_sync();
+    return overflow;
 }
 
 template <int NCH, int STORE, int METRIC, int MROWS>
@@ -838,8 +842,12 @@ __device__ __forceinline__ WgLds carve_wg(const KParams &P, unsigned char *base,
 // ctl words
 constexpr int WG_NSEL = 0, WG_EVALS = 1, WG_MODE = 2, WG_STOP = 3;
 
+// One query with its own k and ef (query Q; sys_query: Q is in host memory,
+// read with system-scope loads).  Returns true on every wave if the
+// query's visited table overflowed.
 template <int NCH, int STORE, int METRIC, int WROWS>
-__device__ void search_query_mw(const KParams &P, int64_t qi, int64_t slot, unsigned char *lds)
+__device__ bool search_query_mw_req(const KParams &P, int64_t qi, int64_t slot, unsigned char *lds, int k, int ef,
+                                    const float *Q, bool sys_query)
 {
     constexpr int E = StoreTraits<STORE>::E;
     const int lane = lane_id();
@@ -860,16 +868,17 @@ __device__ void search_query_mw(const KParams &P, int64_t qi, int64_t slot, unsi
     L.selB = G.sel;
     L.nbw = 1u << P.blog2;
 
-    const int k = P.k;
-    const int ef = max(P.ef, k);
-    const float *Q = P.queries + qi * P.ldq;
+    if (ef < k) ef = k;
     float qf[NCH * E];
 #pragma unroll
     for (int i = 0; i < NCH; i++)
 #pragma unroll
         for (int t = 0; t < E; t++) {
             const int idx = (p + 8 * i) * E + t;
-            qf[i * E + t] = idx < P.dim ? Q[idx] : 0.0f;
+            float v = 0.0f;
+            if (idx < P.dim)
+                v = sys_query ? __hip_atomic_load(&Q[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) : Q[idx];
+            qf[i * E + t] = v;
         }
     // every wave runs the (cheap, deterministic) descent on its own lists
     int up_hops = 0, up_evals = 0;
@@ -970,7 +979,8 @@ __device__ void search_query_mw(const KParams &P, int64_t qi, int64_t slot, unsi
     }
 
     if (w == 0) {
-        for (int i = lane; i < P.k; i += WAVE) {
+        const int nout = P.out_k_only ? min(k, P.k) : P.k;
+        for (int i = lane; i < nout; i += WAVE) {
             const uint64_t key = (!overflow && i < np && i < k) ? pool[i] : EMPTY_KEY;
             int64_t lab = -1;
             float d = __builtin_inff();
@@ -1001,6 +1011,14 @@ __device__ void search_query_mw(const KParams &P, int64_t qi, int64_t slot, unsi
         __builtin_amdgcn_s_waitcnt(0);
     }
     wg_sync();
+    return overflow;
+}
+
+template <int NCH, int STORE, int METRIC, int WROWS>
+__device__ __forceinline__ void search_query_mw(const KParams &P, int64_t qi, int64_t slot, unsigned char *lds)
+{
+    search_query_mw_req<NCH, STORE, METRIC, WROWS>(P, qi, slot, lds, P.k, max(P.ef, P.k), P.queries + qi * P.ldq,
+                                                   false);
 }
 
 }  // namespace gvsk

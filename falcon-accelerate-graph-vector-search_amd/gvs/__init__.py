@@ -54,7 +54,7 @@ class Params(C.Structure):
 
 class ServerConfig(C.Structure):
     _fields_ = [(n, C.c_int32) for n in (
-        "max_inflight", "max_k", "max_ef", "queue_capacity", "grid_waves", "mc", "storage", "reserved")]
+        "max_inflight", "max_k", "max_ef", "queue_capacity", "grid_waves", "mc", "storage", "waves_per_request")]
 
 
 def _load():
@@ -94,6 +94,10 @@ def _load():
         "gvs_bruteforce_f64_device": (C.c_int, [P, P, I64, I64, I32, P, P, P]),
         "gvs_server_open": (C.c_int, [P, P, P]),
         "gvs_server_submit": (I64, [P, P, I32, I32]),
+        "gvs_server_submit_batch": (I64, [P, P, I64, I64, P, P, P]),
+        "gvs_server_poll_status": (C.c_int, [P, I32, P, P, P, P, P, P]),
+        "gvs_server_counters": (C.c_int, [P, P, I32]),
+        "gvs_server_loadgen_breakdown": (C.c_int, [P, P]),
         "gvs_server_poll": (C.c_int, [P, I32, P, P, P, P]),
         "gvs_server_cancel": (C.c_int, [P, I64]),
         "gvs_server_poll_stats": (C.c_int, [P, I32, P, P, P, P, P]),
@@ -122,7 +126,8 @@ def exported_symbols():
              "gvs_gen_mixture", "gvs_index_create", "gvs_index_destroy", "gvs_index_device_bytes",
              "gvs_search_device", "gvs_search_host", "gvs_search_device_noalloc", "gvs_search_overflow_count", "gvs_search_counters", "gvs_autotune", "gvs_search_graph_create",
              "gvs_search_graph_launch", "gvs_search_graph_destroy", "gvs_device_copy", "gvs_stats_per_query", "gvs_search_plan", "gvs_hash_murmur2_keylen4_device", "gvs_bruteforce_f64_device", "gvs_server_open",
-             "gvs_server_submit", "gvs_server_poll", "gvs_server_poll_stats", "gvs_server_cancel",
+             "gvs_server_submit", "gvs_server_submit_batch", "gvs_server_poll", "gvs_server_poll_stats",
+             "gvs_server_poll_status", "gvs_server_counters", "gvs_server_loadgen_breakdown", "gvs_server_cancel",
              "gvs_server_stats", "gvs_server_close", "gvs_server_loadgen"]
     return [n for n in names if hasattr(lib, n)]
 
@@ -397,10 +402,11 @@ class SearchGraph:
 class Server:
     """Continuous-batching serving loop over a DeviceIndex (C ABI gvs_server_*)."""
 
-    def __init__(self, dev, max_inflight=1024, max_k=100, max_ef=512, queue_capacity=0, grid_waves=0, mc=1):
+    def __init__(self, dev, max_inflight=1024, max_k=100, max_ef=512, queue_capacity=0, grid_waves=0, mc=1,
+                 waves_per_request=1):
         self.dev = dev
         self.max_k = max_k
-        cfg = ServerConfig(max_inflight, max_k, max_ef, queue_capacity, grid_waves, mc, F32, 0)
+        cfg = ServerConfig(max_inflight, max_k, max_ef, queue_capacity, grid_waves, mc, F32, waves_per_request)
         self._h = C.c_void_p()
         _check(lib.gvs_server_open(dev.handle, C.byref(cfg), C.byref(self._h)))
 
@@ -421,29 +427,46 @@ class Server:
             raise GvsError(int(r), lib.gvs_last_error().decode())
         return r
 
-    def poll(self, max_done=256, with_stats=False):
+    def submit_batch(self, queries, ks, efs):
+        """Admit as many requests as slots allow, in order, published together
+        -> request ids of the admitted prefix."""
+        q = np.ascontiguousarray(queries, dtype=np.float32)
+        ks = np.ascontiguousarray(ks, dtype=np.int32)
+        efs = np.ascontiguousarray(efs, dtype=np.int32)
+        ids = np.empty(q.shape[0], dtype=np.int64)
+        r = lib.gvs_server_submit_batch(self._h, _ptr(q), q.shape[0], q.shape[1], _ptr(ks), _ptr(efs), _ptr(ids))
+        if r < 0:
+            raise GvsError(int(r), lib.gvs_last_error().decode())
+        return ids[:r]
+
+    def poll(self, max_done=256, with_stats=False, with_status=False):
         rid = np.empty(max_done, dtype=np.int64)
         lab = np.empty((max_done, self.max_k), dtype=np.int64)
         dist = np.empty((max_done, self.max_k), dtype=np.float32)
         lat = np.empty(max_done, dtype=np.float64)
-        if with_stats:
-            st = np.empty((max_done, STATS_ROW), dtype=np.int32)
-            n = lib.gvs_server_poll_stats(self._h, max_done, _ptr(rid), _ptr(lab), _ptr(dist), _ptr(lat), _ptr(st))
-        else:
-            st = None
-            n = lib.gvs_server_poll(self._h, max_done, _ptr(rid), _ptr(lab), _ptr(dist), _ptr(lat))
+        st = np.empty((max_done, STATS_ROW), dtype=np.int32) if with_stats else None
+        status = np.empty(max_done, dtype=np.int32) if with_status else None
+        n = lib.gvs_server_poll_status(self._h, max_done, _ptr(rid), _ptr(lab), _ptr(dist), _ptr(lat), _ptr(st),
+                                       _ptr(status))
         if n < 0:
             raise GvsError(n, lib.gvs_last_error().decode())
         out = (rid[:n], lab[:n], dist[:n], lat[:n])
-        return out + (st[:n, :STATS_PER_QUERY],) if with_stats else out
+        if with_stats:
+            out = out + (st[:n, :STATS_PER_QUERY],)
+        if with_status:
+            out = out + (status[:n],)
+        return out
 
     def cancel(self, req):
         return lib.gvs_server_cancel(self._h, req) == GVS_OK
 
     def stats(self):
-        c = np.zeros(6, dtype=np.int64)
-        _check(lib.gvs_server_stats(self._h, _ptr(c)))
-        return dict(zip(("submitted", "admitted", "completed", "cancelled", "rejected", "device_loops"), c.tolist()))
+        c = np.zeros(7, dtype=np.int64)
+        n = lib.gvs_server_counters(self._h, _ptr(c), 7)
+        if n < 0:
+            raise GvsError(n, lib.gvs_last_error().decode())
+        return dict(zip(("submitted", "admitted", "completed", "cancelled", "rejected", "device_loops",
+                         "overflowed"), c.tolist()))
 
     def loadgen(self, queries, rate_qps, ks, efs, seed=1):
         q = np.ascontiguousarray(queries, dtype=np.float32)
@@ -457,6 +480,13 @@ class Server:
         _check(lib.gvs_server_loadgen(self._h, _ptr(q), nq, q.shape[1], float(rate_qps), seed, _ptr(ks), _ptr(efs),
                                       _ptr(lab), _ptr(dist), _ptr(lat), C.byref(wall)))
         return lab, dist, lat, wall.value
+
+    def loadgen_breakdown(self):
+        """Host-side time split of the last loadgen (gvs_server_loadgen_breakdown)."""
+        out = np.zeros(6, dtype=np.float64)
+        _check(lib.gvs_server_loadgen_breakdown(self._h, _ptr(out)))
+        return dict(zip(("submit_ns_per_request", "poll_ns_per_request", "submit_busy", "poll_busy",
+                         "requests_per_batch", "completions_per_poll"), [round(float(v), 3) for v in out]))
 
     def close(self):
         if self._h is not None and self._h.value:

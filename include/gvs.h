@@ -303,7 +303,9 @@ typedef struct gvs_server_config {
                                 * mixes can saturate higher below that */
     int32_t mc;                /* DST candidates per step used for every request */
     int32_t storage;           /* vector copy traversed */
-    int32_t reserved;
+    int32_t waves_per_request; /* 1 (0) = one wave per request; 2, 4, 8 = one workgroup of that many
+                                * waves per request (the intra-query search, DST(mc, 1); lower
+                                * latency for large ef) */
 } gvs_server_config;
 
 GVS_API int gvs_server_open(gvs_index *ix, const gvs_server_config *cfg, gvs_server **out);
@@ -312,6 +314,16 @@ GVS_API int gvs_server_open(gvs_index *ix, const gvs_server_config *cfg, gvs_ser
  * rejected), or GVS_ERR_STATE when all max_inflight device slots are busy
  * (the caller polls and retries). */
 GVS_API int64_t gvs_server_submit(gvs_server *s, const float *query, int32_t k, int32_t ef);
+/* Enqueue up to n requests (host queries, row stride ldq floats, with their
+ * own ks[i] and efs[i]) in order, stopping at the first that finds no free
+ * slot, and publish them to the device together (one release of the ring
+ * tail instead of one per request).  Writes req_ids[0..m) and returns m >= 0,
+ * the number admitted; the caller re-submits the rest after a poll.  Returns
+ * GVS_ERR_ARG when the first request's k or ef is outside the admitted
+ * range (later ones stop the batch there).  The reference client sends its
+ * queries in batches of batch_size (CPU_client.cpp:450-488). */
+GVS_API int64_t gvs_server_submit_batch(gvs_server *s, const float *queries, int64_t n, int64_t ldq,
+                                        const int32_t *ks, const int32_t *efs, int64_t *req_ids);
 /* Copy out up to max_done finished requests: ids[i], labels[i*max_k..],
  * dists[i*max_k..], latency_us[i] (submit -> completion observed on host).
  * Returns the number copied (>= 0). */
@@ -320,10 +332,20 @@ GVS_API int gvs_server_poll(gvs_server *s, int32_t max_done, int64_t *req_ids,
 /* As gvs_server_poll, plus the request's GVS_STATS_PER_QUERY counters. */
 GVS_API int gvs_server_poll_stats(gvs_server *s, int32_t max_done, int64_t *req_ids, int64_t *labels,
                                   float *dists, double *latency_us, int32_t *stats);
+/* As gvs_server_poll_stats (any output may be NULL), plus status[i]: GVS_OK,
+ * or GVS_ERR_OVERFLOW when the request's visited table overflowed (its rows
+ * are then label -1 / +inf; re-issue it, e.g. on gvs_search_host, which
+ * retries with larger tables). */
+GVS_API int gvs_server_poll_status(gvs_server *s, int32_t max_done, int64_t *req_ids, int64_t *labels,
+                                   float *dists, double *latency_us, int32_t *stats, int32_t *status);
 /* Cancel a request that has not completed; returns GVS_OK or GVS_ERR_STATE. */
 GVS_API int gvs_server_cancel(gvs_server *s, int64_t req_id);
 /* counters: submitted, admitted, completed, cancelled, rejected, device_loops */
 GVS_API int gvs_server_stats(gvs_server *s, int64_t counters[6]);
+/* The first n of: submitted, admitted, completed, cancelled, rejected,
+ * device_loops, overflowed (completed with GVS_ERR_OVERFLOW); returns how
+ * many were written. */
+GVS_API int gvs_server_counters(gvs_server *s, int64_t *counters, int32_t n);
 GVS_API int gvs_server_close(gvs_server *s);
 /* Open-loop load generator over an open server: query i (host, stride ldq)
  * arrives at a seeded Poisson time of rate rate_qps with its own k = ks[i]
@@ -332,6 +354,13 @@ GVS_API int gvs_server_close(gvs_server *s);
  * arrival to completion; wall_s = time to drain all nq.  The reference's
  * client measures per-batch latency the same way around its windows
  * (networked_FPGA/CPU_programs/CPU_client.cpp:742-790). */
+/* Host-side breakdown of the last gvs_server_loadgen on this server: [0] ns
+ * per request spent in the submit thread's batched submits (lock
+ * included), [1] ns per request in the collecting thread's polls that
+ * returned completions, [2] and [3] the fraction of the run's wall time each
+ * thread spent there, [4] requests per submitted batch, [5] completions per
+ * productive poll.  No reference counterpart (diagnostics). */
+GVS_API int gvs_server_loadgen_breakdown(gvs_server *s, double out[6]);
 GVS_API int gvs_server_loadgen(gvs_server *s, const float *queries, int64_t nq, int64_t ldq, double rate_qps,
                                uint64_t seed, const int32_t *ks, const int32_t *efs, int64_t *labels,
                                float *dists, double *latency_us, double *wall_s);

@@ -396,13 +396,18 @@ def test_intra_query_waves_match_single_wave_dst(W, mc):
 
 @pytest.mark.parametrize("W", [2, 8])
 def test_intra_query_waves_hbm_spill_ip_and_entries(W):
-    """The intra-query path through the HBM visited table (large ef), a
-    spilling LDS hash, inner product, and given entry points."""
+    """The intra-query path through the HBM visited table (forced at large
+    ef), the shared LDS hash above ef 128 (the default) and forced to spill,
+    inner product, and given entry points."""
     x, g, dev = graph(20000, 128, 16, 100)
     og = orc.OracleGraph(g.arrays())
     q = queries(100, 128, seed=51)
     print(f"case intra hbm W={W}", flush=True)
+    assert dev.plan(100, k=10, ef=200, mc=4, waves_per_query=W)["lds_hash_log2"] == 13
     check_equal(dev, og, q, 10, 200, mc=4, waves_per_query=W)
+    check_equal(dev, og, q, 10, 400, mc=4, waves_per_query=W)
+    check_equal(dev, og, q, 10, 200, mc=4, waves_per_query=W, visited="hbm")
+    check_equal(dev, og, q, 10, 300, mc=4, waves_per_query=W, visited="lds4", visited_log2=10)
     check_equal(dev, og, q, 10, 64, mc=2, waves_per_query=W, visited="lds", visited_log2=10)
     ent = np.random.default_rng(3).integers(0, 20000, size=100).astype(np.int32)
     check_equal(dev, og, q, 10, 64, mc=2, entries=ent, waves_per_query=W)

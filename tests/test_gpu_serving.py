@@ -1,13 +1,17 @@
 """GPU tests of the serving loop (continuous batching over the persistent
 kernel): each request's result equals the same query searched alone with its
-own k and ef, whatever else is in flight; admission, cancellation, rejection
-and the open-loop load generator."""
+own k and ef, whatever else is in flight, and the oracle's search with that
+k and ef; admission, batched submission, cancellation, rejection and the
+open-loop load generator; one wave or one workgroup of waves per request;
+the 1M x 384 inner-product serving shape (k in {1, 10, 100}, ef in
+[32, 512]) at reduced N."""
 import time
 
 import numpy as np
 import pytest
 
 import gvs
+from oracle import oracle as orc
 
 pytestmark = pytest.mark.gpu
 
@@ -18,7 +22,8 @@ def setup():
     if "dev" not in _G:
         x = gvs.gen_mixture(20000, 128, 32, 51, 52)
         g = gvs.HostGraph.build_hnsw(x, M=16, ef_construction=100, nthreads=16)
-        _G.update(x=x, g=g, dev=gvs.DeviceIndex(g), q=gvs.gen_mixture(600, 128, 32, 51, 53))
+        _G.update(x=x, g=g, dev=gvs.DeviceIndex(g), q=gvs.gen_mixture(600, 128, 32, 51, 53),
+                  og=orc.OracleGraph(g.arrays()))
     return _G
 
 
@@ -31,6 +36,18 @@ def drain(srv, n, timeout=60.0, with_stats=False):
             got[int(r)] = tuple(o[j] for o in out[1:])
         assert time.time() - t0 < timeout, f"serving loop did not drain: {len(got)} of {n} done"
     return got
+
+
+def check_against_oracle(og, q, ks, efs, labels, dists, mc=1):
+    """Every served row equals the oracle's search with that request's k and
+    ef (the kernel's summation order, DST(mc, 1)); -1 / +inf beyond k."""
+    for k in np.unique(ks):
+        for ef in np.unique(efs[ks == k]):
+            sel = np.nonzero((ks == k) & (efs == ef))[0]
+            ids, ds, _ = og.search(q[sel], int(k), int(ef), mc=mc, order=orc.WAVE)
+            np.testing.assert_array_equal(labels[sel, :k], ids.astype(np.int64))
+            np.testing.assert_array_equal(dists[sel, :k], ds)
+            assert np.all(labels[sel, k:] == -1) and np.all(np.isinf(dists[sel, k:]))
 
 
 def test_mixed_requests_equal_alone():
@@ -59,6 +76,8 @@ def test_mixed_requests_equal_alone():
     st = srv.stats()
     srv.close()
     assert st["completed"] == len(q) and st["submitted"] == len(q)
+    L = np.full((len(q), 100), -1, dtype=np.int64)
+    D = np.full((len(q), 100), np.inf, dtype=np.float32)
     for r, i in ids:
         lab, dist, cnt = got[r]
         k, ef = int(ks[i]), int(efs[i])
@@ -67,6 +86,8 @@ def test_mixed_requests_equal_alone():
         np.testing.assert_array_equal(dist[:k], adist[0])
         np.testing.assert_array_equal(cnt, ast[0])
         assert np.all(lab[k:] == -1)
+        L[i], D[i] = lab, dist
+    check_against_oracle(s["og"], q, ks, efs, L, D)
 
 
 def test_reject_cancel_and_counters():
@@ -113,5 +134,95 @@ def test_loadgen_poisson_matches_alone():
     for i in range(len(q)):
         k, ef = int(ks[i]), int(efs[i])
         alone, adist = dev.search(q[i:i + 1], k=k, ef=ef)
+        np.testing.assert_array_equal(lab[i, :k], alone[0])
+        np.testing.assert_array_equal(dist[i, :k], adist[0])
+    check_against_oracle(s["og"], q, ks, efs, lab, dist)
+
+
+def test_submit_batch_admits_a_prefix_and_publishes_it():
+    """gvs_server_submit_batch admits requests in order up to the free slots;
+    the rest are submitted after completions; every result equals the
+    oracle's; counters add up."""
+    s = setup()
+    dev, q = s["dev"], s["q"][:300]
+    ks = np.full(300, 10, np.int32)
+    efs = np.full(300, 64, np.int32)
+    srv = gvs.Server(dev, max_inflight=64, max_k=10, max_ef=64)
+    ids = srv.submit_batch(q, ks, efs)
+    assert 0 < len(ids) <= 64
+    idx = {int(r): i for i, r in enumerate(ids)}
+    nxt = len(ids)
+    got = {}
+    t0 = time.time()
+    while len(got) < 300:
+        if nxt < 300:
+            more = srv.submit_batch(q[nxt:], ks[nxt:], efs[nxt:])
+            for j, r in enumerate(more):
+                idx[int(r)] = nxt + j
+            nxt += len(more)
+        out = srv.poll(256, with_status=True)
+        for j, r in enumerate(out[0]):
+            assert out[4][j] == 0
+            got[idx[int(r)]] = (out[1][j], out[2][j])
+        assert time.time() - t0 < 60, f"{len(got)} of 300 done"
+    st = srv.stats()
+    srv.close()
+    assert st["submitted"] == 300 and st["completed"] == 300 and st["overflowed"] == 0
+    L = np.stack([got[i][0] for i in range(300)])
+    D = np.stack([got[i][1] for i in range(300)])
+    check_against_oracle(s["og"], q, ks, efs, L, D)
+    with pytest.raises(gvs.GvsError):
+        srv2 = gvs.Server(dev, max_inflight=8, max_k=10, max_ef=64)
+        try:
+            srv2.submit_batch(q[:2], np.array([10, 10], np.int32), np.array([65, 64], np.int32))
+        finally:
+            srv2.close()
+
+
+@pytest.mark.parametrize("W", [2, 4])
+def test_workgroup_per_request_server(W):
+    """waves_per_request = W: one workgroup of W waves per request (DST(mc, 1)
+    with mc = W): results equal the batch path's intra-query search alone
+    and the oracle's DST(mc, 1)."""
+    s = setup()
+    dev, q = s["dev"], s["q"]
+    rng = np.random.default_rng(7)
+    ks = rng.choice([1, 10, 100], size=len(q)).astype(np.int32)
+    efs = rng.integers(32, 257, size=len(q)).astype(np.int32)
+    srv = gvs.Server(dev, max_inflight=256, max_k=100, max_ef=256, mc=W, waves_per_request=W)
+    print(f"case serving W={W}", flush=True)
+    lab, dist, lat, wall = srv.loadgen(q, 20000.0, ks, efs, seed=4)
+    srv.close()
+    for i in range(0, len(q), 7):
+        k, ef = int(ks[i]), int(efs[i])
+        alone, adist = dev.search(q[i:i + 1], k=k, ef=ef, mc=W, waves_per_query=W)
+        np.testing.assert_array_equal(lab[i, :k], alone[0])
+        np.testing.assert_array_equal(dist[i, :k], adist[0])
+    check_against_oracle(s["og"], q, ks, efs, lab, dist, mc=W)
+
+
+@pytest.mark.parametrize("W", [1, 4])
+def test_inner_product_d384_mixed_k_and_ef(W):
+    """The 1M x 384 unit-norm inner-product serving shape at 30k points:
+    k in {1, 10, 100}, ef uniform in [32, 512], a server admitting up to
+    ef 512; every result equals the oracle's and the query searched alone."""
+    if "ip" not in _G:
+        x = gvs.gen_mixture(30000, 384, 64, 57, 58, normalize=True)
+        g = gvs.HostGraph.build_hnsw(x, M=32, ef_construction=200, metric=gvs.IP, nthreads=16)
+        _G["ip"] = dict(dev=gvs.DeviceIndex(g), og=orc.OracleGraph(g.arrays()),
+                        q=gvs.gen_mixture(400, 384, 64, 57, 59, normalize=True))
+    c = _G["ip"]
+    rng = np.random.default_rng(11)
+    ks = rng.choice([1, 10, 100], size=400).astype(np.int32)
+    efs = rng.integers(32, 513, size=400).astype(np.int32)
+    mc = 1 if W == 1 else 4
+    srv = gvs.Server(c["dev"], max_inflight=512, max_k=100, max_ef=512, mc=mc, waves_per_request=W)
+    print(f"case serving IP d384 W={W}", flush=True)
+    lab, dist, lat, wall = srv.loadgen(c["q"], 5000.0, ks, efs, seed=6)
+    srv.close()
+    check_against_oracle(c["og"], c["q"], ks, efs, lab, dist, mc=mc)
+    for i in range(0, 400, 13):
+        k, ef = int(ks[i]), int(efs[i])
+        alone, adist = c["dev"].search(c["q"][i:i + 1], k=k, ef=ef, mc=mc, waves_per_query=W)
         np.testing.assert_array_equal(lab[i, :k], alone[0])
         np.testing.assert_array_equal(dist[i, :k], adist[0])
