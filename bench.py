@@ -269,6 +269,13 @@ def main():
                     help="instead of the throughput run: NSG-style flat graph (out-degree 64, navigating-node "
                          "entry) on 10M x 128 (1024 centres), fp32 / BF16 / FP8-e4m3 traversal with and without "
                          "exact fp32 rerank, ef=128, k=10: recall and distance error against FP64")
+    ap.add_argument("--serve-only", action="store_true",
+                    help="instead of the throughput run: only the serving leg (saturated rate, then Poisson load "
+                         "at 25/50/75/90%% of it), e.g. under a profiler; with --c5, one configuration")
+    ap.add_argument("--serve-grid", type=int, default=-1, help="serving grid waves for --serve-only (-1 = default)")
+    ap.add_argument("--serve-w", type=int, default=1, help="waves per request for --serve-only")
+    ap.add_argument("--serve-mc", type=int, default=1, help="DST candidates per step for --serve-only")
+    ap.add_argument("--serve-requests", type=int, default=100000, help="requests per load level for --serve-only")
     ap.add_argument("--serve-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--serve-ef", type=int, default=64, help=argparse.SUPPRESS)
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "k_search_1Mx128_hbm_traffic.txt"),
@@ -312,12 +319,23 @@ def main():
     if a.c4:
         precision_c4(a, gvs, g, x, q)
         return
+    if a.serve_only and not a.c5:
+        if a.graph_cache and not os.path.exists(a.graph_cache):
+            g.save_hnswlib(a.graph_cache)
+        serve_child(a, gvs, g, q, nreq=a.serve_requests, grid_waves=max(a.serve_grid, 0),
+                    waves_per_request=a.serve_w, mc=a.serve_mc)
+        return
     if a.c5:
         if a.graph_cache and not os.path.exists(a.graph_cache):
             g.save_hnswlib(a.graph_cache)
         rng = np.random.default_rng(5)
         ks = rng.choice([1, 10, 100], size=len(q)).astype(np.int32)
         efs = rng.integers(32, 513, size=len(q)).astype(np.int32)
+        if a.serve_only:
+            serve_child(a, gvs, g, q, ks=ks, efs=efs, metric_name="inner product, unit-norm", max_inflight=4096,
+                        grid_waves=max(a.serve_grid, 0), nreq=a.serve_requests, waves_per_request=a.serve_w,
+                        mc=a.serve_mc)
+            return
         # one wave per request (throughput), then one workgroup of 8 waves
         # running DST(8, 1) per request (latency of the heavy requests)
         serve_child(a, gvs, g, q, ks=ks, efs=efs, metric_name="inner product, unit-norm", max_inflight=4096,
