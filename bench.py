@@ -61,6 +61,85 @@ def read_traffic(path, cfg):
     return None
 
 
+def host_cpu():
+    """Host CPU model, logical CPU count and the CPUs this process may use."""
+    model = "unknown"
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                model = ln.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    allowed = sorted(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else list(range(os.cpu_count() or 1))
+    return model, os.cpu_count() or 1, allowed
+
+
+def cpu_baseline(a, g, q, gt, ef, index_path):
+    """The reference's own hnswlib searchKnn (oracle/_ref/ref_hnsw_simd:
+    compiled from the reference's headers with its SIMD distance kernels,
+    -O3 -march=x86-64-v3, OpenMP across queries), pinned with taskset to this
+    process's host CPUs (at most 16, the GPU's share of the host), on the
+    same index and queries as the GPU run, for about cpu_seconds; plus batch=1
+    latency on one thread.  Where that binary is not built (no reference
+    tree where the library was built), the hnswlib-order oracle stands in
+    (kind "port")."""
+    import subprocess
+    import tempfile
+    model, nproc, allowed = host_cpu()
+    cpus = allowed[:16]
+    threads = len(cpus)
+    ref = os.path.join(ROOT, "oracle", "_ref", "ref_hnsw_simd")
+    if os.path.exists(ref) and os.path.exists(index_path):
+        with tempfile.TemporaryDirectory() as td:
+            qf, idf = os.path.join(td, "q.f32"), os.path.join(td, "ids.i32")
+            np.ascontiguousarray(q, dtype=np.float32).tofile(qf)
+            cmd = ["taskset", "-c", ",".join(str(c) for c in cpus), ref, "bench", index_path, str(a.dim), "l2", qf,
+                   str(len(q)), str(a.k), str(ef), str(threads), str(a.cpu_seconds), idf]
+            r = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
+            if r.returncode == 0:
+                res = json.loads(r.stdout.strip().splitlines()[-1])
+                ids = np.fromfile(idf, dtype=np.int32).reshape(len(q), a.k)
+                cpu = {"value": res["qps"], "unit": "QPS", "cores": threads, "kind": "reference",
+                       "sample": f"{res['passes']} pass(es) over the {len(q)} queries ({res['seconds']} s), ef={ef}, "
+                                 f"k={a.k}: the reference's hnswlib searchKnn (oracle/_ref/ref_hnsw_simd, SIMD "
+                                 f"-O3 -march=x86-64-v3, OpenMP over queries) on the GPU run's index, taskset to "
+                                 f"{threads} host CPUs",
+                       "recall_at_10": round(recall_at(ids, gt, a.k), 4), "cpu_model": model, "nproc": nproc,
+                       "batch1_latency_us": {"p50": res["batch1_p50_us"], "p99": res["batch1_p99_us"],
+                                             "queries": res["batch1_queries"], "cores": 1}}
+                log(f"[bench] CPU reference hnswlib {cpu['value']} QPS on {threads} threads ({model}, nproc {nproc}); "
+                    f"batch=1 p50 {res['batch1_p50_us']} us")
+                return cpu
+            log(f"[bench] reference CPU baseline failed ({r.returncode}): {r.stderr[-300:]}; timing the oracle")
+    from oracle import oracle as orc
+    og = orc.OracleGraph(g.arrays())
+    probe = 200
+    t0 = time.perf_counter()
+    og.search(q[:probe], a.k, ef, order=orc.SEQ, nthreads=threads)
+    per_q = (time.perf_counter() - t0) / probe
+    reps = max(1, int(a.cpu_seconds / max(per_q * len(q), 1e-6)))
+    ns = int(min(len(q), max(probe, a.cpu_seconds / max(per_q, 1e-7))))
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ids, _, _ = og.search(q[:ns], a.k, ef, order=orc.SEQ, nthreads=threads)
+    cs = time.perf_counter() - t0
+    cpu = {"value": round(reps * ns / cs, 1), "unit": "QPS", "cores": threads, "kind": "port",
+           "sample": f"{reps} pass(es) over the first {ns} of the {len(q)} queries ({cs:.1f} s), ef={ef}, k={a.k}, "
+                     f"hnswlib-order oracle (oracle/oracle.c searchKnn), OpenMP over queries on {threads} host threads",
+           "recall_at_10": round(recall_at(ids, gt[:ns], a.k), 4), "cpu_model": model, "nproc": nproc}
+    lat = []
+    for i in range(502):
+        t0 = time.perf_counter()
+        og.search(q[i:i + 1], a.k, ef, order=orc.SEQ, nthreads=1)
+        lat.append(time.perf_counter() - t0)
+    lat = np.array(lat[1:-1]) * 1e6
+    cpu["batch1_latency_us"] = {"p50": round(float(np.percentile(lat, 50)), 1),
+                                "p99": round(float(np.percentile(lat, 99)), 1), "queries": 500, "cores": 1}
+    log(f"[bench] CPU oracle {cpu['value']} QPS on {threads} threads")
+    return cpu
+
+
 def serving_leg(a, cache, ef):
     """Serving latency in a child process (bounded by a timeout, so that the
     batch figures above are reported whatever happens to the serving loop)."""
@@ -525,46 +604,15 @@ def main():
     serving = None
     if not a.no_serving:
         serving = serving_leg(a, cache, plain_ef)
-    if not a.graph_cache and os.path.exists(cache):
-        os.remove(cache)
-
     traffic_hbm = read_traffic(a.traffic_file, tuned_cfg)
     if traffic_hbm is None:
         log(f"[bench] no PMC traffic summary for {tuned_cfg} in {a.traffic_file}")
 
     cpu = None
     if not a.no_cpu:
-        from oracle import oracle as orc
-        og = orc.OracleGraph(g.arrays())
-        threads = min(16, os.cpu_count() or 1)
-        probe = 200
-        t0 = time.perf_counter()
-        og.search(q[:probe], a.k, plain_ef, order=orc.SEQ, nthreads=threads)
-        per_q = (time.perf_counter() - t0) / probe
-        # a bounded sample of about cpu_seconds: the query batch, repeated as needed
-        reps = max(1, int(a.cpu_seconds / max(per_q * a.nq, 1e-6)))
-        ns = int(min(a.nq, max(probe, a.cpu_seconds / max(per_q, 1e-7))))
-        t0 = time.perf_counter()
-        for _ in range(reps):
-            ids, _, _ = og.search(q[:ns], a.k, plain_ef, order=orc.SEQ, nthreads=threads)
-        cs = time.perf_counter() - t0
-        cpu = {"value": round(reps * ns / cs, 1), "unit": "QPS", "cores": threads, "kind": "port",
-               "sample": f"{reps} pass(es) over the first {ns} of the {a.nq} queries ({cs:.1f} s), ef={plain_ef}, "
-                         f"k={a.k}, hnswlib-order oracle (oracle/oracle.c searchKnn), OpenMP over queries on "
-                         f"{threads} host threads",
-               "recall_at_10": round(recall_at(ids, gt[:ns], a.k), 4)}
-        log(f"[bench] CPU oracle {cpu['value']} QPS on {threads} threads ({ns} queries, {cs:.1f}s)")
-        # batch=1 latency, one host thread, one query per call (the reference's
-        # CPU latency mode: run_all_hnsw_search.py:104-127, first and last dropped)
-        lat = []
-        for i in range(502):
-            t0 = time.perf_counter()
-            og.search(q[i:i + 1], a.k, plain_ef, order=orc.SEQ, nthreads=1)
-            lat.append(time.perf_counter() - t0)
-        lat = np.array(lat[1:-1]) * 1e6
-        cpu["batch1_latency_us"] = {"p50": round(float(np.percentile(lat, 50)), 1),
-                                    "p99": round(float(np.percentile(lat, 99)), 1), "queries": 500, "cores": 1}
-        log(f"[bench] CPU oracle batch=1 latency p50 {cpu['batch1_latency_us']['p50']} us")
+        cpu = cpu_baseline(a, g, q, gt, plain_ef, cache)
+    if not a.graph_cache and os.path.exists(cache):
+        os.remove(cache)
 
     out = {
         "metric": METRIC, "value": round(qps, 1), "unit": "QPS", "n_gpus": 1, "steps": a.steps,
