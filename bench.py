@@ -140,6 +140,45 @@ def cpu_baseline(a, g, q, gt, ef, index_path):
     return cpu
 
 
+FP64_MFMA_PEAK_TFLOPS = 78.6
+
+
+def gt_only(a, gvs):
+    """Time gvs_bruteforce_f64_device on nq x n x dim (the graph is a ring,
+    unused by the brute force): 1 warm-up and 3 timed calls with HIP events;
+    FLOP = 2 * nq * n * dim."""
+    from gvs import hip as H
+    x = gvs.gen_mixture(a.n, a.dim, a.centres, 1234, 1235, nthreads=a.build_threads)
+    q = gvs.gen_mixture(a.nq, a.dim, a.centres, 1234, 98765, nthreads=a.build_threads)
+    ring = ((np.arange(a.n, dtype=np.int64) + 1) % a.n).astype(np.int32).reshape(a.n, 1)
+
+    class Ring:
+        view = gvs.Graph()
+    v = Ring.view
+    v.n, v.dim, v.metric, v.vectors, v.ld_vectors = a.n, a.dim, gvs.L2, x.ctypes.data, a.dim
+    v.r0, v.adj0, v.max_level, v.entry, v.ru, v.up_rows = 1, ring.ctypes.data, 0, 0, 0, 0
+    dev = gvs.DeviceIndex(Ring)
+    st = H.Stream()
+    dq = H.DeviceArray.from_host(q)
+    ids = H.DeviceArray((a.nq, a.k), np.int32)
+    ds = H.DeviceArray((a.nq, a.k), np.float64)
+    dev.bruteforce_f64_device(dq.address, a.nq, a.dim, a.k, ids.address, ds.address, st.handle)
+    st.synchronize()
+    e0, e1 = H.Event(), H.Event()
+    e0.record(st)
+    for _ in range(3):
+        dev.bruteforce_f64_device(dq.address, a.nq, a.dim, a.k, ids.address, ds.address, st.handle)
+    e1.record(st)
+    e1.synchronize()
+    ms = e0.elapsed_ms(e1) / 3
+    tflops = 2.0 * a.nq * a.n * a.dim / (ms / 1e3) / 1e12
+    print(json.dumps({"metric": "FP64 brute-force kNN (MFMA) time", "ms": round(ms, 3), "tflops": round(tflops, 2),
+                      "peak_tflops": FP64_MFMA_PEAK_TFLOPS, "frac": round(tflops / FP64_MFMA_PEAK_TFLOPS, 4),
+                      "config": {"workload": f"{a.nq} queries x {a.n} x {a.dim} fp32 (FP64 arithmetic), k={a.k}"}}),
+          flush=True)
+    del Ring, dev
+
+
 def serving_leg(a, cache, ef):
     """Serving latency in a child process (bounded by a timeout, so that the
     batch figures above are reported whatever happens to the serving loop)."""
@@ -355,6 +394,9 @@ def main():
     ap.add_argument("--serve-w", type=int, default=1, help="waves per request for --serve-only")
     ap.add_argument("--serve-mc", type=int, default=1, help="DST candidates per step for --serve-only")
     ap.add_argument("--serve-requests", type=int, default=100000, help="requests per load level for --serve-only")
+    ap.add_argument("--gt-only", action="store_true",
+                    help="instead of the throughput run: time the FP64 MFMA brute-force ground truth "
+                         "(nq queries x n base vectors x dim) and report TFLOP/s against the FP64 matrix peak")
     ap.add_argument("--serve-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--serve-ef", type=int, default=64, help=argparse.SUPPRESS)
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "k_search_1Mx128_hbm_traffic.txt"),
@@ -371,6 +413,9 @@ def main():
     import gvs
     from gvs import hip as H
 
+    if a.gt_only:
+        gt_only(a, gvs)
+        return
     t0 = time.time()
     metric = gvs.L2
     if a.c4:
