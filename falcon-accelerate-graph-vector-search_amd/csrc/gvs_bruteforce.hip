@@ -218,7 +218,12 @@ constexpr int KSTEP = 4 * TILE + 2;
 #ifndef GVS_BF_PREFETCH
 #define GVS_BF_PREFETCH 2  // k-steps of B operands loaded ahead of their products
 #endif
-template <int NK>
+//
+// Top-k lists: with REGK (k <= 16) each row's list lives in registers of the
+// 16 lanes that hold the row in D (lane ai keeps entry ai), and a survivor
+// is inserted by shuffles within the group, the four groups in parallel;
+// otherwise each row's list is in LDS and survivors go in one at a time.
+template <int NK, bool REGK>
 __global__ __launch_bounds__(256, 2) void k_bf64_v2(const float *X, const double *xn, int64_t n, int dim,
                                                  const float *Q, int64_t ldq, int64_t nq, int k, int metric,
                                                  int64_t chunk, double *part_d, int32_t *part_i)
@@ -242,9 +247,11 @@ __global__ __launch_bounds__(256, 2) void k_bf64_v2(const float *X, const double
     const int64_t q0 = ((int64_t)blockIdx.x * nw + w) * 16;
     const int64_t c0 = (int64_t)blockIdx.y * chunk;
     const int64_t c1 = min(n, c0 + chunk);
-    for (int i = lane; i < 16 * k; i += WAVE) {
-        ld[i] = __builtin_inf();
-        li[i] = 0x7FFFFFFF;
+    if (!REGK) {
+        for (int i = lane; i < 16 * k; i += WAVE) {
+            ld[i] = __builtin_inf();
+            li[i] = 0x7FFFFFFF;
+        }
     }
     const int ai = lane & 15, ak = lane >> 4;
     // query operands, fp64, for the whole chunk
@@ -259,16 +266,23 @@ __global__ __launch_bounds__(256, 2) void k_bf64_v2(const float *X, const double
             areg[kk] = (col < dim && qa < nq) ? qrow[col] : 0.0f;
         }
     }
+    // D of v_mfma_f64_16x16x4f64 on gfx950: element r of lane l is
+    // D[l / 16 + 4 r][l % 16] (the probe in k_bf64 reads the same from the
+    // instruction), so a 16-lane group holds one row per r, all 16 columns
     int rowid[4], colid[4];
-    {
-        const double4_t z = (double4_t){0.0, 0.0, 0.0, 0.0};
-        double4_t pr = __builtin_amdgcn_mfma_f64_16x16x4f64(ak == 0 ? (double)ai : 0.0, ak == 0 ? 1.0 : 0.0, z, 0, 0, 0);
-        double4_t pc = __builtin_amdgcn_mfma_f64_16x16x4f64(ak == 0 ? 1.0 : 0.0, ak == 0 ? (double)ai : 0.0, z, 0, 0, 0);
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-            rowid[r] = min(15, max(0, (int)pr[r]));
-            colid[r] = min(15, max(0, (int)pc[r]));
-        }
+    for (int r = 0; r < 4; r++) {
+        rowid[r] = ak + 4 * r;
+        colid[r] = ai;
+    }
+    // REGK lists: entry ai of row ak + 4 r in rl_d[r] / rl_i[r]; thr_* is
+    // entry k - 1 (the bound a candidate must beat), kept in every lane
+    double rl_d[4], thr_d[4];
+    int rl_i[4], thr_i[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        rl_d[r] = thr_d[r] = __builtin_inf();
+        rl_i[r] = thr_i[r] = 0x7FFFFFFF;
     }
     double qn[4];
 #pragma unroll
@@ -335,10 +349,44 @@ __global__ __launch_bounds__(256, 2) void k_bf64_v2(const float *X, const double
                     acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, (double)bq[kk % (PF + 1)][c], acc[c], 0, 0, 0);
                 __builtin_amdgcn_sched_barrier(0);
             }
+            if (REGK) {
+#pragma unroll
+                for (int c = 0; c < 4; c++) {
+                    const int j = 16 * c + ai;
+                    const int64_t id64 = t0 + j;
+                    const bool valid = id64 < c1;
+                    const int id = (int)id64;
+                    const double xnj = xs_n0[buf * TILE + j];
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        const double dot = acc[c][r];
+                        const double d = (metric == 0) ? (qn[r] + xnj - 2.0 * dot) : (1.0 - dot);
+                        bool sv = valid && dlt(d, id, thr_d[r], thr_i[r]);
+                        uint64_t bal = __ballot(sv);
+                        while (bal) {
+                            // each group takes its lowest survivor
+                            const uint32_t m = (uint32_t)(bal >> (16 * ak)) & 0xFFFFu;
+                            const int src = m ? 16 * ak + __ffs((int)m) - 1 : lane;
+                            const double nd = __shfl(d, src);
+                            const int nid = __shfl(id, src);
+                            const bool lt = ai < k && dlt(rl_d[r], rl_i[r], nd, nid);
+                            const int pos = __popcll((__ballot(lt) >> (16 * ak)) & 0xFFFFull);
+                            const double pd = __shfl(rl_d[r], max(lane - 1, 0));
+                            const int pi = __shfl(rl_i[r], max(lane - 1, 0));
+                            if (m && ai >= pos) {
+                                rl_d[r] = ai == pos ? nd : pd;
+                                rl_i[r] = ai == pos ? nid : pi;
+                            }
+                            thr_d[r] = __shfl(rl_d[r], 16 * ak + k - 1);
+                            thr_i[r] = __shfl(rl_i[r], 16 * ak + k - 1);
+                            sv = sv && lane != src && dlt(d, id, thr_d[r], thr_i[r]);
+                            bal = __ballot(sv);
+                        }
+                    }
+                }
+            } else {
             // the rows' current k-th entries (a list only tightens, so a
             // stale bound lets a few more candidates into the exact check)
-            double thr_d[4];
-            int thr_i[4];
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 thr_d[r] = ld[rowid[r] * k + k - 1];
@@ -406,11 +454,22 @@ __global__ __launch_bounds__(256, 2) void k_bf64_v2(const float *X, const double
                     }
                 }
             }
+            }
         }
         if (more) store_tile(buf ^ 1);  // nobody reads buf ^ 1 until after the barrier
         __syncthreads();
     }
-    if (q0 < nq) {
+    if (REGK) {
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int64_t qi = q0 + rowid[r];
+            if (qi < nq && ai < k) {
+                const size_t o = ((size_t)blockIdx.y * nq + qi) * k + ai;
+                part_d[o] = rl_d[r];
+                part_i[o] = rl_i[r] == 0x7FFFFFFF ? -1 : rl_i[r];
+            }
+        }
+    } else if (q0 < nq) {
         for (int row = 0; row < 16; row++) {
             int64_t qi = q0 + row;
             if (qi >= nq) break;
@@ -425,13 +484,15 @@ __global__ __launch_bounds__(256, 2) void k_bf64_v2(const float *X, const double
 
 typedef void (*bf64v2_t)(const float *, const double *, int64_t, int, const float *, int64_t, int64_t, int, int,
                          int64_t, double *, int32_t *);
-static bf64v2_t pick_v2(int dp)
+constexpr int REGK_MAX = 16;  // lists in registers up to this k
+static bf64v2_t pick_v2(int dp, int k)
 {
+    const bool rk = k <= REGK_MAX;
     switch (dp) {
-    case 32: return k_bf64_v2<8>;
-    case 64: return k_bf64_v2<16>;
-    case 96: return k_bf64_v2<24>;
-    case 128: return k_bf64_v2<32>;
+    case 32: return rk ? k_bf64_v2<8, true> : k_bf64_v2<8, false>;
+    case 64: return rk ? k_bf64_v2<16, true> : k_bf64_v2<16, false>;
+    case 96: return rk ? k_bf64_v2<24, true> : k_bf64_v2<24, false>;
+    case 128: return rk ? k_bf64_v2<32, true> : k_bf64_v2<32, false>;
     default: return nullptr;
     }
 }
@@ -502,18 +563,19 @@ extern "C" int gvs_bruteforce_f64_device(const gvs_index *cix, const float *quer
     const size_t per_wave = (size_t)16 * k * 12;
     // v2 (register-resident queries, double-buffered tiles) where the padded
     // dim is compiled in and four waves' lists fit beside two tiles
-    bf64v2_t v2 = pick_v2(dp);
+    bf64v2_t v2 = pick_v2(dp, k);
     const int xs_v2 = KSTEP * (dp / 4) / TILE + 1;  // as the kernel's XS
     const size_t xs2 = (((size_t)2 * TILE * xs_v2 * 4 + 15) & ~(size_t)15) + (size_t)2 * TILE * 8;
-    if (v2 && xs2 + 4 * per_wave > 160 * 1024) v2 = nullptr;
+    const size_t v2_lists = k <= REGK_MAX ? 0 : 4 * per_wave;  // register lists need no LDS
+    if (v2 && xs2 + v2_lists > 160 * 1024) v2 = nullptr;
     const size_t xs_bytes = v2 ? xs2 : (((size_t)TILE * (dp + 1) * 4 + 15) & ~(size_t)15);
     int wpb = 4;
     while (!v2 && wpb > 1 && xs_bytes + wpb * per_wave > 160 * 1024) wpb--;
-    if (xs_bytes + wpb * per_wave > 160 * 1024) {
+    const size_t shm = v2 ? xs2 + v2_lists : xs_bytes + wpb * per_wave;
+    if (shm > 160 * 1024) {
         gvs::set_error("gvs_bruteforce_f64_device: k=%d too large for LDS at dim %d", k, ix->dim);
         return GVS_ERR_UNSUPPORTED;
     }
-    const size_t shm = xs_bytes + wpb * per_wave;
     const void *kfn = v2 ? (const void *)v2 : (const void *)k_bf64;
     if (shm > 65536 && !gvs::hip_ok(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm),
                                     "hipFuncSetAttribute"))

@@ -548,3 +548,46 @@ def test_fp8_decode_every_code():
     q = np.random.default_rng(4).normal(0, 30, size=(8, 128)).astype(np.float32)
     print("case fp8 every code", flush=True)
     check_equal(dev, og, q, 2, 2, storage=gvs.FP8)
+
+
+def ring_index(x, metric=gvs.L2):
+    n, dim = x.shape
+    a = dict(n=n, dim=dim, metric=metric, vectors=x, r0=1, entry=0,
+             adj0=((np.arange(n, dtype=np.int64) + 1) % n).astype(np.int32).reshape(n, 1))
+    keep = ArrayGraph(a)
+    return keep, gvs.DeviceIndex(keep)
+
+
+@pytest.mark.parametrize("dim", [20, 64, 96, 128, 160])
+@pytest.mark.parametrize("k", [1, 10, 16, 17, 100])
+def test_bruteforce_dims_k_ragged_and_ties(dim, k):
+    """Every padded width (32/64/96/128 on the register-resident kernel, 160
+    on the general one), lists in registers (k <= 16) and in LDS (k > 16), a
+    base set and a query count that are not multiples of the tile sizes, and
+    exact ties (every vector appears twice): results equal the oracle's FP64
+    scan and tied distances come in ascending id order."""
+    from gvs import hip as H
+    rng = np.random.default_rng(dim * 1000 + k)
+    half = rng.standard_normal((2600, dim)).astype(np.float32)
+    x = np.concatenate([half, half[:2403]])          # n = 5003, ids i and 2600 + i tie
+    q = rng.standard_normal((37, dim)).astype(np.float32)
+    metric = gvs.IP if k == 17 else gvs.L2
+    keep, dev = ring_index(x, metric)
+    qt = H.DeviceArray.from_host(q)
+    ids = H.DeviceArray((37, k), np.int32)
+    ds = H.DeviceArray((37, k), np.float64)
+    print(f"case bruteforce dim={dim} k={k} metric={metric}", flush=True)
+    dev.bruteforce_f64_device(qt.address, 37, dim, k, ids.address, ds.address)
+    H.synchronize()
+    gi, gd = ids.to_host(), ds.to_host()
+    oi, od = orc.bruteforce_f64(x, q, k, metric=metric)
+    np.testing.assert_allclose(gd, od, rtol=1e-9, atol=1e-9)
+    assert np.all(np.diff(gd, axis=1) >= 0)
+    tie = np.diff(gd, axis=1) == 0
+    assert np.all(np.diff(gi, axis=1)[tie] > 0)
+    mism = gi != oi
+    for r, c in zip(*np.nonzero(mism)):
+        assert abs(gd[r, c] - od[r, c]) <= 1e-9 * max(1.0, abs(od[r, c]))
+    assert mism.mean() < 0.02
+    if k >= 10:
+        assert tie.sum() >= 37 * (k // 4)  # the twins do tie exactly
