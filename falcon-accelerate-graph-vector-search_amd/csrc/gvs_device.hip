@@ -358,7 +358,7 @@ namespace gvs {
 // block) and the resident wave count, for lp's tables.
 static int fit_waves(const gvs_index *ix, const void *fn, LaunchPlan *lp)
 {
-    int bytes = 16 * lp->ef_cap + 8 * lp->lcap + 8 * (1 << lp->blog2) + 4 * lp->lcap + 64 +
+    int bytes = 16 * lp->ef_cap + 8 * lp->lcap + 8 * (1 << lp->blog2) + 4 * lp->lcap + 4 * gvsk::gvs_sel_words(lp->mg) +
                 (lp->hlog2 > 0 ? (lp->hnarrow ? 2 : 4) * (1 << lp->hlog2) : 0);
     lp->lds_per_wave = (bytes + 15) / 16 * 16;
     lp->wpb = 1;
@@ -394,7 +394,7 @@ int plan_search(const gvs_index *ix, const gvs_params *p, int max_ef, int max_k,
         return GVS_ERR_ARG;
     }
     if (max_k < 1 || max_k > 4096 || max_ef < 1 || max_ef > 4096 || p->mc < 1 || p->mc > 8 || p->mg < 1 ||
-        p->mg > 2 || (p->waves_per_query != 1 && p->waves_per_query != 2 && p->waves_per_query != 4 &&
+        p->mg > MAX_MG || (p->waves_per_query != 1 && p->waves_per_query != 2 && p->waves_per_query != 4 &&
                       p->waves_per_query != 8)) {
         set_error("gvs_search: unsupported k=%d ef=%d mc=%d mg=%d waves_per_query=%d", max_k, max_ef, p->mc,
                   p->mg, p->waves_per_query);
@@ -414,8 +414,8 @@ int plan_search(const gvs_index *ix, const gvs_params *p, int max_ef, int max_k,
     const int W = p->waves_per_query;
     int wrows = 0;  // adjacency rows per wave when several waves share a query
     if (W > 1) {
-        if (p->storage != GVS_STORE_F32 || p->mg != 1) {
-            set_error("gvs_search: waves_per_query > 1 needs fp32 storage and mg = 1");
+        if (p->storage != GVS_STORE_F32) {
+            set_error("gvs_search: waves_per_query > 1 needs fp32 storage");
             return GVS_ERR_UNSUPPORTED;
         }
         const int rpw = (p->mc * rpc + W - 1) / W;
@@ -439,6 +439,7 @@ int plan_search(const gvs_index *ix, const gvs_params *p, int max_ef, int max_k,
     int efe = std::max(max_ef, max_k);
     lp->fn = (void *)fn;
     lp->dfn = W == 1 ? (void *)pick_descend(p->storage, ix->metric, nch, seq) : nullptr;
+    lp->mg = p->mg;
     lp->ef_cap = (efe + 63) / 64 * 64;
     lp->lcap = p->mc * rpc * WAVE;
     lp->blog2 = p->bloom_log2 > 0 ? p->bloom_log2 : std::min(12, std::max(8, ilog2_ceil((int64_t)efe * 8)));
@@ -491,7 +492,7 @@ int plan_search(const gvs_index *ix, const gvs_params *p, int max_ef, int max_k,
     if (W > 1) {
         // one workgroup per query: shared pool, Bloom words, selection and
         // control words, visited hash; per wave wrows*64 keys and ids
-        int bytes = 16 * lp->ef_cap + 8 * (1 << lp->blog2) + W * wrows * WAVE * 12 + 64 +
+        int bytes = 16 * lp->ef_cap + 8 * (1 << lp->blog2) + W * wrows * WAVE * 12 + 32 * p->mg + 32 +
                     (lp->hlog2 > 0 ? 4 * (1 << lp->hlog2) : 0);
         lp->lds_per_wave = (bytes + 15) / 16 * 16;  // per workgroup
         lp->wpb = W;
@@ -1041,7 +1042,7 @@ extern "C" int gvs_autotune(gvs_index *ix, const float *queries, int64_t nq, int
     std::vector<gvs_params> cand;
     for (int mc : {1, 2, 4, 8}) {
         if (mc * rpc > MAX_ROWS) continue;
-        for (int mg : {1, 2})
+        for (int mg : {1, 2, 4})
             for (int pers : {0, 1}) {
                 gvs_params c = *p;
                 c.mc = mc;
@@ -1066,16 +1067,17 @@ extern "C" int gvs_autotune(gvs_index *ix, const float *queries, int64_t nq, int
             }
         // several waves per query pay off only when there are few queries
         if (nq <= 64 && p->storage == GVS_STORE_F32)
-            for (int W : {2, 4, 8}) {
-                gvs_params c = *p;
-                c.mc = mc;
-                c.mg = 1;
-                c.persistent = 0;
-                c.waves_per_query = W;
-                c.grid_waves = 0;
-                c.flags = (p->flags | GVS_FLAG_ASYNC);
-                cand.push_back(c);
-            }
+            for (int W : {2, 4, 8})
+                for (int mg : {1, 2, 4}) {
+                    gvs_params c = *p;
+                    c.mc = mc;
+                    c.mg = mg;
+                    c.persistent = 0;
+                    c.waves_per_query = W;
+                    c.grid_waves = 0;
+                    c.flags = (p->flags | GVS_FLAG_ASYNC);
+                    cand.push_back(c);
+                }
     }
     // Above ef 128 the visited set's placement is timed too, unless the
     // caller fixed it: the LDS hash costs fewer instructions and HBM atomics

@@ -45,6 +45,7 @@ struct LaunchPlan {
     void *fn = nullptr;
     void *dfn = nullptr;           // upper-layer descent kernel (region order)
     int ef_cap = 0, lcap = 0, blog2 = 0, vlog2 = 0, hlog2 = 0, hnarrow = 0, hbits = 0, lds_per_wave = 0, wpb = 1;
+    int mg = 1;                    // DST groups in flight (sizes the LDS ring of selected groups)
     int64_t resident_waves = 0;
 };
 }  // namespace gvs

@@ -32,6 +32,11 @@ namespace gvsk {
 constexpr int WAVE = 64;
 constexpr uint64_t EMPTY_KEY = ~0ull;
 constexpr int MAX_ROWS = 8;        // largest mc * ceil(r0/64) adjacency rows per step
+constexpr int MAX_MG = 8;          // largest number of DST groups in flight (4-bit sizes in one word)
+
+// LDS words of one wave's (or one workgroup's) ring of selected groups: mg
+// lists of up to 8 ids (at least two lists, 64 bytes, for a single wave).
+__host__ __device__ constexpr int gvs_sel_words(int mg) { return 8 * (mg < 2 ? 2 : mg); }
 constexpr int STORE_F32 = 0, STORE_BF16 = 1, STORE_FP8 = 2;
 // fp32 rows summed in hnswlib's scalar order (one lane per row, res += t*t
 // for i = 0..D-1, HNSWLIB/space_l2.h:6-20, space_ip.h:7-19) instead of the

@@ -19,8 +19,8 @@ namespace gvsk {
 // LDS carve-up of one wave's partition (sizes in P).
 struct WaveLds {
     uint64_t *poolA, *poolB, *lkeys, *bloom;
-    uint32_t *lids, *selA, *selB, *vh;
-    uint32_t nbw;  // Bloom words
+    uint32_t *lids, *sel, *vh;  // sel: ring of selected groups, 8 ids per group
+    uint32_t nbw;               // Bloom words
 };
 
 __device__ __forceinline__ WaveLds carve(const KParams &P, unsigned char *base)
@@ -31,9 +31,8 @@ __device__ __forceinline__ WaveLds carve(const KParams &P, unsigned char *base)
     L.lkeys = L.poolB + P.ef_cap;
     L.bloom = L.lkeys + P.lcap;
     L.lids = (uint32_t *)(L.bloom + (1u << P.blog2));
-    L.selA = L.lids + P.lcap;
-    L.selB = L.selA + 8;
-    L.vh = L.selB + 8;
+    L.sel = L.lids + P.lcap;
+    L.vh = L.sel + gvs_sel_words(P.mg);
     L.nbw = 1u << P.blog2;
     return L;
 }
@@ -652,27 +651,41 @@ __device__ bool search_query_req(const KParams &P, int64_t qi, int64_t slot, uns
 
     int nres = 0;
     bool haveQ = false;
-    uint32_t *selP = L.selA, *selN = L.selB;
+    // DST pipeline: up to mg groups in flight, i.e. selected and not yet
+    // merged.  Selected groups wait in a ring of mg lists of up to 8 ids in
+    // LDS (list sizes 4 bits per ring position in gsz); each step expands the
+    // oldest, after issuing the adjacency loads of the one behind it, so
+    // those overlap its gathers.  A group is selected from the pool as it
+    // stands once the oldest in-flight group has been merged.
+    const int mg = P.mg;
+    uint32_t gsz = 0u;
+    int gh = 0, pend = 0, inflight = 0;
+    bool pLoaded = false;
     int32_t idsP[MROWS], idsN[MROWS];
-    int nselP = 0, nselN = 0;
     for (;;) {
         if (haveQ) {  // complete the oldest group
             np = merge_into_pool(pool, pool2, np, ef, L.lkeys, nres, hint);
             haveQ = false;
+            inflight--;
         }
         PROF_MARK(6);
         // refill the pipeline up to mg groups
-        if (nselP == 0) {
-            nselP = select_group(pool, np, P.mc, hint, selP);
-            if (nselP) load_adj<MROWS>(P, selP, nselP, rpc, idsP);
-            hops += nselP;
+        while (inflight < mg) {
+            int pos = gh + pend;
+            if (pos >= mg) pos -= mg;
+            const int ns = select_group(pool, np, P.mc, hint, L.sel + 8 * pos);
+            if (ns == 0) break;
+            gsz = (gsz & ~(15u << (4 * pos))) | ((uint32_t)ns << (4 * pos));
+            pend++;
+            inflight++;
+            hops += ns;
         }
-        if (P.mg > 1 && nselP > 0 && nselN == 0) {
-            nselN = select_group(pool, np, P.mc, hint, selN);
-            if (nselN) load_adj<MROWS>(P, selN, nselN, rpc, idsN);
-            hops += nselN;
-        }
-        if (nselP == 0) break;
+        if (pend == 0) break;
+        const int nselP = (int)((gsz >> (4 * gh)) & 15u);
+        if (!pLoaded) load_adj<MROWS>(P, L.sel + 8 * gh, nselP, rpc, idsP);
+        const int gn = gh + 1 == mg ? 0 : gh + 1;
+        pLoaded = pend > 1;
+        if (pLoaded) load_adj<MROWS>(P, L.sel + 8 * gn, (int)((gsz >> (4 * gn)) & 15u), rpc, idsN);
         PROF_MARK(1);
         if (lds_mode)
             nres = expand_lds<NCH, STORE, METRIC, MROWS>(P, L, qf, idsP, nselP * rpc, evals EXP_PROF_ARGS);
@@ -685,11 +698,8 @@ __device__ bool search_query_req(const KParams &P, int64_t qi, int64_t slot, uns
         }
         haveQ = true;
         // the next group moves up
-        uint32_t *ts = selP;
-        selP = selN;
-        selN = ts;
-        nselP = nselN;
-        nselN = 0;
+        pend--;
+        gh = gn;
 #pragma unroll
         for (int r = 0; r < MROWS; r++) idsP[r] = idsN[r];
         if (evals > tlimit) {
@@ -804,13 +814,15 @@ __device__ __forceinline__ void search_query(const KParams &P, int64_t qi, int64
 // ---- several waves per query (intra-query parallelism) -------------------
 // One workgroup of W waves owns one query (the reference's INTRA design
 // spreads one query's neighbour work over channels, INTRA/utils.hpp:396-570).
-// Each step: wave 0 selects the group of up to mc candidates; the group's
-// mc * ceil(r0/64) adjacency rows are dealt round-robin to the W waves, which
-// test them against the shared visited set and gather and evaluate the new
-// ids into their own key lists; wave 0 then merges all lists into the pool.
-// The visited set is exact and shared, and the merge keeps the ef smallest
-// keys whatever the order of the lists, so the result, hops and evaluation
-// counts equal those of one wave running DST(mc, 1).
+// Each step: wave 0 merges the lists of the group expanded last and refills
+// the pipeline of selected groups (up to mg in flight, each of up to mc
+// candidates); the oldest group's mc * ceil(r0/64) adjacency rows are dealt
+// round-robin to the W waves, which issue the loads of their rows of the
+// next group, then test the current rows against the shared visited set and
+// gather and evaluate the new ids into their own key lists.  The visited set
+// is exact and shared, groups are expanded in selection order, and the merge
+// keeps the ef smallest keys whatever the order of the lists, so the result,
+// hops and evaluation counts equal those of one wave running DST(mc, mg).
 __device__ __forceinline__ void wg_sync()
 {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
@@ -834,13 +846,15 @@ __device__ __forceinline__ WgLds carve_wg(const KParams &P, unsigned char *base,
     G.lkeys = G.bloom + (1u << P.blog2);
     G.lids = (uint32_t *)(G.lkeys + W * WROWS * WAVE);
     G.sel = G.lids + W * WROWS * WAVE;
-    G.ctl = G.sel + 8;
+    G.ctl = G.sel + 8 * P.mg;
     G.vh = G.ctl + 8;
     return G;
 }
 
-// ctl words
-constexpr int WG_NSEL = 0, WG_EVALS = 1, WG_MODE = 2, WG_STOP = 3;
+// ctl words: the group to expand (size, ring position), the group behind it
+// (size, 0 if none, and position), evaluations so far, visited-mode switch,
+// overflow stop
+constexpr int WG_NSEL = 0, WG_EVALS = 1, WG_MODE = 2, WG_STOP = 3, WG_POS = 4, WG_NNEXT = 5, WG_PNEXT = 6;
 
 // One query with its own k and ef (query Q; sys_query: Q is in host memory,
 // read with system-scope loads).  Returns true on every wave if the
@@ -864,8 +878,7 @@ __device__ bool search_query_mw_req(const KParams &P, int64_t qi, int64_t slot, 
     L.lids = G.lids + w * WROWS * WAVE;
     L.bloom = G.bloom;
     L.vh = G.vh;
-    L.selA = G.sel;
-    L.selB = G.sel;
+    L.sel = G.sel;
     L.nbw = 1u << P.blog2;
 
     if (ef < k) ef = k;
@@ -916,53 +929,56 @@ __device__ bool search_query_mw_req(const KParams &P, int64_t qi, int64_t slot, 
     wg_sync();
     const int rpc = (P.r0 + WAVE - 1) / WAVE;
     bool overflow = false;
+    // wave 0's pipeline state (DST groups, as in the one-wave search)
+    const int mg = P.mg;
+    uint32_t gsz = 0u;
+    int gh = 0, pend = 0, inflight = 0;
+    bool haveQ = false;
+    // every wave: its rows of the current group, and of the next one when
+    // their loads were issued during the previous step
+    int32_t idsP[WROWS], idsN[WROWS];
+    bool pLoaded = false;
     for (;;) {
         if (w == 0) {
-            const int nsel = select_group(pool, np, P.mc, hint, G.sel);
-            hops += nsel;
-            if (lane == 0) G.ctl[WG_NSEL] = (uint32_t)nsel;
-        }
-        wg_sync();
-        const int nsel = (int)G.ctl[WG_NSEL];
-        if (nsel == 0) break;
-        const int nrows = nsel * rpc;
-        int32_t ids[WROWS];
-        int myrows = 0;
-#pragma unroll
-        for (int j = 0; j < WROWS; j++) {
-            ids[j] = -1;
-            const int r = w + W * j;
-            if (r < nrows) {
-                const int ci = r / rpc, jj = (r - ci * rpc) * WAVE + lane;
-                if (jj < P.r0) ids[j] = P.adj0[(size_t)G.sel[ci] * P.r0 + jj];
-                myrows = j + 1;
-            }
-        }
-        int evals_w = 0, nres = 0;
-        if (myrows > 0) {
-            if (lds_mode)
-                nres = expand_lds<NCH, STORE, METRIC, WROWS>(P, L, qf, ids, myrows, evals_w EXP_PROF_ARGS);
-            else
-                nres = expand_group<NCH, STORE, METRIC, WROWS>(P, L, qf, T, ids, myrows, evals_w,
-                                                               frozen EXP_PROF_ARGS);
-        }
-        for (int i = nres + lane; i < WROWS * WAVE; i += WAVE) L.lkeys[i] = EMPTY_KEY;
-        if (lane == 0 && evals_w) atomicAdd(&G.ctl[WG_EVALS], (uint32_t)evals_w);
-        wg_sync();
-        if (w == 0) {
-            np = merge_into_pool(pool, pool2, np, ef, G.lkeys, W * WROWS * WAVE, hint);
-            const int evals = (int)G.ctl[WG_EVALS];
-            if (evals > tlimit) {
-                if (lds_mode && P.vtab) {
-                    for (uint32_t i = lane; i < L.nbw; i += WAVE) G.bloom[i] = 0ull;
-                    if (lane == 0) {
-                        G.ctl[WG_MODE] = 1u;
-                        atomicAdd(&P.counters[2], 1u);
+            if (haveQ) {  // complete the oldest group
+                np = merge_into_pool(pool, pool2, np, ef, G.lkeys, W * WROWS * WAVE, hint);
+                haveQ = false;
+                inflight--;
+                const int evals = (int)G.ctl[WG_EVALS];
+                if (evals > tlimit) {
+                    if (lds_mode && P.vtab) {
+                        for (uint32_t i = lane; i < L.nbw; i += WAVE) G.bloom[i] = 0ull;
+                        if (lane == 0) {
+                            G.ctl[WG_MODE] = 1u;
+                            atomicAdd(&P.counters[2], 1u);
+                        }
+                        tlimit = evals + (int)((3u << P.vlog2) >> 2) - P.lcap;
+                    } else if (lane == 0) {
+                        G.ctl[WG_STOP] = 1u;
                     }
-                    tlimit = evals + (int)((3u << P.vlog2) >> 2) - P.lcap;
-                } else if (lane == 0) {
-                    G.ctl[WG_STOP] = 1u;
                 }
+            }
+            while (inflight < mg) {  // refill the pipeline up to mg groups
+                int pos = gh + pend;
+                if (pos >= mg) pos -= mg;
+                const int ns = select_group(pool, np, P.mc, hint, G.sel + 8 * pos);
+                if (ns == 0) break;
+                gsz = (gsz & ~(15u << (4 * pos))) | ((uint32_t)ns << (4 * pos));
+                pend++;
+                inflight++;
+                hops += ns;
+            }
+            const int gn = gh + 1 == mg ? 0 : gh + 1;
+            if (lane == 0) {
+                G.ctl[WG_NSEL] = pend > 0 ? (gsz >> (4 * gh)) & 15u : 0u;
+                G.ctl[WG_POS] = (uint32_t)gh;
+                G.ctl[WG_NNEXT] = pend > 1 ? (gsz >> (4 * gn)) & 15u : 0u;
+                G.ctl[WG_PNEXT] = (uint32_t)gn;
+            }
+            if (pend > 0) {
+                pend--;
+                gh = gn;
+                haveQ = true;
             }
         }
         wg_sync();
@@ -974,8 +990,42 @@ __device__ bool search_query_mw_req(const KParams &P, int64_t qi, int64_t slot, 
             lds_mode = false;
             frozen = G.vh;
             T = P.vtab + ((size_t)slot << P.vlog2);
-            tlimit = (int)G.ctl[WG_EVALS] + (int)((3u << P.vlog2) >> 2) - P.lcap;
         }
+        const int nsel = (int)G.ctl[WG_NSEL];
+        if (nsel == 0) break;
+        const int nnext = (int)G.ctl[WG_NNEXT];
+        const uint32_t *selC = G.sel + 8 * G.ctl[WG_POS];
+        const uint32_t *selN = G.sel + 8 * G.ctl[WG_PNEXT];
+        // this wave's rows r = w + W * j of a group of n candidates
+        const auto rows_of = [&](const uint32_t *sl, int n, int32_t(&ids)[WROWS]) {
+#pragma unroll
+            for (int j = 0; j < WROWS; j++) {
+                ids[j] = -1;
+                const int r = w + W * j;
+                if (r < n * rpc) {
+                    const int ci = r / rpc, jj = (r - ci * rpc) * WAVE + lane;
+                    if (jj < P.r0) ids[j] = P.adj0[(size_t)sl[ci] * P.r0 + jj];
+                }
+            }
+        };
+        if (!pLoaded) rows_of(selC, nsel, idsP);
+        pLoaded = nnext > 0;
+        if (pLoaded) rows_of(selN, nnext, idsN);
+        const int nrows = nsel * rpc;
+        const int myrows = w < nrows ? min(WROWS, (nrows - w + W - 1) / W) : 0;
+        int evals_w = 0, nres = 0;
+        if (myrows > 0) {
+            if (lds_mode)
+                nres = expand_lds<NCH, STORE, METRIC, WROWS>(P, L, qf, idsP, myrows, evals_w EXP_PROF_ARGS);
+            else
+                nres = expand_group<NCH, STORE, METRIC, WROWS>(P, L, qf, T, idsP, myrows, evals_w,
+                                                               frozen EXP_PROF_ARGS);
+        }
+        for (int i = nres + lane; i < WROWS * WAVE; i += WAVE) L.lkeys[i] = EMPTY_KEY;
+        if (lane == 0 && evals_w) atomicAdd(&G.ctl[WG_EVALS], (uint32_t)evals_w);
+#pragma unroll
+        for (int j = 0; j < WROWS; j++) idsP[j] = idsN[j];
+        wg_sync();
     }
 
     if (w == 0) {

@@ -67,7 +67,7 @@ typedef struct gvs_params {
     int32_t k;
     int32_t ef;
     int32_t mc;                /* candidates expanded per step (1..8) */
-    int32_t mg;                /* steps in flight per query (1..2) */
+    int32_t mg;                /* groups in flight per query (1..8) */
     int32_t waves_per_query;   /* 1 = inter-query; 2,4,8 = intra-query (one workgroup per query) */
     int32_t storage;           /* gvs_storage traversed; the copy must exist */
     int32_t rerank;            /* 1: exact fp32 rerank of the final ef candidates */
@@ -248,9 +248,9 @@ GVS_API int gvs_hash_murmur2_keylen4_device(const uint32_t *keys, const uint32_t
                                             int64_t n, void *stream);
 /* Autotuner: times the search of nq DEVICE queries (k, ef, storage, rerank
  * and flags from *p; labels/dists are device result buffers it may overwrite)
- * over mc in {1,2,4,8} (as r0 allows) x mg in {1,2} x persistent in {0,1}
+ * over mc in {1,2,4,8} (as r0 allows) x mg in {1,2,4} x persistent in {0,1}
  * (persistent also at half the resident waves), and for nq <= 64 also
- * waves_per_query in {2,4,8}; above ef_eff 128, unless p->flags fixes the
+ * waves_per_query in {2,4,8} x mg in {1,2,4}; above ef_eff 128, unless p->flags fixes the
  * visited placement, each one-wave candidate also with GVS_FLAG_HBM_VISITED
  * (best->flags says which won); writes the fastest configuration to *best and
  * its time per launch to *best_ms.  DST settings trade evaluations for steps

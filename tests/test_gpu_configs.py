@@ -79,7 +79,8 @@ def c3_graph():
 @pytest.mark.parametrize("W", [1, 2, 4, 8])
 def test_c3_batch1_intra_query_dst(W, ef):
     """One query per call, eager and replayed from a captured graph, equals
-    the oracle's DST(mc, 1) for every mc."""
+    the oracle's DST(mc, mg) for every mc, with one and two groups in
+    flight."""
     from gvs import hip as H
     c = c3_graph()
     dev, og, q = c["dev"], c["og"], c["q"]
@@ -88,13 +89,14 @@ def test_c3_batch1_intra_query_dst(W, ef):
     dd = H.DeviceArray((1, 10), np.float32)
     ds = H.DeviceArray((1, gvs.STATS_ROW), np.int32)
     st = H.Stream()
-    for mc in (1, 2, 4, 8):
-        print(f"case C3 W={W} mc={mc} ef={ef}", flush=True)
-        ids, dis, ost = og.search(q, 10, ef, mc=mc, order=orc.WAVE)
-        sg = dev.capture(dq.address, 1, 128, dl.address, dd.address, k=10, ef=ef, mc=mc, waves_per_query=W,
-                         stats_ptr=ds.address)
+    for mc, mg in ((1, 1), (2, 1), (4, 1), (8, 1), (1, 2), (2, 2), (8, 2)):
+        print(f"case C3 W={W} mc={mc} mg={mg} ef={ef}", flush=True)
+        ids, dis, ost = og.search(q, 10, ef, mc=mc, mg=mg, order=orc.WAVE)
+        sg = dev.capture(dq.address, 1, 128, dl.address, dd.address, k=10, ef=ef, mc=mc, mg=mg,
+                         waves_per_query=W, stats_ptr=ds.address)
         for i in range(q.shape[0]):
-            lab, dist, cnt = dev.search(q[i:i + 1], k=10, ef=ef, mc=mc, waves_per_query=W, with_stats=True)
+            lab, dist, cnt = dev.search(q[i:i + 1], k=10, ef=ef, mc=mc, mg=mg, waves_per_query=W,
+                                        with_stats=True)
             np.testing.assert_array_equal(lab[0], ids[i])
             np.testing.assert_array_equal(dist[0], dis[i])
             np.testing.assert_array_equal(cnt[0].astype(np.int64), ost[i])

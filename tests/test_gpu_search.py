@@ -58,7 +58,7 @@ def test_dst_multi_candidate_matches_oracle(mc):
     check_equal(dev, og, q, 100, 400, mc=mc, persistent=True, region_order=True)
 
 
-@pytest.mark.parametrize("mc,mg", [(1, 2), (2, 2), (4, 2)])
+@pytest.mark.parametrize("mc,mg", [(1, 2), (2, 2), (4, 2), (1, 3), (2, 4), (4, 8), (8, 4)])
 def test_dst_pipelined_groups_match_oracle(mc, mg):
     x, g, dev = graph(20000, 128, 16, 100)
     og = orc.OracleGraph(g.arrays())
@@ -382,16 +382,18 @@ def test_captured_graph_replays_match_eager(persistent):
     sg.close()
 
 
-@pytest.mark.parametrize("W,mc", [(2, 1), (2, 2), (4, 4), (8, 8), (8, 2), (4, 8)])
-def test_intra_query_waves_match_single_wave_dst(W, mc):
+@pytest.mark.parametrize("W,mc,mg", [(2, 1, 1), (2, 2, 1), (4, 4, 1), (8, 8, 1), (8, 2, 1), (4, 8, 1),
+                                     (2, 1, 2), (2, 2, 2), (4, 4, 2), (8, 8, 2), (8, 2, 4), (4, 8, 4),
+                                     (8, 8, 8), (2, 1, 3)])
+def test_intra_query_waves_match_single_wave_dst(W, mc, mg):
     """Several waves per query (one workgroup) give the ids, distances and
-    counters of one wave running DST(mc, 1), i.e. the oracle's."""
+    counters of one wave running DST(mc, mg), i.e. the oracle's."""
     x, g, dev = graph(20000, 128, 16, 100)
     og = orc.OracleGraph(g.arrays())
     q = queries(150, 128, seed=50)
     for ef in (32, 128):
-        print(f"case intra W={W} mc={mc} ef={ef}", flush=True)
-        check_equal(dev, og, q, 10, ef, mc=mc, waves_per_query=W)
+        print(f"case intra W={W} mc={mc} mg={mg} ef={ef}", flush=True)
+        check_equal(dev, og, q, 10, ef, mc=mc, mg=mg, waves_per_query=W)
 
 
 @pytest.mark.parametrize("W", [2, 8])
@@ -421,9 +423,13 @@ def test_intra_query_rejects_unsupported():
     x, g, dev = graph(20000, 128, 16, 100)
     q = queries(4, 128, seed=53)
     with pytest.raises(gvs.GvsError):
-        dev.search(q, k=10, ef=64, mc=2, mg=2, waves_per_query=4)
+        dev.search(q, k=10, ef=64, mc=2, mg=9, waves_per_query=4)
+    with pytest.raises(gvs.GvsError):
+        dev.search(q, k=10, ef=64, mc=2, mg=9)
     with pytest.raises(gvs.GvsError):
         dev.search(q, k=10, ef=64, waves_per_query=3)
+    with pytest.raises(gvs.GvsError):
+        dev.search(q, k=10, ef=64, waves_per_query=2, storage=gvs.BF16)
 
 
 @pytest.mark.parametrize("nq", [16, 500])
@@ -439,7 +445,7 @@ def test_autotuner_returns_a_working_configuration(nq):
     dd = H.DeviceArray((nq, 10), np.float32)
     print(f"case autotune nq={nq}", flush=True)
     cfg, ms = dev.autotune(dq.address, nq, 128, dl.address, dd.address, k=10, ef=64)
-    assert ms > 0 and cfg["ef"] == 64 and cfg["mc"] in (1, 2, 4, 8) and cfg["mg"] in (1, 2)
+    assert ms > 0 and cfg["ef"] == 64 and cfg["mc"] in (1, 2, 4, 8) and cfg["mg"] in (1, 2, 4)
     assert cfg["waves_per_query"] in ((1, 2, 4, 8) if nq <= 64 else (1,))
     kw = {kk: v for kk, v in cfg.items() if kk not in ("ef", "mc", "mg")}
     check_equal(dev, og, q, 10, 64, mc=cfg["mc"], mg=cfg["mg"], **kw)
@@ -496,16 +502,35 @@ class ArrayGraph:
 def test_dst_hand_traced_fixture_on_gpu(mc, mg):
     """The hand-traced DST fixture (tests/test_oracle.py): the kernel gives
     the derived results, hops and evaluations for each (mc, mg), one wave
-    per query and (mg = 1) two waves per query."""
+    per query and two waves per query."""
     import test_oracle as T
     dev = gvs.DeviceIndex(ArrayGraph(T.dst_fixture_arrays()))
     q = np.zeros((1, 1), dtype=np.float32)
     want_ids, want_groups, want_evals = T.DST_EXPECT[(mc, mg)]
     x = np.float32(T.DST_X)
-    runs = [dict()] + ([dict(waves_per_query=2)] if mg == 1 else [])
+    runs = [dict(), dict(waves_per_query=2)]
     for kw in runs:
         print(f"case hand-traced DST mc={mc} mg={mg} {kw}", flush=True)
         lab, dist, st = dev.search(q, k=2, ef=2, mc=mc, mg=mg, with_stats=True, **kw)
+        assert lab[0].tolist() == want_ids
+        np.testing.assert_array_equal(dist[0], x[want_ids] * x[want_ids])
+        assert st[0, 0] == sum(len(gr) for gr in want_groups)
+        assert st[0, 1] == want_evals
+
+
+@pytest.mark.parametrize("mc,mg", [(1, 2), (1, 3), (1, 4), (2, 3)])
+def test_dst_hand_traced_pipeline_depth_on_gpu(mc, mg):
+    """The second hand-traced DST fixture (tests/test_oracle.py), where two
+    and three groups in flight give different results: one wave per query
+    and two and four waves per query."""
+    import test_oracle as T
+    dev = gvs.DeviceIndex(ArrayGraph(T.dst2_fixture_arrays()))
+    q = np.zeros((1, 1), dtype=np.float32)
+    want_ids, want_groups, want_evals = T.DST2_EXPECT[(mc, mg)]
+    x = np.float32(T.DST2_X)
+    for kw in (dict(), dict(waves_per_query=2), dict(waves_per_query=4)):
+        print(f"case hand-traced DST2 mc={mc} mg={mg} {kw}", flush=True)
+        lab, dist, st = dev.search(q, k=3, ef=3, mc=mc, mg=mg, with_stats=True, **kw)
         assert lab[0].tolist() == want_ids
         np.testing.assert_array_equal(dist[0], x[want_ids] * x[want_ids])
         assert st[0, 0] == sum(len(gr) for gr in want_groups)

@@ -285,6 +285,70 @@ def test_dst_hand_traced_expansion_order(mc, mg):
     assert st[4] == len(want_groups)                           # groups
 
 
+# A second hand-traced fixture, where the depth of the pipeline matters.
+# Ten nodes on a line, query at x = 0, L2, flat graph, entry P0, ef = k = 3:
+#   id:  0 x=10 (d=100)  1 x=4.5 (20.25)  2 x=9.5 (90.25)  3 x=5.5 (30.25)
+#        4 x=8 (64)      5 x=2.5 (6.25)   6 x=7.5 (56.25)  7 x=8.5 (72.25)
+#        8 x=7 (49)      9 x=5 (25)
+#   links: 0->2,6,4  1->3,9  2->5,0,3  3->6,0  4->8,9  5->7  6->1,0,9
+#          7->6,9  8->6,9  9->4,6
+# Node 5 (the nearest) hides behind node 2, the worst of the first three
+# results; only a threshold that lags two groups behind lets 2 be expanded.
+DST2_X = [10.0, 4.5, 9.5, 5.5, 8.0, 2.5, 7.5, 8.5, 7.0, 5.0]
+DST2_ADJ = [[2, 6, 4], [3, 9], [5, 0, 3], [6, 0], [8, 9], [7], [1, 0, 9], [6, 9], [6, 9], [4, 6]]
+DST2_EXPECT = {
+    # Every run starts: group {0} evaluates 2, 6, 4 (4 evaluations); merged,
+    # the results are {6 56.25, 4 64, 2 90.25}.
+    # mg = 2: groups {6} (evaluates 1, 9) and {4} (8) go out.  {6} merges
+    # (results {1, 9, 6}, worst 56.25); {1} goes out (3).  {4} merges ({1, 9,
+    # 8}, worst 49); {9} goes out (nothing new).  {1} merges ({1, 9, 3},
+    # worst 30.25); {3} goes out (nothing).  Node 2 (90.25) and node 8 (49)
+    # are beyond 30.25: stop.  8 evaluations.
+    (1, 2): ([1, 9, 3], [[0], [6], [4], [1], [9], [3]], 8),
+    # mg = 3: {6}, {4} and {2} go out while the worst result is still 90.25;
+    # {2} evaluates 5 and 3.  {6} merges ({6, 1, 9}, worst 56.25); {1} goes
+    # out (3 and 9 already visited).  {4} merges ({1, 9, 8}, worst 49); {9}
+    # goes out (nothing).  {2} merges 5 ({5, 1, 9}, worst 25; 3 at 30.25
+    # stays out); {5} goes out (7 at 72.25).  8 is beyond 25: stop.
+    # 10 evaluations.
+    (1, 3): ([5, 1, 9], [[0], [6], [4], [2], [1], [9], [5]], 10),
+    # mg = 4: {6}, {4}, {2} go out (the candidate set is then empty).  {6}
+    # merges ({6, 1, 9}); {1} and {9} go out (nothing new).  {4} merges
+    # ({1, 9, 8}, worst 49); {8} goes out (nothing).  {2} merges ({5, 1, 9});
+    # {5} goes out (7).  10 evaluations, 8 groups.
+    (1, 4): ([5, 1, 9], [[0], [6], [4], [2], [1], [9], [8], [5]], 10),
+    # mc = 2, mg = 3: {6, 4} (evaluates 1, 9, 8) then {2} (5, 3).  {6, 4}
+    # merges ({1, 9, 8}, worst 49); {1, 9} and {8} go out (nothing new).  {2}
+    # merges ({5, 1, 9}, worst 25); {5} goes out (7).  10 evaluations.
+    (2, 3): ([5, 1, 9], [[0], [6, 4], [2], [1, 9], [8], [5]], 10),
+}
+
+
+def dst2_fixture_arrays():
+    adj = -np.ones((10, 3), dtype=np.int32)
+    for v, row in enumerate(DST2_ADJ):
+        adj[v, :len(row)] = row
+    x = np.zeros((10, 1), dtype=np.float32)
+    x[:, 0] = DST2_X
+    return {"n": 10, "dim": 1, "metric": orc.L2, "r0": 3, "max_level": 0, "entry": 0, "ru": 0, "vectors": x,
+            "adj0": adj, "upper_off": np.zeros(0, np.int64), "adj_up": np.zeros((0, 1), np.int32)}
+
+
+@pytest.mark.parametrize("mc,mg", sorted(DST2_EXPECT))
+def test_dst_hand_traced_pipeline_depth(mc, mg):
+    og = orc.OracleGraph(dst2_fixture_arrays())
+    q = np.zeros(1, dtype=np.float32)
+    ids, ds, st, groups = og.trace(q, 3, 3, mc=mc, mg=mg)
+    want_ids, want_groups, want_evals = DST2_EXPECT[(mc, mg)]
+    assert ids.tolist() == want_ids
+    x = np.float32(DST2_X)
+    np.testing.assert_array_equal(ds, x[want_ids] * x[want_ids])
+    assert groups == want_groups
+    assert st[0] == sum(len(gr) for gr in want_groups)
+    assert st[1] == want_evals
+    assert st[4] == len(want_groups)
+
+
 def test_dst_group_count_below_hops_for_multi_candidate_groups():
     x = gvs.gen_mixture(6000, 32, 32, 21, 22)
     q = gvs.gen_mixture(300, 32, 32, 21, 23)
