@@ -258,7 +258,8 @@ def precision_c4(a, gvs, g, x, q):
 
 def intra_latency(a, gvs, g, q):
     """Batch=1 latency: one query per launch through a captured graph, for
-    waves per query x candidates per step x ef; p50/p99 over 190 queries
+    waves per query x candidates per step (and a few pipelined DST settings,
+    mg > 1) x ef; p50/p99 over 190 queries
     (10 warm-up), mean evaluations and recall@10 against FP64 ground truth."""
     from gvs import hip as H
     dev = gvs.DeviceIndex(g)
@@ -277,31 +278,30 @@ def intra_latency(a, gvs, g, q):
     rows = []
     for ef in (64, 128, 256, 512):
         base = None
-        for W in (1, 2, 4, 8):
-            for mc in (1, 2, 4, 8):
-                if W == 8 and mc == 1:
-                    continue
-                sg = dev.capture(dq.address, 1, a.dim, dl.address, dd.address, k=a.k, ef=ef, mc=mc,
-                                 waves_per_query=W, stats_ptr=ds.address)
-                lat, ev, hits = [], [], 0
-                for i in range(nq):
-                    dq.upload(q[i:i + 1])
-                    t0 = time.perf_counter()
-                    sg.launch(st.handle)
-                    st.synchronize()
-                    lat.append(time.perf_counter() - t0)
-                    ev.append(int(ds.to_host()[0, 1]))
-                    hits += len(set(dl.to_host()[0].tolist()) & set(gt[i].tolist()))
-                sg.close()
-                lat = np.array(lat[10:]) * 1e6
-                r = {"ef": ef, "waves": W, "mc": mc, "p50_us": round(float(np.percentile(lat, 50)), 1),
-                     "p99_us": round(float(np.percentile(lat, 99)), 1), "evals": round(float(np.mean(ev)), 1),
-                     "recall_at_10": round(hits / (nq * a.k), 4)}
-                if W == 1 and mc == 1:
-                    base = r
-                r["p50_speedup_vs_plain"] = round(base["p50_us"] / r["p50_us"], 2)
-                rows.append(r)
-                log(f"[intra] {r}")
+        configs = [(W, mc, 1) for W in (1, 2, 4, 8) for mc in (1, 2, 4, 8) if not (W == 8 and mc == 1)]
+        configs += [(1, 1, 2), (1, 2, 2), (4, 4, 2), (8, 8, 2), (8, 8, 4)]
+        for W, mc, mg in configs:
+            sg = dev.capture(dq.address, 1, a.dim, dl.address, dd.address, k=a.k, ef=ef, mc=mc, mg=mg,
+                             waves_per_query=W, stats_ptr=ds.address)
+            lat, ev, hits = [], [], 0
+            for i in range(nq):
+                dq.upload(q[i:i + 1])
+                t0 = time.perf_counter()
+                sg.launch(st.handle)
+                st.synchronize()
+                lat.append(time.perf_counter() - t0)
+                ev.append(int(ds.to_host()[0, 1]))
+                hits += len(set(dl.to_host()[0].tolist()) & set(gt[i].tolist()))
+            sg.close()
+            lat = np.array(lat[10:]) * 1e6
+            r = {"ef": ef, "waves": W, "mc": mc, "mg": mg, "p50_us": round(float(np.percentile(lat, 50)), 1),
+                 "p99_us": round(float(np.percentile(lat, 99)), 1), "evals": round(float(np.mean(ev)), 1),
+                 "recall_at_10": round(hits / (nq * a.k), 4)}
+            if W == 1 and mc == 1 and mg == 1:
+                base = r
+            r["p50_speedup_vs_plain"] = round(base["p50_us"] / r["p50_us"], 2)
+            rows.append(r)
+            log(f"[intra] {r}")
     best = {ef: max((r for r in rows if r["ef"] == ef), key=lambda r: r["p50_speedup_vs_plain"]) for ef in (64, 128, 256, 512)}
     print(json.dumps({"metric": "batch=1 search latency, intra-query DST vs plain", "unit": "us",
                       "config": {"workload": f"HNSW search, {a.n}x{a.dim} fp32 L2, M={a.M}, efC={a.efc}, "

@@ -492,7 +492,7 @@ int plan_search(const gvs_index *ix, const gvs_params *p, int max_ef, int max_k,
     if (W > 1) {
         // one workgroup per query: shared pool, Bloom words, selection and
         // control words, visited hash; per wave wrows*64 keys and ids
-        int bytes = 16 * lp->ef_cap + 8 * (1 << lp->blog2) + W * wrows * WAVE * 12 + 32 * p->mg + 32 +
+        int bytes = 16 * lp->ef_cap + 8 * (1 << lp->blog2) + W * wrows * WAVE * 12 + 32 * p->mg + 4 * WG_CTL_WORDS +
                     (lp->hlog2 > 0 ? 4 * (1 << lp->hlog2) : 0);
         lp->lds_per_wave = (bytes + 15) / 16 * 16;  // per workgroup
         lp->wpb = W;

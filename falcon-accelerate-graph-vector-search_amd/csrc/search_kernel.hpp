@@ -37,6 +37,8 @@ constexpr int MAX_MG = 8;          // largest number of DST groups in flight (4-
 // LDS words of one wave's (or one workgroup's) ring of selected groups: mg
 // lists of up to 8 ids (at least two lists, 64 bytes, for a single wave).
 __host__ __device__ constexpr int gvs_sel_words(int mg) { return 8 * (mg < 2 ? 2 : mg); }
+// Control words of a workgroup that runs one query on several waves.
+constexpr int WG_CTL_WORDS = 16;
 constexpr int STORE_F32 = 0, STORE_BF16 = 1, STORE_FP8 = 2;
 // fp32 rows summed in hnswlib's scalar order (one lane per row, res += t*t
 // for i = 0..D-1, HNSWLIB/space_l2.h:6-20, space_ip.h:7-19) instead of the

@@ -847,14 +847,74 @@ __device__ __forceinline__ WgLds carve_wg(const KParams &P, unsigned char *base,
     G.lids = (uint32_t *)(G.lkeys + W * WROWS * WAVE);
     G.sel = G.lids + W * WROWS * WAVE;
     G.ctl = G.sel + 8 * P.mg;
-    G.vh = G.ctl + 8;
+    G.vh = G.ctl + WG_CTL_WORDS;
     return G;
 }
 
 // ctl words: the group to expand (size, ring position), the group behind it
 // (size, 0 if none, and position), evaluations so far, visited-mode switch,
-// overflow stop
-constexpr int WG_NSEL = 0, WG_EVALS = 1, WG_MODE = 2, WG_STOP = 3, WG_POS = 4, WG_NNEXT = 5, WG_PNEXT = 6;
+// overflow stop, the smallest pool place a new key took in the last merge,
+// then per wave the number of keys it offers to the merge
+constexpr int WG_NSEL = 0, WG_EVALS = 1, WG_MODE = 2, WG_STOP = 3, WG_POS = 4, WG_NNEXT = 5, WG_PNEXT = 6,
+              WG_HINT = 7, WG_CNT = 8;
+
+// Merge by all W waves of a workgroup: list j (lists + j * lstride) holds
+// cnt[j] keys, unsorted, each below the pool's ef-th key; keys are distinct
+// (distinct ids).  The place of a key among the ef best is the number of
+// pool keys below it plus the number of list keys below it, so wave w
+// places its own list's keys (the pool rank by binary search, the list rank
+// by counting over every list) and the pool entries (t * W + w) * 64 + lane.
+// Writes pool2 (merged: the caller swaps the pools), returns the new pool
+// size, and lowers *hint_word to the smallest place a list key took.
+__device__ __forceinline__ int wg_merge(const uint64_t *pool, uint64_t *pool2, int np, int ef,
+                                        const uint64_t *lists, int lstride, const uint32_t *cnt, int W, int w,
+                                        uint32_t *hint_word, bool &merged)
+{
+    const int lane = lane_id();
+    int S = 0;
+    for (int j = 0; j < W; j++) S += (int)cnt[j];
+    S = uni(S);
+    merged = S > 0;
+    if (S == 0) return np;
+    const int cw = uni((int)cnt[w]);
+    const uint64_t *mine = lists + (size_t)w * lstride;
+    const int npw = np > w * WAVE ? (np - w * WAVE + W * WAVE - 1) / (W * WAVE) : 0;
+    const int passes = max(npw, (cw + WAVE - 1) / WAVE);
+    int top = 1;  // largest power of two <= np
+    while (2 * top <= np) top *= 2;
+    int mind = ef;
+    for (int t = 0; t < passes; t++) {
+        const int si = t * WAVE + lane;
+        const bool hs = si < cw;
+        const uint64_t sv = hs ? mine[si] : EMPTY_KEY;
+        const int pi = (t * W + w) * WAVE + lane;
+        const bool hp = pi < np;
+        const uint64_t pv = hp ? pool[pi] : EMPTY_KEY;
+        int cs = 0, cp = 0;
+        for (int j = 0; j < W; j++) {
+            const uint64_t *lj = lists + (size_t)j * lstride;
+            const int cj = uni((int)cnt[j]);
+#pragma unroll 8
+            for (int m = 0; m < cj; m++) {
+                const uint64_t v = lj[m];
+                cs += v < sv ? 1 : 0;
+                cp += v < pv ? 1 : 0;
+            }
+        }
+        if (hp && pi + cp < ef) pool2[pi + cp] = pv;
+        if (hs) {
+            int lo = 0;  // pool keys below sv
+            for (int step = top; step > 0; step >>= 1)
+                if (lo + step <= np && pool[lo + step - 1] < sv) lo += step;
+            const int d = lo + cs;
+            if (d < ef) pool2[d] = sv;
+            mind = min(mind, d);
+        }
+    }
+    const uint32_t m = (uint32_t)wave_min_u64((uint64_t)(uint32_t)mind);
+    if (lane == 0) atomicMin(hint_word, m);
+    return min(ef, np + S);
+}
 
 // One query with its own k and ef (query Q; sys_query: Q is in host memory,
 // read with system-scope loads).  Returns true on every wave if the
@@ -898,6 +958,7 @@ __device__ bool search_query_mw_req(const KParams &P, int64_t qi, int64_t slot, 
     uint32_t ep;
     float epd;
     find_entry<NCH, STORE, METRIC>(P, qi, L, qf, ep, epd, up_hops, up_evals);
+    PROF_MARK(0);
 
     bool lds_mode = P.hlog2 > 0;
     const uint32_t *frozen = nullptr;
@@ -909,7 +970,7 @@ __device__ bool search_query_mw_req(const KParams &P, int64_t qi, int64_t slot, 
     } else {
         for (uint32_t i = tid; i < L.nbw; i += nthr) G.bloom[i] = 0ull;
     }
-    if (tid < 8) G.ctl[tid] = 0u;
+    if (tid < WG_CTL_WORDS) G.ctl[tid] = 0u;
     wg_sync();
     uint64_t *pool = G.poolA, *pool2 = G.poolB;
     int np = 1, hint = 0, hops = 0;
@@ -938,12 +999,15 @@ __device__ bool search_query_mw_req(const KParams &P, int64_t qi, int64_t slot, 
     // their loads were issued during the previous step
     int32_t idsP[WROWS], idsN[WROWS];
     bool pLoaded = false;
+#ifdef GVS_PROFILE
+    int steps = 0;
+#endif
     for (;;) {
         if (w == 0) {
-            if (haveQ) {  // complete the oldest group
-                np = merge_into_pool(pool, pool2, np, ef, G.lkeys, W * WROWS * WAVE, hint);
+            if (haveQ) {  // the oldest group was merged at the end of the last step
                 haveQ = false;
                 inflight--;
+                hint = min(hint, (int)G.ctl[WG_HINT]);
                 const int evals = (int)G.ctl[WG_EVALS];
                 if (evals > tlimit) {
                     if (lds_mode && P.vtab) {
@@ -974,6 +1038,7 @@ __device__ bool search_query_mw_req(const KParams &P, int64_t qi, int64_t slot, 
                 G.ctl[WG_POS] = (uint32_t)gh;
                 G.ctl[WG_NNEXT] = pend > 1 ? (gsz >> (4 * gn)) & 15u : 0u;
                 G.ctl[WG_PNEXT] = (uint32_t)gn;
+                G.ctl[WG_HINT] = 0x7fffffffu;
             }
             if (pend > 0) {
                 pend--;
@@ -981,7 +1046,9 @@ __device__ bool search_query_mw_req(const KParams &P, int64_t qi, int64_t slot, 
                 haveQ = true;
             }
         }
+        PROF_MARK(7);
         wg_sync();
+        PROF_MARK(1);
         if (G.ctl[WG_STOP]) {
             overflow = true;
             break;
@@ -993,6 +1060,9 @@ __device__ bool search_query_mw_req(const KParams &P, int64_t qi, int64_t slot, 
         }
         const int nsel = (int)G.ctl[WG_NSEL];
         if (nsel == 0) break;
+#ifdef GVS_PROFILE
+        steps++;
+#endif
         const int nnext = (int)G.ctl[WG_NNEXT];
         const uint32_t *selC = G.sel + 8 * G.ctl[WG_POS];
         const uint32_t *selN = G.sel + 8 * G.ctl[WG_PNEXT];
@@ -1021,13 +1091,43 @@ __device__ bool search_query_mw_req(const KParams &P, int64_t qi, int64_t slot, 
                 nres = expand_group<NCH, STORE, METRIC, WROWS>(P, L, qf, T, idsP, myrows, evals_w,
                                                                frozen EXP_PROF_ARGS);
         }
-        for (int i = nres + lane; i < WROWS * WAVE; i += WAVE) L.lkeys[i] = EMPTY_KEY;
-        if (lane == 0 && evals_w) atomicAdd(&G.ctl[WG_EVALS], (uint32_t)evals_w);
+        // keep the keys that can enter the pool (below its ef-th key),
+        // compacted in place
+        {
+            const uint64_t thr = np >= ef ? pool[ef - 1] : EMPTY_KEY;
+            int c = 0;
+            for (int b = 0; b < nres; b += WAVE) {
+                const int i = b + lane;
+                const uint64_t key = i < nres ? L.lkeys[i] : EMPTY_KEY;
+                const bool sv = key < thr;
+                const uint64_t bal = __ballot(sv);
+                wave_sync();
+                if (sv) L.lkeys[c + mbcnt(bal)] = key;
+                c += __popcll(bal);
+            }
+            if (lane == 0) {
+                G.ctl[WG_CNT + w] = (uint32_t)c;
+                if (evals_w) atomicAdd(&G.ctl[WG_EVALS], (uint32_t)evals_w);
+            }
+        }
 #pragma unroll
         for (int j = 0; j < WROWS; j++) idsP[j] = idsN[j];
+        PROF_MARK(4);
         wg_sync();
+        PROF_MARK(5);
+        // every wave merges its share; the pools swap when keys came in
+        bool merged = false;
+        np = wg_merge(pool, pool2, np, ef, G.lkeys, WROWS * WAVE, G.ctl + WG_CNT, W, w, G.ctl + WG_HINT, merged);
+        if (merged) {
+            uint64_t *t = pool;
+            pool = pool2;
+            pool2 = t;
+        }
+        wg_sync();
+        PROF_MARK(6);
     }
 
+    PROF_MARK(7);
     if (w == 0) {
         const int nout = P.out_k_only ? min(k, P.k) : P.k;
         for (int i = lane; i < nout; i += WAVE) {
@@ -1044,10 +1144,18 @@ __device__ bool search_query_mw_req(const KParams &P, int64_t qi, int64_t slot, 
         }
         if (lane == 0) {
             if (P.out_stats) {
-                P.out_stats[qi * GVS_STATS_ROW + 0] = hops;
-                P.out_stats[qi * GVS_STATS_ROW + 1] = (int)G.ctl[WG_EVALS];
-                P.out_stats[qi * GVS_STATS_ROW + 2] = up_hops;
-                P.out_stats[qi * GVS_STATS_ROW + 3] = up_evals;
+                int32_t *row = P.out_stats + qi * GVS_STATS_ROW;
+                row[0] = hops;
+                row[1] = (int)G.ctl[WG_EVALS];
+                row[2] = up_hops;
+                row[3] = up_evals;
+#ifdef GVS_PROFILE
+                // wave 0's phases: descent, wait at the step barrier, visit,
+                // gather + distances, filter, wait at the list barrier,
+                // merge (with its barrier), select + output
+                for (int ph = 0; ph < 8; ph++) row[4 + ph] = (int32_t)(prof_acc[ph] >> 6);
+                row[12] = (int32_t)steps;
+#endif
             }
             if (P.status) P.status[qi] = overflow ? 1 : 0;
             if (overflow) atomicAdd(&P.counters[1], 1u);
