@@ -586,6 +586,26 @@ extern "C" int gvs_server_counters(gvs_server *s, int64_t *counters, int32_t n)
     return std::min<int32_t>(n, 7);
 }
 
+namespace gvs {
+// The index a server searches, the parameters its requests run with (k and
+// ef are per request) and its k / ef limits, so that a request can be re-run
+// on the batch path, which retries visited-table overflows with larger
+// tables.
+int server_search_setup(gvs_server *s, gvs_index **ix, gvs_params *p, int *max_k, int *max_ef)
+{
+    if (!s || !ix || !p || !max_k || !max_ef) return GVS_ERR_ARG;
+    *ix = s->ix;
+    *max_k = s->cfg.max_k;
+    *max_ef = s->cfg.max_ef;
+    std::memset(p, 0, sizeof *p);
+    p->mc = s->cfg.mc > 0 ? s->cfg.mc : 1;
+    p->mg = 1;
+    p->waves_per_query = s->cfg.waves_per_request > 0 ? s->cfg.waves_per_request : 1;
+    p->storage = GVS_STORE_F32;
+    return GVS_OK;
+}
+}  // namespace gvs
+
 extern "C" int gvs_server_close(gvs_server *s)
 {
     if (!s) return GVS_ERR_ARG;

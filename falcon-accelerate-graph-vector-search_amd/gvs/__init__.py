@@ -57,6 +57,11 @@ class ServerConfig(C.Structure):
         "max_inflight", "max_k", "max_ef", "queue_capacity", "grid_waves", "mc", "storage", "waves_per_request")]
 
 
+class NetConfig(C.Structure):
+    _fields_ = [("c2f_port", C.c_int32), ("f2c_host", C.c_char_p), ("f2c_port", C.c_int32), ("dim", C.c_int32),
+                ("ef", C.c_int32), ("timeout_ms", C.c_int32)]
+
+
 def _load():
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
@@ -104,6 +109,8 @@ def _load():
         "gvs_server_loadgen": (C.c_int, [P, P, I64, I64, C.c_double, U64, P, P, P, P, P, P]),
         "gvs_server_stats": (C.c_int, [P, P]),
         "gvs_server_close": (C.c_int, [P]),
+        "gvs_net_serve": (C.c_int, [P, P, P, P]),
+        "gvs_net_frame_bytes": (C.c_int, [I32, I32, P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name, None)
@@ -128,7 +135,7 @@ def exported_symbols():
              "gvs_search_graph_launch", "gvs_search_graph_destroy", "gvs_device_copy", "gvs_stats_per_query", "gvs_search_plan", "gvs_hash_murmur2_keylen4_device", "gvs_bruteforce_f64_device", "gvs_server_open",
              "gvs_server_submit", "gvs_server_submit_batch", "gvs_server_poll", "gvs_server_poll_stats",
              "gvs_server_poll_status", "gvs_server_counters", "gvs_server_loadgen_breakdown", "gvs_server_cancel",
-             "gvs_server_stats", "gvs_server_close", "gvs_server_loadgen"]
+             "gvs_server_stats", "gvs_server_close", "gvs_server_loadgen", "gvs_net_serve", "gvs_net_frame_bytes"]
     return [n for n in names if hasattr(lib, n)]
 
 

@@ -365,6 +365,38 @@ GVS_API int gvs_server_loadgen(gvs_server *s, const float *queries, int64_t nq, 
                                uint64_t seed, const int32_t *ks, const int32_t *efs, int64_t *labels,
                                float *dists, double *latency_us, double *wall_s);
 
+/* Network front end with the reference's wire format
+ * (networked_FPGA/CPU_programs/README.md:40-72): the node side of the
+ * two-connection protocol that networked_FPGA/CPU_programs/CPU_client.cpp
+ * drives (:450-620), standing where the FPGA network kernel does
+ * (networked_FPGA/kernel/user_krnl/FPGA_inter_query_v1_3/src/hls/
+ * FPGA_inter_query_v1_3.cpp:64-100, :187-230).  It listens for the client's
+ * query connection on c2f_port (0 = any free port, written to *bound_port as
+ * soon as it listens), connects to the client's result listener at
+ * f2c_host:f2c_port, and serves until a batch header with batch_size <= 0:
+ * per batch a 64-byte header (int32 batch_size first), then batch_size
+ * queries of ceil(dim/16) 64-byte packets (fp32, zero padded); per query, in
+ * arrival order, a 64-byte header (int32 ef first), ceil(ef/16) packets of
+ * int32 ids and as many of fp32 distances, ascending.  Queries are searched
+ * on the open server s with k = ef as they arrive; one that overflows its
+ * visited table is re-run on the batch path.  Ids are the index's labels
+ * (its node ids when it was created without labels, which is what the FPGA
+ * returns).  stats (may be NULL): batches, queries, replies, re-runs, bytes
+ * in, bytes out.  Blocks until the session ends; GVS_ERR_IO on a socket
+ * error or a connection closed mid-session. */
+typedef struct gvs_net_config {
+    int32_t c2f_port;          /* port the client sends queries to (0 = any) */
+    const char *f2c_host;      /* client address that receives results */
+    int32_t f2c_port;          /* client port that receives results */
+    int32_t dim;               /* query dimension on the wire */
+    int32_t ef;                /* results per query (<= the server's max_k and max_ef) */
+    int32_t timeout_ms;        /* connection set-up limit (0 = 10 s) */
+} gvs_net_config;
+GVS_API int gvs_net_serve(gvs_server *s, const gvs_net_config *cfg, int32_t *bound_port, int64_t stats[6]);
+/* Wire sizes for dim and ef: [0] batch header, [1] one query, [2] one
+ * reply (header + ids + distances), in bytes. */
+GVS_API int gvs_net_frame_bytes(int32_t dim, int32_t ef, int64_t out[3]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -567,3 +567,14 @@ def test_fpga_layout_roundtrip_and_bytes(tmp_path, nc):
     assert fh["entry"] == fa["entry"] and fh["max_level"] == 0
     with pytest.raises(gvs.GvsError):
         gvs.HostGraph.load_fpga(fd, 1, 24, nc + 1)  # wrong channel count: files missing
+
+
+def test_net_frame_sizes_follow_the_reference_packets():
+    """Wire sizes of the network front end: 64-byte packets of 16 values,
+    as networked_FPGA/CPU_programs/CPU_client.cpp:183-203 sizes them."""
+    from gvs import net
+    for dim, ef in ((128, 64), (100, 10), (96, 16), (384, 512), (1, 1)):
+        hdr, qb, rb = net.frame_bytes(dim, ef)
+        assert hdr == 64
+        assert qb == 64 * (dim // 16 + (dim % 16 != 0))
+        assert rb == 64 + 2 * 64 * (ef // 16 + (ef % 16 != 0))
