@@ -590,6 +590,7 @@ void fill_params(const gvs_index *ix, const gvs_params *p, const LaunchPlan &lp,
     P.max_level = ix->max_level;
     P.entry = ix->entry;
     P.labels = ix->d_labels;
+    P.n = ix->n;
     P.mc = p->mc;
     P.mg = p->mg;
     P.rerank = p->rerank;
@@ -752,12 +753,34 @@ static int64_t batch_waves(const gvs_params *p, int64_t nq)
 
 }  // namespace gvs
 
+// An initial candidate pool per query (the NSG search's start): mode 1 =
+// len ids per query row at ids + row * ld, mode 2 = the entry's first
+// neighbours.
+struct PoolInit {
+    int mode = 0;
+    const int32_t *ids = nullptr;
+    int len = 0;
+    int64_t ld = 0;
+};
+
 static int search_impl(gvs_index *ix, const float *queries, int64_t nq, int64_t ldq, const int32_t *entries,
-                       const gvs_params *p, int64_t *labels, float *dists, int32_t *stats, hipStream_t stream)
+                       const gvs_params *p, int64_t *labels, float *dists, int32_t *stats, hipStream_t stream,
+                       const PoolInit *pi = nullptr)
 {
     if (!ix || !p || nq < 0 || (nq > 0 && (!queries || !labels || !dists)) || ldq < ix->dim) {
         gvs::set_error("gvs_search: invalid argument");
         return GVS_ERR_ARG;
+    }
+    if (pi && pi->mode) {
+        if (entries || p->waves_per_query > 1 || (p->flags & GVS_FLAG_REGION_ORDER)) {
+            gvs::set_error("gvs_search_pool: an initial pool excludes entries, several waves per query and "
+                           "region order");
+            return GVS_ERR_UNSUPPORTED;
+        }
+        if (pi->mode == 1 && (!pi->ids || pi->len < 1 || pi->len > std::max(p->ef, p->k) || pi->ld < pi->len)) {
+            gvs::set_error("gvs_search_pool: need 1 <= init_len <= max(ef, k) and init_ld >= init_len");
+            return GVS_ERR_ARG;
+        }
     }
     if (nq == 0) return GVS_OK;
     if (nq > INT32_MAX) {
@@ -777,6 +800,12 @@ static int search_impl(gvs_index *ix, const float *queries, int64_t nq, int64_t 
     P.out_labels = labels;
     P.out_dists = dists;
     P.out_stats = stats;
+    if (pi && pi->mode) {
+        P.init_mode = pi->mode;
+        P.init_ids = pi->ids;
+        P.init_len = pi->len;
+        P.init_ld = pi->ld;
+    }
     rc = gvs::launch_search_ix(ix, p, lp, P, nq, stream);
     if (rc) return rc;
     if (p->flags & GVS_FLAG_ASYNC) return GVS_OK;
@@ -838,6 +867,11 @@ static int search_impl(gvs_index *ix, const float *queries, int64_t nq, int64_t 
             P2.out_labels = t_lab;
             P2.out_dists = t_d;
             P2.out_stats = t_st;
+            // the pool rows follow the query rows (q_index)
+            P2.init_mode = P.init_mode;
+            P2.init_ids = P.init_ids;
+            P2.init_len = P.init_len;
+            P2.init_ld = P.init_ld;
             ok = gvs::launch_search_ix(ix, &p2, lp2, P2, m, stream) == GVS_OK &&
                  hip_ok(hipStreamSynchronize(stream), "sync retry") &&
                  hip_ok(hipMemcpy(st2.data(), ix->d_status, sizeof(int32_t) * m, hipMemcpyDeviceToHost),
@@ -1215,6 +1249,62 @@ extern "C" int gvs_search_graph_destroy(gvs_search_graph *g)
     if (!g) return GVS_ERR_ARG;
     free_search_graph(g);
     return GVS_OK;
+}
+
+extern "C" int gvs_search_pool_device(gvs_index *ix, const float *queries, int64_t nq, int64_t ldq,
+                                      const int32_t *init_ids, int32_t init_len, int64_t init_ld,
+                                      const gvs_params *p, int64_t *labels, float *dists, int32_t *stats,
+                                      void *stream)
+{
+    PoolInit pi;
+    pi.mode = init_ids ? 1 : 2;
+    pi.ids = init_ids;
+    pi.len = init_len;
+    pi.ld = init_ld;
+    return search_impl(ix, queries, nq, ldq, nullptr, p, labels, dists, stats, (hipStream_t)stream, &pi);
+}
+
+extern "C" int gvs_search_pool_host(gvs_index *ix, const float *queries, int64_t nq, int64_t ldq,
+                                    const int32_t *init_ids, int32_t init_len, int64_t init_ld, const gvs_params *p,
+                                    int64_t *labels, float *dists, int32_t *stats)
+{
+    if (!ix || !p || nq < 0 || (nq > 0 && (!queries || !labels || !dists)) || ldq < ix->dim || p->k < 1 ||
+        (init_ids && (init_len < 1 || init_ld < init_len))) {
+        gvs::set_error("gvs_search_pool_host: invalid argument");
+        return GVS_ERR_ARG;
+    }
+    if (nq == 0) return GVS_OK;
+    const int k = p->k;
+    float *dq = nullptr;
+    int32_t *di = nullptr, *ds = nullptr;
+    int64_t *dl = nullptr;
+    float *dd = nullptr;
+    int rc = GVS_ERR_HIP;
+    bool ok = hip_ok(hipMalloc(&dq, sizeof(float) * nq * ldq), "hipMalloc queries") &&
+              hip_ok(hipMemcpy(dq, queries, sizeof(float) * nq * ldq, hipMemcpyHostToDevice), "copy queries") &&
+              hip_ok(hipMalloc(&dl, sizeof(int64_t) * nq * k), "hipMalloc labels") &&
+              hip_ok(hipMalloc(&dd, sizeof(float) * nq * k), "hipMalloc dists");
+    if (ok && init_ids)
+        ok = hip_ok(hipMalloc(&di, sizeof(int32_t) * nq * init_ld), "hipMalloc pool") &&
+             hip_ok(hipMemcpy(di, init_ids, sizeof(int32_t) * nq * init_ld, hipMemcpyHostToDevice), "copy pool");
+    if (ok && stats) ok = hip_ok(hipMalloc(&ds, sizeof(int32_t) * nq * GVS_STATS_ROW), "hipMalloc stats");
+    if (ok) {
+        rc = gvs_search_pool_device(ix, dq, nq, ldq, di, init_len, init_ld, p, dl, dd, ds, nullptr);
+        if (rc == GVS_OK) {
+            ok = hip_ok(hipDeviceSynchronize(), "hipDeviceSynchronize") &&
+                 hip_ok(hipMemcpy(labels, dl, sizeof(int64_t) * nq * k, hipMemcpyDeviceToHost), "copy labels") &&
+                 hip_ok(hipMemcpy(dists, dd, sizeof(float) * nq * k, hipMemcpyDeviceToHost), "copy dists") &&
+                 (!stats || hip_ok(hipMemcpy(stats, ds, sizeof(int32_t) * nq * GVS_STATS_ROW, hipMemcpyDeviceToHost),
+                                    "copy stats"));
+            if (!ok) rc = GVS_ERR_HIP;
+        }
+    }
+    (void)hipFree(dq);
+    (void)hipFree(di);
+    (void)hipFree(ds);
+    (void)hipFree(dl);
+    (void)hipFree(dd);
+    return rc;
 }
 
 extern "C" int gvs_search_host(gvs_index *ix, const float *queries, int64_t nq, int64_t ldq,

@@ -106,6 +106,15 @@ struct KParams {
     int key_bits;              // region key: bits per node id (level-3 node above level-2 node)
     int xcd_queues;            // 8 work queues (counters[4..11]), one per XCD segment of the order
     int out_k_only;            // write only the query's own k results (rows keep stride k above)
+    // Initial candidate pool instead of the descent and one entry (the NSG
+    // search's start): init_mode 1 = init_len ids per query at
+    // init_ids + query row * init_ld (-1 = none), 2 = the first
+    // min(ef, r0) neighbours of `entry`; ids outside [0, n) are skipped.
+    int init_mode;
+    int init_len;
+    int64_t init_ld;
+    const int32_t *init_ids;
+    int64_t n;
 };
 
 __device__ __forceinline__ uint32_t ord_f(float f)

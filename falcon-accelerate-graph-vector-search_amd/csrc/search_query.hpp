@@ -573,7 +573,7 @@ __device__ __forceinline__ void find_entry(const KParams &P, int64_t qi, const W
 // then label -1 / +inf and its status word is 1).
 template <int NCH, int STORE, int METRIC, int MROWS>
 __device__ bool search_query_req(const KParams &P, int64_t qi, int64_t slot, unsigned char *lds, int k, int ef,
-                                 const float *Q, bool sys_query)
+                                 const float *Q, bool sys_query, const int32_t *init = nullptr, int init_len = 0)
 {
     constexpr int E = StoreTraits<STORE>::E;
     const int lane = lane_id();
@@ -596,9 +596,9 @@ __device__ bool search_query_req(const KParams &P, int64_t qi, int64_t slot, uns
         }
 
     int up_hops = 0, up_evals = 0;
-    uint32_t ep;
-    float epd;
-    find_entry<NCH, STORE, METRIC>(P, qi, L, qf, ep, epd, up_hops, up_evals);
+    uint32_t ep = 0u;
+    float epd = 0.0f;
+    if (!init) find_entry<NCH, STORE, METRIC>(P, qi, L, qf, ep, epd, up_hops, up_evals);
 
     PROF_MARK(0);
     // layer-0 search, HNSWLIB/hnswalg.h:240-326; groups of up to mc
@@ -631,7 +631,10 @@ __device__ bool search_query_req(const KParams &P, int64_t qi, int64_t slot, uns
         for (uint32_t w = lane; w < L.nbw; w += WAVE) L.bloom[w] = 0ull;
     }
     wave_sync();
-    if (lane == 0) {
+    if (init) {
+        np = 0;
+        evals = 0;
+    } else if (lane == 0) {
         pool[0] = mk_key(epd, ep);
         if (lds_mode && P.hnarrow) {
             uint32_t home, q;
@@ -649,7 +652,50 @@ __device__ bool search_query_req(const KParams &P, int64_t qi, int64_t slot, uns
     }
     wave_sync();
 
+    // After an expansion: a query whose visited set passed the table's limit
+    // freezes the LDS hash and goes on with the HBM table, or overflows.
+    const auto table_check = [&]() -> bool {
+        if (evals <= tlimit) return true;
+        if (lds_mode && P.vtab) {
+            lds_mode = false;
+            frozen = L.vh;
+            T = P.vtab + ((size_t)slot << P.vlog2);
+            tlimit = evals + (int)((3u << P.vlog2) >> 2) - P.lcap;
+            for (uint32_t w = lane; w < L.nbw; w += WAVE) L.bloom[w] = 0ull;
+            if (lane == 0) atomicAdd(&P.counters[2], 1u);
+            wave_sync();
+            return true;
+        }
+        return false;
+    };
+
     int nres = 0;
+    // Initial pool (the NSG search's start, nsg.py:66-90): the listed ids are
+    // visited and evaluated in chunks of MROWS rows, each merged into the
+    // pool as a completed group would be.
+    if (init) {
+        for (int c = 0; c < init_len && !overflow; c += MROWS * WAVE) {
+            int32_t ids[MROWS];
+#pragma unroll
+            for (int r = 0; r < MROWS; r++) {
+                const int j = c + r * WAVE + lane;
+                const int32_t id = j < init_len ? init[j] : -1;
+                ids[r] = (id >= 0 && (int64_t)id < P.n) ? id : -1;
+            }
+            const int nrows = min(MROWS, (init_len - c + WAVE - 1) / WAVE);
+            if (lds_mode)
+                nres = expand_lds<NCH, STORE, METRIC, MROWS>(P, L, qf, ids, nrows, evals EXP_PROF_ARGS);
+            else
+                nres = expand_group<NCH, STORE, METRIC, MROWS>(P, L, qf, T, ids, nrows, evals,
+                                                               frozen EXP_PROF_ARGS);
+            if (nres < 0) {
+                overflow = true;
+                break;
+            }
+            np = merge_into_pool(pool, pool2, np, ef, L.lkeys, nres, hint);
+            if (!table_check()) overflow = true;
+        }
+    }
     bool haveQ = false;
     // DST pipeline: up to mg groups in flight, i.e. selected and not yet
     // merged.  Selected groups wait in a ring of mg lists of up to 8 ids in
@@ -662,7 +708,7 @@ __device__ bool search_query_req(const KParams &P, int64_t qi, int64_t slot, uns
     int gh = 0, pend = 0, inflight = 0;
     bool pLoaded = false;
     int32_t idsP[MROWS], idsN[MROWS];
-    for (;;) {
+    for (; !overflow;) {
         if (haveQ) {  // complete the oldest group
             np = merge_into_pool(pool, pool2, np, ef, L.lkeys, nres, hint);
             haveQ = false;
@@ -702,19 +748,9 @@ __device__ bool search_query_req(const KParams &P, int64_t qi, int64_t slot, uns
         gh = gn;
 #pragma unroll
         for (int r = 0; r < MROWS; r++) idsP[r] = idsN[r];
-        if (evals > tlimit) {
-            if (lds_mode && P.vtab) {
-                lds_mode = false;
-                frozen = L.vh;
-                T = P.vtab + ((size_t)slot << P.vlog2);
-                tlimit = evals + (int)((3u << P.vlog2) >> 2) - P.lcap;
-                for (uint32_t w = lane; w < L.nbw; w += WAVE) L.bloom[w] = 0ull;
-                if (lane == 0) atomicAdd(&P.counters[2], 1u);
-                wave_sync();
-            } else {
-                overflow = true;
-                break;
-            }
+        if (!table_check()) {
+            overflow = true;
+            break;
         }
     }
 
@@ -808,7 +844,12 @@ __device__ __forceinline__ void search_query(const KParams &P, int64_t qi, int64
     const int64_t qrow = P.q_index ? P.q_index[qi] : qi;
     const int k = P.q_k ? P.q_k[qi] : P.k;
     const int ef = P.q_ef ? P.q_ef[qi] : P.ef;
-    search_query_req<NCH, STORE, METRIC, MROWS>(P, qi, slot, lds, k, ef, P.queries + qrow * P.ldq, false);
+    const int32_t *init = P.init_mode == 1   ? P.init_ids + qrow * P.init_ld
+                          : P.init_mode == 2 ? P.adj0 + (size_t)P.entry * P.r0
+                                             : nullptr;
+    const int init_len = P.init_mode == 1 ? P.init_len : P.init_mode == 2 ? min(P.r0, max(ef, k)) : 0;
+    search_query_req<NCH, STORE, METRIC, MROWS>(P, qi, slot, lds, k, ef, P.queries + qrow * P.ldq, false, init,
+                                                init_len);
 }
 
 // ---- several waves per query (intra-query parallelism) -------------------

@@ -110,6 +110,8 @@ def _load():
         "gvs_server_stats": (C.c_int, [P, P]),
         "gvs_server_close": (C.c_int, [P]),
         "gvs_net_serve": (C.c_int, [P, P, P, P]),
+        "gvs_search_pool_device": (C.c_int, [P, P, I64, I64, P, I32, I64, P, P, P, P, P]),
+        "gvs_search_pool_host": (C.c_int, [P, P, I64, I64, P, I32, I64, P, P, P, P]),
         "gvs_net_frame_bytes": (C.c_int, [I32, I32, P]),
     }
     for name, (res, args) in sig.items():
@@ -135,7 +137,7 @@ def exported_symbols():
              "gvs_search_graph_launch", "gvs_search_graph_destroy", "gvs_device_copy", "gvs_stats_per_query", "gvs_search_plan", "gvs_hash_murmur2_keylen4_device", "gvs_bruteforce_f64_device", "gvs_server_open",
              "gvs_server_submit", "gvs_server_submit_batch", "gvs_server_poll", "gvs_server_poll_stats",
              "gvs_server_poll_status", "gvs_server_counters", "gvs_server_loadgen_breakdown", "gvs_server_cancel",
-             "gvs_server_stats", "gvs_server_close", "gvs_server_loadgen", "gvs_net_serve", "gvs_net_frame_bytes"]
+             "gvs_server_stats", "gvs_server_close", "gvs_server_loadgen", "gvs_net_serve", "gvs_net_frame_bytes", "gvs_search_pool_device", "gvs_search_pool_host"]
     return [n for n in names if hasattr(lib, n)]
 
 
@@ -301,6 +303,27 @@ class DeviceIndex:
         p = make_params(k=k, ef=ef, **kw)
         _check(lib.gvs_search_host(self._h, _ptr(q), nq, q.shape[1], _ptr(ent), C.byref(p), _ptr(lab),
                                    _ptr(dist), _ptr(st)))
+        return (lab, dist, st[:, :STATS_PER_QUERY]) if with_stats else (lab, dist)
+
+    def search_pool(self, queries, k=10, ef=64, init=None, with_stats=False, **kw):
+        """NSG search from an initial pool per query (init: [nq, L] or [L]
+        ids, -1 = none; None = the entry's first neighbours)."""
+        q = np.ascontiguousarray(queries, dtype=np.float32)
+        if q.ndim == 1:
+            q = q[None, :]
+        nq = q.shape[0]
+        ini, ln, ld = None, 0, 0
+        if init is not None:
+            ini = np.ascontiguousarray(init, dtype=np.int32)
+            if ini.ndim == 1:
+                ini = np.ascontiguousarray(np.tile(ini, (nq, 1)))
+            ln = ld = ini.shape[1]
+        lab = np.empty((nq, k), dtype=np.int64)
+        dist = np.empty((nq, k), dtype=np.float32)
+        st = np.empty((nq, STATS_ROW), dtype=np.int32) if with_stats else None
+        p = make_params(k=k, ef=ef, **kw)
+        _check(lib.gvs_search_pool_host(self._h, _ptr(q), nq, q.shape[1], _ptr(ini), ln, ld, C.byref(p), _ptr(lab),
+                                        _ptr(dist), _ptr(st)))
         return (lab, dist, st[:, :STATS_PER_QUERY]) if with_stats else (lab, dist)
 
     def search_device(self, q_ptr, nq, ldq, labels_ptr, dists_ptr, k=10, ef=64, entries_ptr=None,

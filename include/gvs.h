@@ -278,6 +278,28 @@ GVS_API int gvs_search_host(gvs_index *ix, const float *queries, int64_t nq, int
                             const int32_t *entries, const gvs_params *p,
                             int64_t *labels, float *dists, int32_t *stats);
 
+/* NSG search from an initial candidate pool (IndexNSG::search_with_base_graph,
+ * vector_search_baselines/scripts_nsg/nsg.py:62-125), L_search = max(ef, k):
+ * instead of the upper-layer descent from one entry, query i starts from the
+ * init_len ids at init_ids + i * init_ld (device buffer; -1 = none, ids
+ * outside [0, n) skipped), each marked visited and evaluated, and the search
+ * then expands the best unexpanded of the ef best until none is left (with
+ * DST mc / mg as p gives them; nsg.py expands one at a time, mc = mg = 1).
+ * init_ids NULL: the first min(ef, r0) neighbours of the graph's entry (the
+ * navigating node), which is nsg.py's start without the random ids that the
+ * reference's NSG fork removed (vector_search_baselines/scripts_nsg/README_nsg.md:5);
+ * a caller that wants nsg.py's random fill passes those ids in the pool.
+ * Requires 1 <= init_len <= max(ef, k); one wave per query, no entries, no
+ * region order.  Otherwise as gvs_search_device. */
+GVS_API int gvs_search_pool_device(gvs_index *ix, const float *queries, int64_t nq, int64_t ldq,
+                                   const int32_t *init_ids, int32_t init_len, int64_t init_ld,
+                                   const gvs_params *p, int64_t *labels, float *dists, int32_t *stats,
+                                   void *stream);
+/* gvs_search_pool_device with host buffers, synchronous. */
+GVS_API int gvs_search_pool_host(gvs_index *ix, const float *queries, int64_t nq, int64_t ldq,
+                                 const int32_t *init_ids, int32_t init_len, int64_t init_ld, const gvs_params *p,
+                                 int64_t *labels, float *dists, int32_t *stats);
+
 /* Exact FP64 brute-force kNN on the device (MFMA f64 GEMM + fused top-k):
  * device queries [nq x dim] (stride ldq), out ids (int32) and FP64 dists,
  * ascending, ties by id.  The reference loads ground truth from files

@@ -335,7 +335,8 @@ static int drain_sorted(heap_t *top, int32_t *out_ids, float *out_dists)
 static int layer0_impl(const orc_graph *g, const float *q, int32_t ep, int ef,
                        int mc, int mg, int order,
                        int32_t *out_ids, float *out_dists, orc_stats *st,
-                       int32_t *trace_ids, int32_t *trace_group, int trace_cap, int *ntrace)
+                       int32_t *trace_ids, int32_t *trace_group, int trace_cap, int *ntrace,
+                       const int32_t *init, int init_len)
 {
     int64_t ngroups = 0;
     int nt = 0;
@@ -344,14 +345,34 @@ static int layer0_impl(const orc_graph *g, const float *q, int32_t ep, int ef,
     if (mg < 1) mg = 1;
     visited_t *vis = visited_get(g->n);
     heap_t top, cand;
-    heap_init(&top, ef + 1, 1);
-    heap_init(&cand, 4 * ef, 0);
+    heap_init(&top, ef + 1 + (init ? init_len : 0), 1);
+    heap_init(&cand, 4 * ef + (init ? init_len : 0), 0);
+    int64_t hops = 0, evals = 0;
 
-    item_t e = { orc_dist(g, q, ep, order), ep };
-    heap_push(&top, e);
-    heap_push(&cand, e);
-    vis->tag[ep] = vis->epoch;
-    int64_t hops = 0, evals = 1;
+    if (!init) {
+        item_t e = { orc_dist(g, q, ep, order), ep };
+        heap_push(&top, e);
+        heap_push(&cand, e);
+        vis->tag[ep] = vis->epoch;
+        evals = 1;
+    } else {
+        /* an initial pool instead of one entry: every listed id (in range,
+         * first occurrence) is marked visited, evaluated and offered to the
+         * result set as a completed group would be (ids outside [0, n) are
+         * skipped) */
+        for (int i = 0; i < init_len; i++) {
+            int32_t x = init[i];
+            if (x < 0 || x >= g->n || vis->tag[x] == vis->epoch) continue;
+            vis->tag[x] = vis->epoch;
+            item_t r = { orc_dist(g, q, x, order), x };
+            evals++;
+            if (top.n < ef || key_lt(r, top.v[0])) {
+                heap_push(&cand, r);
+                heap_push(&top, r);
+                if (top.n > ef) heap_pop(&top);
+            }
+        }
+    }
 
     group_t *fifo = (group_t *)calloc((size_t)mg, sizeof(group_t));
     for (int i = 0; i < mg; i++) {
@@ -426,7 +447,7 @@ int orc_search_layer0(const orc_graph *g, const float *q, int32_t ep, int ef,
                       int mc, int mg, int order,
                       int32_t *out_ids, float *out_dists, orc_stats *st)
 {
-    return layer0_impl(g, q, ep, ef, mc, mg, order, out_ids, out_dists, st, NULL, NULL, 0, NULL);
+    return layer0_impl(g, q, ep, ef, mc, mg, order, out_ids, out_dists, st, NULL, NULL, 0, NULL, NULL, 0);
 }
 
 /* HNSWLIB/hnswalg.h:1114-1168 searchKnn, in closer-first order
@@ -442,7 +463,8 @@ int orc_search_knn_trace(const orc_graph *g, const float *q, int k, int ef,
     int efe = ef > k ? ef : k;
     int32_t *ids = (int32_t *)malloc(sizeof(int32_t) * (size_t)efe);
     float *ds = (float *)malloc(sizeof(float) * (size_t)efe);
-    int n = layer0_impl(g, q, ep, efe, mc, mg, order, ids, ds, st, trace_ids, trace_group, trace_cap, ntrace);
+    int n = layer0_impl(g, q, ep, efe, mc, mg, order, ids, ds, st, trace_ids, trace_group, trace_cap, ntrace,
+                        NULL, 0);
     int m = n < k ? n : k;
     memcpy(out_ids, ids, sizeof(int32_t) * (size_t)m);
     memcpy(out_dists, ds, sizeof(float) * (size_t)m);
@@ -458,6 +480,113 @@ int orc_search_knn(const orc_graph *g, const float *q, int k, int ef,
 {
     return orc_search_knn_trace(g, q, k, ef, mc, mg, order, entry_or_null, out_ids, out_dists, st,
                                 NULL, NULL, 0, NULL);
+}
+
+/* searchKnn from an initial candidate pool instead of the descent: the
+ * layer-0 search (DST mc, mg) seeded with init[0..init_len), trimmed to k. */
+int orc_search_pool(const orc_graph *g, const float *q, int k, int ef, int mc, int mg, int order,
+                    const int32_t *init, int init_len, int32_t *out_ids, float *out_dists, orc_stats *st)
+{
+    if (g->n <= 0 || k <= 0) return 0;
+    int efe = ef > k ? ef : k;
+    int32_t *ids = (int32_t *)malloc(sizeof(int32_t) * (size_t)efe);
+    float *ds = (float *)malloc(sizeof(float) * (size_t)efe);
+    int n = layer0_impl(g, q, -1, efe, mc, mg, order, ids, ds, st, NULL, NULL, 0, NULL, init, init_len);
+    int m = n < k ? n : k;
+    memcpy(out_ids, ids, sizeof(int32_t) * (size_t)m);
+    memcpy(out_dists, ds, sizeof(float) * (size_t)m);
+    for (int i = m; i < k; i++) { out_ids[i] = -1; out_dists[i] = INFINITY; }
+    free(ids);
+    free(ds);
+    return m;
+}
+
+/* IndexNSG::search_with_base_graph, vector_search_baselines/scripts_nsg/nsg.py:62-125,
+ * restated on its own data structure: retset, an array of (dist, id, flag)
+ * kept sorted, holding at most L + 1 entries (nsg.py:113-115).  The initial
+ * ids (nsg.py:66-80: the navigating node's first neighbours, then random
+ * ids not yet flagged) are given by the caller; each is flagged, evaluated
+ * and entered unexpanded (nsg.py:82-90).  Then k walks the first L
+ * positions: an unexpanded entry is expanded, each unflagged neighbour is
+ * flagged and evaluated, and kept if below the L-th entry (nsg.py:102-112);
+ * k moves back to the smallest position an insertion took, else on by one
+ * (nsg.py:117-121).  Order is by (dist, id): nsg.py orders by distance only
+ * (insort_left puts a new entry before equal ones, and an entry equal to the
+ * L-th is not kept); the two agree when distances are distinct.  With fewer
+ * than L entries every new one is kept (nsg.py always starts with L). */
+int orc_nsg_search(const orc_graph *g, const float *q, int K, int L, const int32_t *init, int init_len,
+                   int order, int32_t *out_ids, float *out_dists, orc_stats *st)
+{
+    if (L < 1) L = 1;
+    visited_t *vis = visited_get(g->n);
+    item_t *ret = (item_t *)malloc(sizeof(item_t) * (size_t)(L + 2 + init_len));
+    unsigned char *flag = (unsigned char *)malloc((size_t)(L + 2 + init_len));
+    int rn = 0;
+    int64_t evals = 0, hops = 0;
+    for (int i = 0; i < init_len; i++) {
+        int32_t x = init[i];
+        if (x < 0 || x >= g->n || vis->tag[x] == vis->epoch) continue;
+        vis->tag[x] = vis->epoch;
+        item_t r = { orc_dist(g, q, x, order), x };
+        evals++;
+        /* insertion sort keeps the array ordered (nsg.py sorts once) */
+        int pos = rn;
+        while (pos > 0 && key_lt(r, ret[pos - 1])) {
+            ret[pos] = ret[pos - 1];
+            flag[pos] = flag[pos - 1];
+            pos--;
+        }
+        ret[pos] = r;
+        flag[pos] = 1;
+        rn++;
+    }
+    int k = 0;
+    while (k < L) {
+        int nk = L;
+        if (k < rn && flag[k]) {
+            flag[k] = 0;
+            int32_t n = ret[k].id;
+            hops++;
+            const int32_t *nb = g->adj0 + (int64_t)n * g->r0;
+            for (int m = 0; m < g->r0; m++) {
+                int32_t x = nb[m];
+                if (x < 0) break;
+                if (vis->tag[x] == vis->epoch) continue;
+                vis->tag[x] = vis->epoch;
+                item_t r = { orc_dist(g, q, x, order), x };
+                evals++;
+                if (rn >= L && !key_lt(r, ret[L - 1])) continue;
+                int pos = rn;
+                while (pos > 0 && key_lt(r, ret[pos - 1])) {
+                    ret[pos] = ret[pos - 1];
+                    flag[pos] = flag[pos - 1];
+                    pos--;
+                }
+                ret[pos] = r;
+                flag[pos] = 1;
+                rn++;
+                if (rn > L + 1) rn--;
+                if (pos < nk) nk = pos;
+            }
+        }
+        if (nk <= k)
+            k = nk;
+        else
+            k++;
+    }
+    int m = rn < K ? rn : K;
+    if (m > L) m = L;
+    for (int i = 0; i < K; i++) {
+        out_ids[i] = i < m ? ret[i].id : -1;
+        out_dists[i] = i < m ? ret[i].d : INFINITY;
+    }
+    free(ret);
+    free(flag);
+    if (st) {
+        st->hops += hops;
+        st->evals += evals;
+    }
+    return m;
 }
 
 int orc_search_batch_ex(const orc_graph *g, const float *q, int64_t nq, int k, int ef,

@@ -132,6 +132,20 @@ int orc_search_knn_trace(const orc_graph *g, const float *q, int k, int ef,
                          int32_t *out_ids, float *out_dists, orc_stats *st,
                          int32_t *trace_ids, int32_t *trace_group, int trace_cap, int *ntrace);
 
+/* searchKnn from an initial candidate pool (no descent, no single entry):
+ * init[0..init_len) ids are evaluated, marked visited and seeded as the
+ * result set, then the layer-0 search (DST mc, mg) runs with max(ef, k);
+ * the NSG search of vector_search_baselines/scripts_nsg/nsg.py:62-125 at
+ * mc = mg = 1. */
+int orc_search_pool(const orc_graph *g, const float *q, int k, int ef, int mc, int mg, int order,
+                    const int32_t *init, int init_len, int32_t *out_ids, float *out_dists, orc_stats *st);
+
+/* nsg.py:62-125 search_with_base_graph restated on its own sorted retset of
+ * at most L + 1 entries (an independent check of orc_search_pool); K
+ * results, -1 / inf padded. */
+int orc_nsg_search(const orc_graph *g, const float *q, int K, int L, const int32_t *init, int init_len,
+                   int order, int32_t *out_ids, float *out_dists, orc_stats *st);
+
 /* orc_search_batch with stats_width int64 per query: 4 as above, or 5 with
  * the DST group count appended */
 int orc_search_batch_ex(const orc_graph *g, const float *q, int64_t nq, int k, int ef,

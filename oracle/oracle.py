@@ -72,6 +72,10 @@ def _load():
     lib.orc_search_knn_trace.argtypes = [P, P, I, I, I, I, I, P, P, P, P, P, P, I, P]
     lib.orc_rerank_f32.restype = I
     lib.orc_rerank_f32.argtypes = [P, I, I, P, P, I, I, I, P, P]
+    lib.orc_search_pool.restype = I
+    lib.orc_search_pool.argtypes = [P, P, I, I, I, I, I, P, I, P, P, P]
+    lib.orc_nsg_search.restype = I
+    lib.orc_nsg_search.argtypes = [P, P, I, I, P, I, I, P, P, P]
     return lib
 
 
@@ -153,6 +157,51 @@ class OracleGraph:
                 groups.append([])
             groups[tgr[i]].append(int(tid[i]))
         return ids, ds, st, groups
+
+
+class _Stats(C.Structure):
+    _fields_ = [(n, C.c_int64) for n in ("hops", "evals", "up_hops", "up_evals", "groups")]
+
+
+def _init_rows(init, nq):
+    init = np.ascontiguousarray(init, dtype=np.int32)
+    if init.ndim == 1:
+        init = np.tile(init, (nq, 1))
+    return init
+
+
+def search_pool(og, queries, k, ef, init, mc=1, mg=1, order=WAVE):
+    """searchKnn from an initial pool per query (init: [nq, L] or [L] ids,
+    -1 = none) -> ids, dists, stats (hops, evals)."""
+    q = pad_rows(queries, og.dim_pad)
+    nq = q.shape[0]
+    init = _init_rows(init, nq)
+    ids = np.empty((nq, k), dtype=np.int32)
+    ds = np.empty((nq, k), dtype=np.float32)
+    st = np.zeros((nq, 2), dtype=np.int64)
+    for i in range(nq):
+        s = _Stats()
+        lib.orc_search_pool(C.byref(og.g), _p(q[i]), k, ef, mc, mg, order, _p(init[i]), init.shape[1], _p(ids[i]),
+                            _p(ds[i]), C.byref(s))
+        st[i] = (s.hops, s.evals)
+    return ids, ds, st
+
+
+def nsg_search(og, queries, K, L, init, order=WAVE):
+    """nsg.py search_with_base_graph restated on a sorted retset -> ids,
+    dists, stats (hops, evals)."""
+    q = pad_rows(queries, og.dim_pad)
+    nq = q.shape[0]
+    init = _init_rows(init, nq)
+    ids = np.empty((nq, K), dtype=np.int32)
+    ds = np.empty((nq, K), dtype=np.float32)
+    st = np.zeros((nq, 2), dtype=np.int64)
+    for i in range(nq):
+        s = _Stats()
+        lib.orc_nsg_search(C.byref(og.g), _p(q[i]), K, L, _p(init[i]), init.shape[1], order, _p(ids[i]), _p(ds[i]),
+                           C.byref(s))
+        st[i] = (s.hops, s.evals)
+    return ids, ds, st
 
 
 def bruteforce_f64(base, queries, k, metric=L2, nthreads=8):

@@ -139,6 +139,90 @@ def test_flat_nsg_style_graph():
     check_equal(fdev, og, q, 10, 64)
 
 
+def _nsg_pools(nsg, nq, L, seed):
+    """nsg.py:66-80's start: the navigating node's first neighbours, then
+    distinct random ids not yet taken (numpy stands in for Python's
+    random.randint) up to L."""
+    a = nsg.arrays()
+    head = [int(v) for v in a["adj0"][a["entry"], :min(L, a["r0"])] if v >= 0]
+    rng = np.random.default_rng(seed)
+    out = np.empty((nq, L), dtype=np.int32)
+    for i in range(nq):
+        taken, ids = set(head), list(head)
+        while len(ids) < L:
+            v = int(rng.integers(0, a["n"]))
+            if v not in taken:
+                taken.add(v)
+                ids.append(v)
+        out[i] = ids
+    return out
+
+
+@pytest.mark.parametrize("ef", [32, 100, 200])
+def test_nsg_pool_search_matches_oracle(ef):
+    """The NSG search (nsg.py:62-125) from the navigating node's neighbours,
+    and from nsg.py's pool with random fill, equals the oracle's pool search
+    and its separate restatement of nsg.py (ids, distances, expansions,
+    evaluations); with DST on top it equals the oracle's DST from the pool."""
+    x, g, dev = graph(20000, 128, 16, 100)
+    nsg = g.to_nsg(64)
+    ndev = gvs.DeviceIndex(nsg)
+    og = orc.OracleGraph(nsg.arrays())
+    q = queries(120, 128, seed=13)
+    a = nsg.arrays()
+    head = a["adj0"][a["entry"], :min(ef, a["r0"])]
+    print(f"case nsg pool ef={ef} default start", flush=True)
+    lab, dist, st = ndev.search_pool(q, k=10, ef=ef, with_stats=True)
+    ids, ds, ost = orc.search_pool(og, q, 10, ef, head)
+    np.testing.assert_array_equal(lab, ids)
+    np.testing.assert_array_equal(dist, ds)
+    np.testing.assert_array_equal(st[:, :2].astype(np.int64), ost)
+    assert np.all(st[:, 2:] == 0)  # no descent
+    pools = _nsg_pools(nsg, 120, ef, seed=ef)
+    print(f"case nsg pool ef={ef} random fill", flush=True)
+    lab, dist, st = ndev.search_pool(q, k=10, ef=ef, init=pools, with_stats=True)
+    for ids, ds, ost in (orc.search_pool(og, q, 10, ef, pools), orc.nsg_search(og, q, 10, ef, pools)):
+        np.testing.assert_array_equal(lab, ids)
+        np.testing.assert_array_equal(dist, ds)
+        np.testing.assert_array_equal(st[:, :2].astype(np.int64), ost)
+    for mc, mg, kw in ((4, 1, {}), (2, 2, {}), (1, 1, dict(persistent=True)), (1, 1, dict(visited="hbm"))):
+        print(f"case nsg pool ef={ef} mc={mc} mg={mg} {kw}", flush=True)
+        lab, dist, st = ndev.search_pool(q, k=10, ef=ef, init=pools, mc=mc, mg=mg, with_stats=True, **kw)
+        ids, ds, ost = orc.search_pool(og, q, 10, ef, pools, mc=mc, mg=mg)
+        np.testing.assert_array_equal(lab, ids)
+        np.testing.assert_array_equal(dist, ds)
+        np.testing.assert_array_equal(st[:, :2].astype(np.int64), ost)
+
+
+def test_pool_search_repeats_out_of_range_and_errors():
+    """Pools with repeated ids, -1 and ids past n (skipped), a pool of one
+    (the entry: the plain search), and the calls it refuses."""
+    x, g, dev = graph(20000, 128, 16, 100)
+    nsg = g.to_nsg(64)
+    ndev = gvs.DeviceIndex(nsg)
+    og = orc.OracleGraph(nsg.arrays())
+    q = queries(60, 128, seed=14)
+    rng = np.random.default_rng(9)
+    pools = rng.integers(-3, 20010, size=(60, 90)).astype(np.int32)
+    pools[:, 5] = pools[:, 0]
+    pools[:, 70] = pools[:, 2]
+    for ef in (90, 200):
+        lab, dist, st = ndev.search_pool(q, k=10, ef=ef, init=pools, with_stats=True)
+        ids, ds, ost = orc.search_pool(og, q, 10, ef, pools)
+        np.testing.assert_array_equal(lab, ids)
+        np.testing.assert_array_equal(dist, ds)
+        np.testing.assert_array_equal(st[:, :2].astype(np.int64), ost)
+    ent = np.full(60, nsg.view.entry, dtype=np.int32)
+    lab, dist = ndev.search_pool(q, k=10, ef=64, init=ent[:, None])
+    lab2, dist2 = ndev.search(q, k=10, ef=64, entries=ent)
+    np.testing.assert_array_equal(lab, lab2)
+    np.testing.assert_array_equal(dist, dist2)
+    with pytest.raises(gvs.GvsError):
+        ndev.search_pool(q, k=10, ef=64, init=pools)  # 90 ids > ef
+    with pytest.raises(gvs.GvsError):
+        ndev.search_pool(q, k=10, ef=64, waves_per_query=2)
+
+
 def test_visited_table_overflow_retry_is_exact():
     x, g, dev = graph(20000, 128, 16, 100)
     og = orc.OracleGraph(g.arrays())

@@ -578,3 +578,87 @@ def test_net_frame_sizes_follow_the_reference_packets():
         assert hdr == 64
         assert qb == 64 * (dim // 16 + (dim % 16 != 0))
         assert rb == 64 + 2 * 64 * (ef // 16 + (ef % 16 != 0))
+
+
+# NSG search from an initial pool (nsg.py:62-125), hand-traced on the second
+# DST fixture with the initial pool [7, 8, 4] (as if the navigating node's
+# first three neighbours), L = K = 3:
+#   the pool is evaluated (3 evaluations): retset [8 49, 4 64, 7 72.25];
+#   expand 8: 6 (56.25) enters at 1, 9 (25) at 0 -> [9, 8, 6] (+ 4 in the
+#     L+1-th place); k goes back to 0;
+#   expand 9: 4 and 6 already flagged; k moves on past 8 (expanded) to 2;
+#   expand 6: 1 (20.25) enters at 0 -> [1, 9, 8]; 0 (100) is not below 49;
+#     9 flagged; k back to 0;
+#   expand 1: 3 (30.25) enters at 2 -> [1, 9, 3]; k moves to 1 (9 expanded),
+#     then 2;
+#   expand 3: 6 and 0 flagged; k reaches L: stop.
+# Results [1, 9, 3], 5 expansions, 3 + 5 = 8 evaluations.  Node 5 (the
+# nearest) sits behind node 2, which is never evaluated.
+NSG_INIT = [7, 8, 4]
+NSG_EXPECT = ([1, 9, 3], 5, 8)
+
+
+def test_nsg_search_hand_traced():
+    og = orc.OracleGraph(dst2_fixture_arrays())
+    q = np.zeros((1, 1), dtype=np.float32)
+    want_ids, want_hops, want_evals = NSG_EXPECT
+    x = np.float32(DST2_X)
+    for fn in (lambda: orc.nsg_search(og, q, 3, 3, NSG_INIT), lambda: orc.search_pool(og, q, 3, 3, NSG_INIT)):
+        ids, ds, st = fn()
+        assert ids[0].tolist() == want_ids
+        np.testing.assert_array_equal(ds[0], x[want_ids] * x[want_ids])
+        assert st[0].tolist() == [want_hops, want_evals]
+
+
+def test_nsg_search_equals_pool_search_and_seeded_hnswlib_search():
+    """Two independent restatements agree: nsg.py's sorted-retset walk and
+    the two-heap layer-0 search seeded with the same pool (ids, distances,
+    expansions, evaluations), over random pools with repeats and ids out of
+    range; with the pool {ep} both equal the hnswlib search from ep, which
+    is pinned to the reference hnswlib."""
+    x = gvs.gen_mixture(3000, 24, 16, 31, 32)
+    q = gvs.gen_mixture(80, 24, 16, 31, 33)
+    g = gvs.HostGraph.build_hnsw(x, M=8, ef_construction=60, nthreads=8)
+    nsg = g.to_nsg(24)
+    og = orc.OracleGraph(nsg.arrays())
+    rng = np.random.default_rng(5)
+    for L in (10, 32, 70):
+        init = rng.integers(-5, 3005, size=(80, L)).astype(np.int32)
+        init[:, 3] = init[:, 1]  # a repeat
+        a = orc.nsg_search(og, q, 10, L, init)
+        b = orc.search_pool(og, q, 10, L, init)
+        for u, v in zip(a, b):
+            np.testing.assert_array_equal(u, v)
+    ep = np.full((80, 1), nsg.view.entry, dtype=np.int32)
+    for L in (16, 48):
+        ids, ds, st = orc.nsg_search(og, q, 10, L, ep)
+        ids2, ds2, st2 = og.search(q, 10, L, entries=ep[:, 0])
+        np.testing.assert_array_equal(ids, ids2)
+        np.testing.assert_array_equal(ds, ds2)
+        np.testing.assert_array_equal(st[:, 0], st2[:, 0])
+        np.testing.assert_array_equal(st[:, 1], st2[:, 1])
+
+
+def test_nsg_search_exhaustive_pool_gives_exact_knn():
+    """L >= N evaluates every node reachable from the pool: the result is
+    the exact kNN among those nodes (FP64 brute force over the set a
+    breadth-first search reaches)."""
+    x = gvs.gen_mixture(400, 8, 4, 41, 42)
+    q = gvs.gen_mixture(20, 8, 4, 41, 43)
+    g = gvs.HostGraph.build_hnsw(x, M=6, ef_construction=40, nthreads=1, exact_order=True)
+    a = g.to_nsg(12).arrays()
+    og = orc.OracleGraph(a)
+    pool = np.arange(0, 400, 37, dtype=np.int32)
+    seen, todo = set(pool.tolist()), list(pool.tolist())
+    while todo:
+        v = todo.pop()
+        for u in a["adj0"][v]:
+            if u >= 0 and int(u) not in seen:
+                seen.add(int(u))
+                todo.append(int(u))
+    reach = np.array(sorted(seen), dtype=np.int32)
+    assert len(reach) >= 50
+    gt, _ = orc.bruteforce_f64(x[reach], q, 10)
+    ids, ds, st = orc.nsg_search(og, q, 10, 400, pool, order=orc.SEQ)
+    np.testing.assert_array_equal(ids, reach[gt])
+    assert np.all(st[:, 1] == len(reach))  # every reachable node evaluated once
