@@ -907,6 +907,16 @@ static int search_impl(gvs_index *ix, const float *queries, int64_t nq, int64_t 
     return GVS_OK;
 }
 
+namespace gvs {
+// The search for the GPU builder: device queries, entries (negative =
+// descend), synchronous retries.
+int search_rows(gvs_index *ix, const float *queries, int64_t nq, int64_t ldq, const int32_t *entries,
+                const gvs_params *p, int64_t *labels, float *dists, hipStream_t stream)
+{
+    return search_impl(ix, queries, nq, ldq, entries, p, labels, dists, nullptr, stream);
+}
+}  // namespace gvs
+
 extern "C" int gvs_search_device(gvs_index *ix, const float *queries, int64_t nq, int64_t ldq,
                                  const int32_t *entries, const gvs_params *p, int64_t *labels, float *dists,
                                  int32_t *stats, void *stream)

@@ -506,8 +506,9 @@ __device__ __forceinline__ void find_entry_path(const KParams &P, int64_t qi, co
                                                 uint32_t &n2, uint32_t &n3)
 {
     const int lane = lane_id();
-    if (P.entries) {
-        ep = (uint32_t)P.entries[qi];
+    const int32_t given = P.entries ? P.entries[qi] : -1;  // a negative entry: descend as without one
+    if (given >= 0) {
+        ep = (uint32_t)given;
         if (lane == 0) L.lids[0] = ep;
         wave_sync();
         eval_rows<NCH, STORE, METRIC>(P, qf, L.lids, 1, L.lkeys);

@@ -57,6 +57,11 @@ class ServerConfig(C.Structure):
         "max_inflight", "max_k", "max_ef", "queue_capacity", "grid_waves", "mc", "storage", "waves_per_request")]
 
 
+class BuildConfig(C.Structure):
+    _fields_ = [("exact_order", C.c_int32), ("max_batch", C.c_int32), ("host_threads", C.c_int32),
+                ("reserved", C.c_int32), ("batch_ratio", C.c_double)]
+
+
 class NetConfig(C.Structure):
     _fields_ = [("c2f_port", C.c_int32), ("f2c_host", C.c_char_p), ("f2c_port", C.c_int32), ("dim", C.c_int32),
                 ("ef", C.c_int32), ("timeout_ms", C.c_int32)]
@@ -110,6 +115,8 @@ def _load():
         "gvs_server_stats": (C.c_int, [P, P]),
         "gvs_server_close": (C.c_int, [P]),
         "gvs_net_serve": (C.c_int, [P, P, P, P]),
+        "gvs_hnsw_build_device": (C.c_int, [P, I64, I32, I32, I32, I32, U64, P, P, P]),
+        "gvs_hnsw_build_batched": (C.c_int, [P, I64, I32, I32, I32, I32, U64, I32, I32, C.c_double, I32, P]),
         "gvs_search_pool_device": (C.c_int, [P, P, I64, I64, P, I32, I64, P, P, P, P, P]),
         "gvs_search_pool_host": (C.c_int, [P, P, I64, I64, P, I32, I64, P, P, P, P]),
         "gvs_net_frame_bytes": (C.c_int, [I32, I32, P]),
@@ -137,7 +144,7 @@ def exported_symbols():
              "gvs_search_graph_launch", "gvs_search_graph_destroy", "gvs_device_copy", "gvs_stats_per_query", "gvs_search_plan", "gvs_hash_murmur2_keylen4_device", "gvs_bruteforce_f64_device", "gvs_server_open",
              "gvs_server_submit", "gvs_server_submit_batch", "gvs_server_poll", "gvs_server_poll_stats",
              "gvs_server_poll_status", "gvs_server_counters", "gvs_server_loadgen_breakdown", "gvs_server_cancel",
-             "gvs_server_stats", "gvs_server_close", "gvs_server_loadgen", "gvs_net_serve", "gvs_net_frame_bytes", "gvs_search_pool_device", "gvs_search_pool_host"]
+             "gvs_server_stats", "gvs_server_close", "gvs_server_loadgen", "gvs_net_serve", "gvs_net_frame_bytes", "gvs_search_pool_device", "gvs_search_pool_host", "gvs_hnsw_build_device", "gvs_hnsw_build_batched"]
     return [n for n in names if hasattr(lib, n)]
 
 
@@ -184,6 +191,33 @@ class HostGraph:
         h = C.c_void_p()
         _check(lib.gvs_hnsw_build(_ptr(x), x.shape[0], x.shape[1], metric, M, ef_construction, seed,
                                   nthreads, int(bool(exact_order)), C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def build_hnsw_gpu(cls, x, M=16, ef_construction=200, seed=100, metric=L2, exact_order=False,
+                       batch_ratio=0.0, max_batch=0, host_threads=0, with_stats=False):
+        """HNSW built on the GPU (gvs_hnsw_build_device); batch_ratio < 0 =
+        batches of one node (hnswlib's sequential insertion)."""
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        h = C.c_void_p()
+        cfg = BuildConfig(int(bool(exact_order)), max_batch, host_threads, 0, batch_ratio)
+        st = (C.c_double * 8)()
+        _check(lib.gvs_hnsw_build_device(_ptr(x), x.shape[0], x.shape[1], metric, M, ef_construction, seed,
+                                         C.byref(cfg), C.byref(h), st))
+        g = cls(h)
+        if with_stats:
+            return g, dict(zip(("seconds", "upper_s", "search_s", "link_s", "batches"), list(st)[:5]))
+        return g
+
+    @classmethod
+    def build_hnsw_batched(cls, x, M=16, ef_construction=200, seed=100, metric=L2, nthreads=8, exact_order=False,
+                           batch_ratio=0.0, max_batch=0):
+        """The GPU builder's batched insertion on host threads
+        (gvs_hnsw_build_batched)."""
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        h = C.c_void_p()
+        _check(lib.gvs_hnsw_build_batched(_ptr(x), x.shape[0], x.shape[1], metric, M, ef_construction, seed,
+                                          nthreads, int(bool(exact_order)), batch_ratio, max_batch, C.byref(h)))
         return cls(h)
 
     @classmethod

@@ -142,6 +142,37 @@ GVS_API const char *gvs_version(void);
 GVS_API int gvs_hnsw_build(const float *x, int64_t n, int32_t dim, int32_t metric,
                            int32_t M, int32_t ef_construction, uint64_t seed,
                            int32_t nthreads, int32_t exact_order, gvs_host_graph **out);
+/* HNSW construction on the GPU, with hnswlib's insertion rules (addPoint,
+ * HNSWLIB/hnswalg.h:982-1112; the reference builds on the CPU,
+ * vector_search_baselines/scripts_hnsw/construct_and_search_hnsw.py:126-148).
+ * Nodes go in index order in batches that grow with the graph: each node of
+ * a batch is inserted against the graph as it stood at the batch start
+ * (layer 0 on the GPU: construction search, heuristic, reverse links grouped
+ * by target, re-pruned when full; levels >= 1 on host threads), so a node
+ * only misses links to the nodes of its own batch.  The levels, M, maxM0 =
+ * 2M and efc = max(ef_construction, M) are hnswlib's; a batch of one node is
+ * addPoint, and with exact_order (hnswlib's scalar summation order) and
+ * batches of one the graph is hnswlib's single-thread graph bit for bit.
+ * Needs 2 <= M <= 32, dim <= 512, efc <= 1024.  stats (may be NULL):
+ * seconds in all, in the host levels, in the layer-0 searches, in
+ * selection + links; the number of batches. */
+typedef struct gvs_build_config {
+    int32_t exact_order;       /* 1: hnswlib's scalar summation order throughout */
+    int32_t max_batch;         /* largest batch (0 = 65536) */
+    int32_t host_threads;      /* threads for the levels >= 1 (0 = 16) */
+    int32_t reserved;
+    double batch_ratio;        /* batch size / nodes inserted (0 = 0.02; < 0 = batches of one) */
+} gvs_build_config;
+GVS_API int gvs_hnsw_build_device(const float *x, int64_t n, int32_t dim, int32_t metric, int32_t M,
+                                  int32_t ef_construction, uint64_t seed, const gvs_build_config *cfg,
+                                  gvs_host_graph **out, double stats[8]);
+/* The batched insertion of gvs_hnsw_build_device run entirely on host
+ * threads (same batch schedule and rules; a check of the GPU builder: with
+ * exact_order both give the same graph).  No reference counterpart beyond
+ * addPoint itself (a batch of one node). */
+GVS_API int gvs_hnsw_build_batched(const float *x, int64_t n, int32_t dim, int32_t metric, int32_t M,
+                                   int32_t ef_construction, uint64_t seed, int32_t nthreads, int32_t exact_order,
+                                   double batch_ratio, int32_t max_batch, gvs_host_graph **out);
 /* NSG-style flat graph from an HNSW graph: one row of <= R neighbours per
  * node and a single navigating entry node, the form the FPGA path searches
  * (vector_search_baselines/scripts_nsg/nsg_to_FPGA.py:56-137: fixed-width

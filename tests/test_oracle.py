@@ -48,6 +48,49 @@ def test_builder_reproduces_reference_graph(name):
     np.testing.assert_array_equal(mine["vectors"], z["x"])
 
 
+@pytest.mark.parametrize("name", CASES)
+def test_batched_builder_with_batches_of_one_is_the_reference_graph(name):
+    """The batched insertion (the GPU builder's rules, run on the host) with
+    batches of one node and hnswlib's scalar order == the reference graph."""
+    z, g = load_case(name)
+    ref = g.arrays()
+    mine = gvs.HostGraph.build_hnsw_batched(z["x"], M=int(z["M"]), ef_construction=int(z["efc"]), seed=100,
+                                            metric=int(z["metric"]), nthreads=4, exact_order=True,
+                                            batch_ratio=-1).arrays()
+    assert mine["entry"] == ref["entry"] and mine["max_level"] == ref["max_level"]
+    np.testing.assert_array_equal(mine["adj0"], ref["adj0"])
+    np.testing.assert_array_equal(mine["upper_off"], ref["upper_off"])
+    np.testing.assert_array_equal(mine["adj_up"], ref["adj_up"])
+
+
+def test_batched_builder_graph_is_sound_and_close_to_sequential():
+    """Batches of 5 % of the graph: rows within their width, no self links or
+    repeats, every node reachable from the entry, and recall at a fixed ef
+    close to the sequential build's."""
+    x = gvs.gen_mixture(6000, 32, 24, 61, 62)
+    q = gvs.gen_mixture(300, 32, 24, 61, 63)
+    gb = gvs.HostGraph.build_hnsw_batched(x, M=12, ef_construction=80, nthreads=8, batch_ratio=0.05)
+    gs = gvs.HostGraph.build_hnsw(x, M=12, ef_construction=80, nthreads=1, exact_order=True)
+    a = gb.arrays()
+    adj = a["adj0"]
+    assert adj.shape == (6000, 24)
+    for v in range(6000):
+        row = adj[v][adj[v] >= 0]
+        assert np.all(adj[v][len(row):] == -1) and v not in row and len(set(row.tolist())) == len(row)
+    seen, todo = {a["entry"]}, [a["entry"]]
+    while todo:
+        v = todo.pop()
+        for u in adj[v]:
+            if u >= 0 and int(u) not in seen:
+                seen.add(int(u))
+                todo.append(int(u))
+    assert len(seen) == 6000
+    gt, _ = orc.bruteforce_f64(x, q, 10)
+    rb = orc.recall_at(orc.OracleGraph(a).search(q, 10, 32)[0], gt, 10)
+    rs = orc.recall_at(orc.OracleGraph(gs.arrays()).search(q, 10, 32)[0], gt, 10)
+    assert rb >= rs - 0.02, (rb, rs)
+
+
 def test_hnswlib_save_roundtrip(tmp_path):
     z, g = load_case(CASES[0])
     p = str(tmp_path / "rt.bin")
