@@ -377,6 +377,10 @@ def main():
     ap.add_argument("--graph-cache", default="",
                     help="hnswlib .bin file: loaded if present, else written after the build")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-latency leg")
+    ap.add_argument("--builder", choices=("cpu", "gpu"), default="cpu",
+                    help="graph construction: the multithreaded CPU builder, or the GPU builder")
+    ap.add_argument("--no-gpu-build", action="store_true",
+                    help="skip the leg that builds the same graph on the GPU and compares its recall")
     ap.add_argument("--intra", action="store_true",
                     help="instead of the throughput run: batch=1 latency of intra-query DST "
                          "(1-8 waves per query, mc 1-8, ef 64-512) against the plain search")
@@ -433,10 +437,13 @@ def main():
     if a.graph_cache and os.path.exists(a.graph_cache):
         g = gvs.HostGraph.load_hnswlib(a.graph_cache, a.dim, metric=metric)
         log(f"[bench] graph loaded from {a.graph_cache}")
+    elif a.builder == "gpu":
+        g, bst = gvs.HostGraph.build_hnsw_gpu(x, M=a.M, ef_construction=a.efc, metric=metric, with_stats=True)
+        log(f"[bench] GPU build {bst}")
     else:
         g = gvs.HostGraph.build_hnsw(x, M=a.M, ef_construction=a.efc, metric=metric, nthreads=a.build_threads)
     build_s = time.time() - t0
-    log(f"[bench] build {build_s:.1f}s")
+    log(f"[bench] build ({a.builder}) {build_s:.1f}s")
     if a.serve_child:
         serve_child(a, gvs, g, q)
         return
@@ -653,6 +660,22 @@ def main():
     if traffic_hbm is None:
         log(f"[bench] no PMC traffic summary for {tuned_cfg} in {a.traffic_file}")
 
+    gpu_build = None
+    if not a.no_gpu_build and a.builder == "cpu":
+        # the same graph built on the GPU: build time and recall at the plain ef
+        del dev
+        t0 = time.time()
+        g2, bst = gvs.HostGraph.build_hnsw_gpu(x, M=a.M, ef_construction=a.efc, with_stats=True)
+        wall_b = time.time() - t0
+        dev2 = gvs.DeviceIndex(g2)
+        rec2 = recall_at(dev2.search(q, k=a.k, ef=plain_ef)[0], gt, a.k)
+        gpu_build = {"seconds": round(wall_b, 2), "phases_s": {kk: round(v, 3) for kk, v in bst.items()
+                                                                 if kk != "seconds"},
+                     "recall_at_10_at_plain_ef": round(rec2, 4), "cpu_built_recall_at_10": round(plain_rec, 4),
+                     "cpu_build_seconds": round(build_s, 1), "cpu_build_threads": a.build_threads}
+        log(f"[bench] GPU build {gpu_build}")
+        del dev2, g2
+
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline(a, g, q, gt, plain_ef, cache)
@@ -681,6 +704,7 @@ def main():
                      "algorithmic_bytes_per_launch": traffic, "kernel_ms": round(kernel_ms, 4)},
         "cpu_baseline": cpu,
         "serving": serving,
+        "gpu_build": gpu_build,
     }
     print(json.dumps(out), flush=True)
 

@@ -110,8 +110,8 @@ extern "C" int gvs_hnsw_build_device(const float *x, int64_t n, int32_t dim, int
     gvs_build_config c{};
     if (cfg) c = *cfg;
     const bool exact = c.exact_order != 0;
-    const double ratio = c.batch_ratio > 0.0 ? c.batch_ratio : (c.batch_ratio < 0.0 ? 0.0 : 0.02);
-    const int64_t max_batch = c.max_batch > 0 ? c.max_batch : 65536;
+    const double ratio = c.batch_ratio > 0.0 ? c.batch_ratio : (c.batch_ratio < 0.0 ? 0.0 : 0.01);
+    const int64_t max_batch = c.max_batch > 0 ? c.max_batch : 32768;
     const int threads = c.host_threads > 0 ? c.host_threads : 16;
     const Clock::time_point t_start = Clock::now();
     double t_upper = 0.0, t_search = 0.0, t_link = 0.0;

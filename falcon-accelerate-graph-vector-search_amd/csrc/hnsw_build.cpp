@@ -616,8 +616,8 @@ extern "C" int gvs_hnsw_build_batched(const float *x, int64_t n, int32_t dim, in
     }
     try {
         UpperBatch ub(x, n, dim, metric, (size_t)M, (size_t)ef_construction, seed, exact_order != 0, true);
-        const double ratio = batch_ratio > 0.0 ? batch_ratio : (batch_ratio < 0.0 ? 0.0 : 0.02);
-        const int64_t mb = max_batch > 0 ? max_batch : 65536;
+        const double ratio = batch_ratio > 0.0 ? batch_ratio : (batch_ratio < 0.0 ? 0.0 : 0.01);
+        const int64_t mb = max_batch > 0 ? max_batch : 32768;
         std::vector<int32_t> l0e;
         for (int64_t a = 0; a < n;) {
             const int64_t size = a == 0 ? 1 : std::max<int64_t>(1, std::min<int64_t>(mb, (int64_t)(a * ratio)));

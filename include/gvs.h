@@ -158,10 +158,12 @@ GVS_API int gvs_hnsw_build(const float *x, int64_t n, int32_t dim, int32_t metri
  * selection + links; the number of batches. */
 typedef struct gvs_build_config {
     int32_t exact_order;       /* 1: hnswlib's scalar summation order throughout */
-    int32_t max_batch;         /* largest batch (0 = 65536) */
+    int32_t max_batch;         /* largest batch (0 = 32768) */
     int32_t host_threads;      /* threads for the levels >= 1 (0 = 16) */
     int32_t reserved;
-    double batch_ratio;        /* batch size / nodes inserted (0 = 0.02; < 0 = batches of one) */
+    double batch_ratio;        /* batch size / nodes inserted (0 = 0.01; < 0 = batches of one).  On
+                                * 1M x 128, M = 32, efc = 200: 0.01 builds in 2.5 s with recall@10 at
+                                * ef 58 within 0.001 of the sequential build, 0.05 in 1.6 s, 0.017 lower */
 } gvs_build_config;
 GVS_API int gvs_hnsw_build_device(const float *x, int64_t n, int32_t dim, int32_t metric, int32_t M,
                                   int32_t ef_construction, uint64_t seed, const gvs_build_config *cfg,
