@@ -248,6 +248,46 @@ def precision_c4(a, gvs, g, x, q):
              "dist_rel_err_mean": float(f"{rel.mean():.3e}"), "dist_rel_err_max": float(f"{rel.max():.3e}")}
         rows.append(r)
         log(f"[c4] {r}")
+    # NSG search from a pool (nsg.py:62-125): the navigating node's first
+    # ef neighbours, and (fp32) the same filled up to ef with random ids as
+    # nsg.py draws them (numpy's generator standing in for Python's random)
+    a_arr = nsg.arrays()
+    head = [int(v) for v in a_arr["adj0"][a_arr["entry"], :ef] if v >= 0]
+    rng = np.random.default_rng(7)
+    fill = np.empty((nq, ef), dtype=np.int32)
+    for i in range(nq):
+        taken, ids = set(head), list(head)
+        while len(ids) < ef:
+            v = int(rng.integers(0, a.n))
+            if v not in taken:
+                taken.add(v)
+                ids.append(v)
+        fill[i] = ids
+    dfill = H.DeviceArray.from_host(fill)
+    for storage, rerank, use_fill in ((gvs.F32, False, False), (gvs.F32, False, True), (gvs.BF16, True, False),
+                                      (gvs.FP8, True, False)):
+        kw = dict(k=a.k, ef=ef, storage=storage, rerank=rerank)
+        pool = dict(init_ptr=dfill.address, init_len=ef, init_ld=ef) if use_fill else {}
+
+        def go(asynchronous):
+            dev.search_pool_device(dq.address, nq, a.dim, dl.address, dd.address, stream=st.handle,
+                                   asynchronous=asynchronous, **pool, **kw)
+        go(False)
+        st.synchronize()
+        lab = dl.to_host()
+        e0, e1 = H.Event(), H.Event()
+        e0.record(st)
+        for _ in range(3):
+            go(True)
+        e1.record(st)
+        e1.synchronize()
+        ms = e0.elapsed_ms(e1) / 3
+        r = {"storage": names[storage], "rerank": rerank,
+             "start": "NSG pool: navigating node's neighbours" + (" + random fill" if use_fill else ""),
+             "recall_at_10": round(recall_at(lab, gt, a.k), 4), "qps": round(nq / (ms / 1e3), 1),
+             "ms_per_batch": round(ms, 3)}
+        rows.append(r)
+        log(f"[c4] {r}")
     vb = dev.device_bytes() if hasattr(dev, "device_bytes") else None
     print(json.dumps({"metric": "recall@10 and distance error vs FP64, low-precision traversal + fp32 rerank",
                       "config": {"workload": f"NSG-style flat graph from HNSW M={a.M} efC={a.efc} layer 0, "

@@ -360,6 +360,17 @@ class DeviceIndex:
                                         _ptr(dist), _ptr(st)))
         return (lab, dist, st[:, :STATS_PER_QUERY]) if with_stats else (lab, dist)
 
+    def search_pool_device(self, q_ptr, nq, ldq, labels_ptr, dists_ptr, k=10, ef=64, init_ptr=None, init_len=0,
+                           init_ld=0, stats_ptr=None, stream=None, **kw):
+        """gvs_search_pool_device with device buffers (init_ptr None = the
+        entry's first neighbours)."""
+        p = make_params(k=k, ef=ef, **kw)
+        _check(lib.gvs_search_pool_device(self._h, C.c_void_p(q_ptr), nq, ldq,
+                                          C.c_void_p(init_ptr) if init_ptr else None, init_len, init_ld,
+                                          C.byref(p), C.c_void_p(labels_ptr), C.c_void_p(dists_ptr),
+                                          C.c_void_p(stats_ptr) if stats_ptr else None,
+                                          C.c_void_p(stream) if stream else None))
+
     def search_device(self, q_ptr, nq, ldq, labels_ptr, dists_ptr, k=10, ef=64, entries_ptr=None,
                       stats_ptr=None, stream=None, **kw):
         """Search with device pointers (e.g. torch tensors' data_ptr())."""
