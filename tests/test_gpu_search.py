@@ -227,8 +227,10 @@ def test_visited_table_overflow_retry_is_exact():
     x, g, dev = graph(20000, 128, 16, 100)
     og = orc.OracleGraph(g.arrays())
     q = queries(100, 128, seed=12)
-    # smallest table: ef=400 overflows it, the library retries with larger tables
+    # smallest table: ef=400 overflows it, the library retries with larger tables;
+    # at ef=1000 the first retry (2^13 slots) overflows again and the second runs
     check_equal(dev, og, q, 10, 400, visited_log2=11, visited="hbm")
+    check_equal(dev, og, q, 10, 1000, visited_log2=11, visited="hbm")
 
 
 @pytest.mark.parametrize("mc", [1, 4])
