@@ -65,9 +65,16 @@ static dfn_t pick_descend(int storage, int metric, int nch, bool seq)
     return nullptr;
 }
 
-static kfn_t pick_kernel(int storage, int metric, int nch, int rows, bool seq)
+static kfn_t pick_kernel(int storage, int metric, int nch, int rows, bool seq, bool pool)
 {
     const bool l2 = metric == GVS_METRIC_L2;
+    if (pool) {
+        if (seq) return nullptr;
+        if (storage == GVS_STORE_F32) return l2 ? gvs_kernel_pool_f32_l2(nch, rows) : gvs_kernel_pool_f32_ip(nch, rows);
+        if (storage == GVS_STORE_BF16) return l2 ? gvs_kernel_pool_bf16_l2(nch, rows) : gvs_kernel_pool_bf16_ip(nch, rows);
+        if (storage == GVS_STORE_FP8) return l2 ? gvs_kernel_pool_fp8_l2(nch, rows) : gvs_kernel_pool_fp8_ip(nch, rows);
+        return nullptr;
+    }
     if (seq) return storage == GVS_STORE_F32 ? (l2 ? gvs_kernel_f32seq_l2(nch, rows) : gvs_kernel_f32seq_ip(nch, rows)) : nullptr;
     if (storage == GVS_STORE_F32) return l2 ? gvs_kernel_f32_l2(nch, rows) : gvs_kernel_f32_ip(nch, rows);
     if (storage == GVS_STORE_BF16) return l2 ? gvs_kernel_bf16_l2(nch, rows) : gvs_kernel_bf16_ip(nch, rows);
@@ -387,7 +394,7 @@ static int fit_waves(const gvs_index *ix, const void *fn, LaunchPlan *lp)
 // max_waves > 0: the launch runs at most that many waves (a server's grid),
 // so resident waves beyond it do not count when the visited hash is sized.
 int plan_search(const gvs_index *ix, const gvs_params *p, int max_ef, int max_k, LaunchPlan *lp,
-                int64_t max_waves = 0)
+                int64_t max_waves = 0, bool pool = false)
 {
     if (!p) {
         set_error("gvs_search: params is NULL");
@@ -431,7 +438,7 @@ int plan_search(const gvs_index *ix, const gvs_params *p, int max_ef, int max_k,
     if (ix->dim_pad % (8 * E) == 0)
         fn = W > 1 ? (seq ? (l2 ? gvs_kernel_mw_f32seq_l2(nch, wrows) : gvs_kernel_mw_f32seq_ip(nch, wrows))
                           : (l2 ? gvs_kernel_mw_f32_l2(nch, wrows) : gvs_kernel_mw_f32_ip(nch, wrows)))
-                   : pick_kernel(p->storage, ix->metric, nch, p->mc * rpc, seq);
+                   : pick_kernel(p->storage, ix->metric, nch, p->mc * rpc, seq, pool);
     if (!fn) {
         set_error("gvs_search: dim %d (padded %d) not compiled for storage %d", ix->dim, ix->dim_pad, p->storage);
         return GVS_ERR_UNSUPPORTED;
@@ -788,7 +795,8 @@ static int search_impl(gvs_index *ix, const float *queries, int64_t nq, int64_t 
         return GVS_ERR_ARG;
     }
     gvs::LaunchPlan lp;
-    int rc = gvs::plan_search(ix, p, p->ef, p->k, &lp, gvs::batch_waves(p, nq));
+    const bool pool = pi && pi->mode;
+    int rc = gvs::plan_search(ix, p, p->ef, p->k, &lp, gvs::batch_waves(p, nq), pool);
     if (rc) return rc;
     KParams P;
     gvs::fill_params(ix, p, lp, P);
@@ -830,7 +838,7 @@ static int search_impl(gvs_index *ix, const float *queries, int64_t nq, int64_t 
         p2.persistent = 0;
         p2.flags = (p->flags & ~GVS_FLAG_LDS_VISITED) | GVS_FLAG_HBM_VISITED;
         gvs::LaunchPlan lp2;
-        rc = gvs::plan_search(ix, &p2, p2.ef, p2.k, &lp2);
+        rc = gvs::plan_search(ix, &p2, p2.ef, p2.k, &lp2, 0, pool);
         if (rc) return rc;
         KParams P2;
         gvs::fill_params(ix, &p2, lp2, P2);

@@ -34,7 +34,7 @@ using namespace gvsk;
 
 namespace gvs {
 int plan_search(const gvs_index *ix, const gvs_params *p, int max_ef, int max_k, LaunchPlan *lp,
-                int64_t max_waves);
+                int64_t max_waves, bool pool = false);
 int ensure_scratch(gvs_index *ix, int64_t slots, int vlog2, int64_t nstatus);
 void fill_params(const gvs_index *ix, const gvs_params *p, const LaunchPlan &lp, KParams &P);
 }  // namespace gvs

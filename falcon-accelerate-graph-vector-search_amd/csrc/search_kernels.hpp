@@ -6,7 +6,7 @@ namespace gvsk {
 
 typedef void (*kfn_t)(KParams);
 
-template <int NCH, int STORE, int METRIC, int MROWS>
+template <int NCH, int STORE, int METRIC, int MROWS, bool POOL = false>
 #ifndef GVS_MIN_WAVES_PER_EU
 #define GVS_MIN_WAVES_PER_EU 1
 #endif
@@ -55,7 +55,7 @@ __global__ __launch_bounds__(256, GVS_MIN_WAVES_PER_EU) void k_search(KParams P)
             qi = P.q_base + gw;
             if (qi >= P.nq) break;
         }
-        search_query<NCH, STORE, METRIC, MROWS>(P, qi, gw, lds);
+        search_query<NCH, STORE, METRIC, MROWS, POOL>(P, qi, gw, lds);
     }
 }
 
@@ -101,13 +101,14 @@ __global__ __launch_bounds__(256) void k_descend(KParams P, int32_t *ep_out, uin
 
 typedef void (*dfn_t)(KParams, int32_t *, uint64_t *, int32_t *);
 
-// rows = mc * ceil(r0 / 64) rounded up to 1, 2 or 8
-template <int NCH, int STORE, int METRIC>
+// rows = mc * ceil(r0 / 64) rounded up to 1, 2 or 8; POOL: the kernels
+// with the initial-pool start (the NSG search)
+template <int NCH, int STORE, int METRIC, bool POOL = false>
 kfn_t pick_rows(int rows)
 {
-    if (rows <= 1) return k_search<NCH, STORE, METRIC, 1>;
-    if (rows <= 2) return k_search<NCH, STORE, METRIC, 2>;
-    return k_search<NCH, STORE, METRIC, 8>;
+    if (rows <= 1) return k_search<NCH, STORE, METRIC, 1, POOL>;
+    if (rows <= 2) return k_search<NCH, STORE, METRIC, 2, POOL>;
+    return k_search<NCH, STORE, METRIC, 8, POOL>;
 }
 
 // One workgroup of P.waves_per_block waves per query (intra-query).
@@ -140,6 +141,13 @@ gvsk::kfn_t gvs_kernel_bf16_ip(int nch, int rows);
 gvsk::kfn_t gvs_kernel_fp8_l2(int nch, int rows);
 gvsk::kfn_t gvs_kernel_fp8_ip(int nch, int rows);
 // upper-layer descent (region order)
+gvsk::kfn_t gvs_kernel_pool_f32_l2(int nch, int rows);
+gvsk::kfn_t gvs_kernel_pool_f32_ip(int nch, int rows);
+gvsk::kfn_t gvs_kernel_pool_bf16_l2(int nch, int rows);
+gvsk::kfn_t gvs_kernel_pool_bf16_ip(int nch, int rows);
+gvsk::kfn_t gvs_kernel_pool_fp8_l2(int nch, int rows);
+gvsk::kfn_t gvs_kernel_pool_fp8_ip(int nch, int rows);
+
 gvsk::dfn_t gvs_descend_f32_l2(int nch);
 gvsk::dfn_t gvs_descend_f32_ip(int nch);
 gvsk::dfn_t gvs_descend_bf16_l2(int nch);

@@ -572,10 +572,17 @@ __device__ __forceinline__ void find_entry(const KParams &P, int64_t qi, const W
 
 // Returns true if the query's visited table overflowed (its results are
 // then label -1 / +inf and its status word is 1).
-template <int NCH, int STORE, int METRIC, int MROWS>
+// POOL: the query may start from an initial pool (init, init_len) instead
+// of the descent; a compile-time switch, so that the other kernels hold none
+// of that code.
+template <int NCH, int STORE, int METRIC, int MROWS, bool POOL = false>
 __device__ bool search_query_req(const KParams &P, int64_t qi, int64_t slot, unsigned char *lds, int k, int ef,
                                  const float *Q, bool sys_query, const int32_t *init = nullptr, int init_len = 0)
 {
+    if constexpr (!POOL) {
+        init = nullptr;
+        init_len = 0;
+    }
     constexpr int E = StoreTraits<STORE>::E;
     const int lane = lane_id();
     const int p = lane & 7;
@@ -653,105 +660,93 @@ __device__ bool search_query_req(const KParams &P, int64_t qi, int64_t slot, uns
     }
     wave_sync();
 
-    // After an expansion: a query whose visited set passed the table's limit
-    // freezes the LDS hash and goes on with the HBM table, or overflows.
-    const auto table_check = [&]() -> bool {
-        if (evals <= tlimit) return true;
-        if (lds_mode && P.vtab) {
-            lds_mode = false;
-            frozen = L.vh;
-            T = P.vtab + ((size_t)slot << P.vlog2);
-            tlimit = evals + (int)((3u << P.vlog2) >> 2) - P.lcap;
-            for (uint32_t w = lane; w < L.nbw; w += WAVE) L.bloom[w] = 0ull;
-            if (lane == 0) atomicAdd(&P.counters[2], 1u);
-            wave_sync();
-            return true;
-        }
-        return false;
-    };
-
     int nres = 0;
-    // Initial pool (the NSG search's start, nsg.py:66-90): the listed ids are
-    // visited and evaluated in chunks of MROWS rows, each merged into the
-    // pool as a completed group would be.
-    if (init) {
-        for (int c = 0; c < init_len && !overflow; c += MROWS * WAVE) {
-            int32_t ids[MROWS];
-#pragma unroll
-            for (int r = 0; r < MROWS; r++) {
-                const int j = c + r * WAVE + lane;
-                const int32_t id = j < init_len ? init[j] : -1;
-                ids[r] = (id >= 0 && (int64_t)id < P.n) ? id : -1;
-            }
-            const int nrows = min(MROWS, (init_len - c + WAVE - 1) / WAVE);
-            if (lds_mode)
-                nres = expand_lds<NCH, STORE, METRIC, MROWS>(P, L, qf, ids, nrows, evals EXP_PROF_ARGS);
-            else
-                nres = expand_group<NCH, STORE, METRIC, MROWS>(P, L, qf, T, ids, nrows, evals,
-                                                               frozen EXP_PROF_ARGS);
-            if (nres < 0) {
-                overflow = true;
-                break;
-            }
-            np = merge_into_pool(pool, pool2, np, ef, L.lkeys, nres, hint);
-            if (!table_check()) overflow = true;
-        }
-    }
-    bool haveQ = false;
+    bool haveQ = false, haveGroup = false;
     // DST pipeline: up to mg groups in flight, i.e. selected and not yet
     // merged.  Selected groups wait in a ring of mg lists of up to 8 ids in
     // LDS (list sizes 4 bits per ring position in gsz); each step expands the
     // oldest, after issuing the adjacency loads of the one behind it, so
     // those overlap its gathers.  A group is selected from the pool as it
     // stands once the oldest in-flight group has been merged.
+    // An initial pool (the NSG search's start, nsg.py:66-90) goes first, in
+    // chunks of MROWS rows of ids that take a group's place in the same
+    // expansion (one copy of it in the kernel) and are merged like groups.
     const int mg = P.mg;
     uint32_t gsz = 0u;
     int gh = 0, pend = 0, inflight = 0;
+    int ic = init ? 0 : init_len;  // next initial-pool id
     bool pLoaded = false;
     int32_t idsP[MROWS], idsN[MROWS];
-    for (; !overflow;) {
-        if (haveQ) {  // complete the oldest group
+    for (;;) {
+        if (haveQ) {  // complete the oldest group (or pool chunk)
             np = merge_into_pool(pool, pool2, np, ef, L.lkeys, nres, hint);
             haveQ = false;
-            inflight--;
+            if (haveGroup) inflight--;
         }
         PROF_MARK(6);
-        // refill the pipeline up to mg groups
-        while (inflight < mg) {
-            int pos = gh + pend;
-            if (pos >= mg) pos -= mg;
-            const int ns = select_group(pool, np, P.mc, hint, L.sel + 8 * pos);
-            if (ns == 0) break;
-            gsz = (gsz & ~(15u << (4 * pos))) | ((uint32_t)ns << (4 * pos));
-            pend++;
-            inflight++;
-            hops += ns;
+        int nrowsP;
+        int gn = gh;
+        if (POOL && ic < init_len) {
+#pragma unroll
+            for (int r = 0; r < MROWS; r++) {
+                const int j = ic + r * WAVE + lane;
+                const int32_t id = j < init_len ? init[j] : -1;
+                idsP[r] = (id >= 0 && (int64_t)id < P.n) ? id : -1;
+            }
+            nrowsP = min(MROWS, (init_len - ic + WAVE - 1) / WAVE);
+            ic += MROWS * WAVE;
+            haveGroup = false;
+        } else {
+            // refill the pipeline up to mg groups
+            while (inflight < mg) {
+                int pos = gh + pend;
+                if (pos >= mg) pos -= mg;
+                const int ns = select_group(pool, np, P.mc, hint, L.sel + 8 * pos);
+                if (ns == 0) break;
+                gsz = (gsz & ~(15u << (4 * pos))) | ((uint32_t)ns << (4 * pos));
+                pend++;
+                inflight++;
+                hops += ns;
+            }
+            if (pend == 0) break;
+            const int nselP = (int)((gsz >> (4 * gh)) & 15u);
+            if (!pLoaded) load_adj<MROWS>(P, L.sel + 8 * gh, nselP, rpc, idsP);
+            gn = gh + 1 == mg ? 0 : gh + 1;
+            pLoaded = pend > 1;
+            if (pLoaded) load_adj<MROWS>(P, L.sel + 8 * gn, (int)((gsz >> (4 * gn)) & 15u), rpc, idsN);
+            nrowsP = nselP * rpc;
+            haveGroup = true;
         }
-        if (pend == 0) break;
-        const int nselP = (int)((gsz >> (4 * gh)) & 15u);
-        if (!pLoaded) load_adj<MROWS>(P, L.sel + 8 * gh, nselP, rpc, idsP);
-        const int gn = gh + 1 == mg ? 0 : gh + 1;
-        pLoaded = pend > 1;
-        if (pLoaded) load_adj<MROWS>(P, L.sel + 8 * gn, (int)((gsz >> (4 * gn)) & 15u), rpc, idsN);
         PROF_MARK(1);
         if (lds_mode)
-            nres = expand_lds<NCH, STORE, METRIC, MROWS>(P, L, qf, idsP, nselP * rpc, evals EXP_PROF_ARGS);
+            nres = expand_lds<NCH, STORE, METRIC, MROWS>(P, L, qf, idsP, nrowsP, evals EXP_PROF_ARGS);
         else
-            nres = expand_group<NCH, STORE, METRIC, MROWS>(P, L, qf, T, idsP, nselP * rpc, evals,
+            nres = expand_group<NCH, STORE, METRIC, MROWS>(P, L, qf, T, idsP, nrowsP, evals,
                                                            frozen EXP_PROF_ARGS);
         if (nres < 0) {  // probe sequence of the 2-byte hash too long
             overflow = true;
             break;
         }
         haveQ = true;
-        // the next group moves up
-        pend--;
-        gh = gn;
+        if (haveGroup) {  // the next group moves up
+            pend--;
+            gh = gn;
 #pragma unroll
-        for (int r = 0; r < MROWS; r++) idsP[r] = idsN[r];
-        if (!table_check()) {
-            overflow = true;
-            break;
+            for (int r = 0; r < MROWS; r++) idsP[r] = idsN[r];
+        }
+        if (evals > tlimit) {
+            if (lds_mode && P.vtab) {
+                lds_mode = false;
+                frozen = L.vh;
+                T = P.vtab + ((size_t)slot << P.vlog2);
+                tlimit = evals + (int)((3u << P.vlog2) >> 2) - P.lcap;
+                for (uint32_t w = lane; w < L.nbw; w += WAVE) L.bloom[w] = 0ull;
+                if (lane == 0) atomicAdd(&P.counters[2], 1u);
+                wave_sync();
+            } else {
+                overflow = true;
+                break;
+            }
         }
     }
 
@@ -839,18 +834,20 @@ __device__ bool search_query_req(const KParams &P, int64_t qi, int64_t slot, uns
     return overflow;
 }
 
-template <int NCH, int STORE, int METRIC, int MROWS>
+template <int NCH, int STORE, int METRIC, int MROWS, bool POOL = false>
 __device__ __forceinline__ void search_query(const KParams &P, int64_t qi, int64_t slot, unsigned char *lds)
 {
     const int64_t qrow = P.q_index ? P.q_index[qi] : qi;
     const int k = P.q_k ? P.q_k[qi] : P.k;
     const int ef = P.q_ef ? P.q_ef[qi] : P.ef;
-    const int32_t *init = P.init_mode == 1   ? P.init_ids + qrow * P.init_ld
-                          : P.init_mode == 2 ? P.adj0 + (size_t)P.entry * P.r0
-                                             : nullptr;
-    const int init_len = P.init_mode == 1 ? P.init_len : P.init_mode == 2 ? min(P.r0, max(ef, k)) : 0;
-    search_query_req<NCH, STORE, METRIC, MROWS>(P, qi, slot, lds, k, ef, P.queries + qrow * P.ldq, false, init,
-                                                init_len);
+    const int32_t *init = nullptr;
+    int init_len = 0;
+    if constexpr (POOL) {
+        init = P.init_mode == 1 ? P.init_ids + qrow * P.init_ld : P.adj0 + (size_t)P.entry * P.r0;
+        init_len = P.init_mode == 1 ? P.init_len : min(P.r0, max(ef, k));
+    }
+    search_query_req<NCH, STORE, METRIC, MROWS, POOL>(P, qi, slot, lds, k, ef, P.queries + qrow * P.ldq, false,
+                                                      init, init_len);
 }
 
 // ---- several waves per query (intra-query parallelism) -------------------
