@@ -20,6 +20,7 @@
 #include <atomic>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <mutex>
 #include <random>
@@ -45,6 +46,7 @@ struct Slot {
     uint64_t ring_idx = 0;
     double t_arrival = 0.0;  // microseconds, steady clock
     int32_t k = 0;           // the request's k (its results are k entries)
+    int64_t tag = -1;        // the load generator's query index
     bool busy = false;
     bool cancelled = false;
 };
@@ -78,26 +80,26 @@ struct gvs_server {
     uint64_t *d_head = nullptr, *d_served = nullptr, *d_mirror = nullptr;
     uint32_t *d_vtab = nullptr;  // the server's own per-wave HBM visited tables
     int32_t *d_status = nullptr;
-    // host bookkeeping.  Three locks: submit_mu serialises submitters (ring
-    // tails, request writes, publication), poll_mu serialises pollers
-    // (completion ring head, result copies), and mu guards what both touch
-    // (free slots, slot records, the in-flight list, ring-entry reuse,
-    // counters, the device loop's state).  mu is held for bookkeeping only;
-    // the per-request copies run outside it.
+    // host bookkeeping.  Three locks: submit_mu serialises the claims of
+    // ring positions (ring tails), poll_mu the claims of completion entries
+    // (completion ring head), and mu guards what both touch (free slots,
+    // slot records, the in-flight list, ring-entry reuse, counters, the
+    // device loop's state).  Locks are held for bookkeeping only: request
+    // writes and result copies run outside them, so several threads can
+    // submit and poll at once.
     std::mutex mu, submit_mu, poll_mu;
     std::vector<Slot> slots;
     std::vector<int> free_slots, inflight;
     std::vector<int> inflight_pos;   // slot -> index in inflight
-    uint64_t *cring = nullptr;       // pinned completion ring (written by the device)
-    uint64_t chead = 0;              // completions consumed
+    uint64_t *cring = nullptr;       // pinned completion rings, one per request ring (written by the device)
+    uint64_t chead[SERVE_QUEUES] = {};  // poll_mu: completions consumed per ring
+    int cnext = 0;                      // poll_mu: ring a poll looks at first
     double last_query_us = -1e30;    // last hipStreamQuery
     std::vector<uint8_t> ring_busy;  // ring entry (q * rcap + i) still names an incomplete request
-    struct Admitted { int slot; size_t rpos; int64_t m; };
-    std::vector<Admitted> adm;       // submit_mu: the batch being admitted
-    struct Done { int slot; bool ovf, cancelled; int64_t req; double t_arrival; int k; };
-    std::vector<Done> comp;          // poll_mu: the completions being delivered
-    uint64_t tails[SERVE_QUEUES] = {};      // requests appended to each ring
-    uint64_t published[SERVE_QUEUES] = {};  // ... and published to the device
+    struct Admitted { int slot; size_t rpos; int64_t m; uint64_t seq; };
+    struct Done { int slot; bool ovf, cancelled; int64_t req, tag; double t_arrival; int k; };
+    uint64_t tails[SERVE_QUEUES] = {};                  // submit_mu: ring positions claimed
+    std::atomic<uint64_t> published[SERVE_QUEUES]{};    // ... and published to the device, in claim order
     int rr = 0;                              // ring the next request tries first
     uint64_t next_seq = 1;
     int64_t next_req = 0;
@@ -226,8 +228,9 @@ extern "C" int gvs_server_open(gvs_index *ix, const gvs_server_config *cfg, gvs_
     // Requests go to the rings round robin.
     uint32_t rcap = 16;
     while (rcap < 4u * (uint32_t)((R + SERVE_QUEUES - 1) / SERVE_QUEUES)) rcap <<= 1;
-    uint32_t ccap = 2;  // completion ring: at most R entries are unconsumed
-    while (ccap < 2u * (uint32_t)R) ccap <<= 1;
+    // completion ring q: at most rcap entries are unconsumed (a request's
+    // ring entry is reused only after its completion was polled)
+    const uint32_t ccap = 2u * rcap;
     const int dim = ix->dim;
     bool ok = gvs::hip_ok(hipHostMalloc((void **)&s->ctl, sizeof(ServeCtl), hipHostMallocCoherent), "hipHostMalloc") &&
               gvs::hip_ok(hipHostMalloc((void **)&s->ring, 4 * (size_t)rcap * SERVE_QUEUES, hipHostMallocCoherent),
@@ -237,16 +240,16 @@ extern "C" int gvs_server_open(gvs_index *ix, const gvs_server_config *cfg, gvs_
               gvs::hip_ok(hipHostMalloc((void **)&s->qef, 4 * (size_t)R, hipHostMallocCoherent), "hipHostMalloc") &&
               gvs::hip_ok(hipHostMalloc((void **)&s->qseq, 8 * (size_t)R, hipHostMallocCoherent), "hipHostMalloc") &&
               gvs::hip_ok(hipHostMalloc((void **)&s->done, 8 * (size_t)R, hipHostMallocCoherent), "hipHostMalloc") &&
-              gvs::hip_ok(hipHostMalloc((void **)&s->cring, 8 * (size_t)ccap, hipHostMallocCoherent), "hipHostMalloc") &&
+              gvs::hip_ok(hipHostMalloc((void **)&s->cring, 8 * (size_t)ccap * SERVE_QUEUES, hipHostMallocCoherent), "hipHostMalloc") &&
               gvs::hip_ok(hipHostMalloc((void **)&s->labels, 8 * (size_t)R * cfg->max_k, hipHostMallocCoherent), "hipHostMalloc") &&
               gvs::hip_ok(hipHostMalloc((void **)&s->dists, 4 * (size_t)R * cfg->max_k, hipHostMallocCoherent), "hipHostMalloc") &&
               gvs::hip_ok(hipHostMalloc((void **)&s->stats, 4 * GVS_STATS_ROW * (size_t)R, hipHostMallocCoherent), "hipHostMalloc") &&
               gvs::hip_ok(hipMalloc(&s->d_head, 8 * SERVE_QUEUES * SERVE_HEAD_STRIDE), "hipMalloc") &&
-              gvs::hip_ok(hipMalloc(&s->d_served, 8), "hipMalloc") &&
+              gvs::hip_ok(hipMalloc(&s->d_served, 8 * SERVE_QUEUES * SERVE_HEAD_STRIDE), "hipMalloc") &&
               gvs::hip_ok(hipMalloc(&s->d_mirror, 8 * SERVE_MIRROR_WORDS), "hipMalloc") &&
               gvs::hip_ok(hipMemset(s->d_mirror, 0, 8 * SERVE_MIRROR_WORDS), "hipMemset") &&
               gvs::hip_ok(hipMemset(s->d_head, 0, 8 * SERVE_QUEUES * SERVE_HEAD_STRIDE), "hipMemset") &&
-              gvs::hip_ok(hipMemset(s->d_served, 0, 8), "hipMemset") &&
+              gvs::hip_ok(hipMemset(s->d_served, 0, 8 * SERVE_QUEUES * SERVE_HEAD_STRIDE), "hipMemset") &&
               gvs::hip_ok(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "hipStreamCreateWithFlags");
     const size_t vwords = (size_t)s->grid_waves << s->lp.vlog2;
     if (ok)
@@ -259,7 +262,7 @@ extern "C" int gvs_server_open(gvs_index *ix, const gvs_server_config *cfg, gvs_
     }
     std::memset(s->ctl, 0, sizeof(ServeCtl));
     std::memset(s->done, 0, 8 * (size_t)R);
-    std::memset(s->cring, 0, 8 * (size_t)ccap);
+    std::memset(s->cring, 0, 8 * (size_t)ccap * SERVE_QUEUES);
     s->inflight_pos.assign((size_t)R, -1);
     std::memset(s->qseq, 0, 8 * (size_t)R);
     s->slots.assign((size_t)R, Slot());
@@ -308,17 +311,28 @@ extern "C" int gvs_server_open(gvs_index *ix, const gvs_server_config *cfg, gvs_
 // the admitted range (GVS_ERR_ARG, counted as rejected) or finds no free
 // slot or ring entry (GVS_ERR_STATE), write them into their slots and ring
 // entries, and publish them together (one release of each ring's tail).
-// Caller holds s->submit_mu.  Returns the number admitted, or the error of
-// the first request when none was.
+// Claims of ring positions are serialised (submit_mu); the writes are not,
+// so several threads may submit at once.  A ring's tail is published in
+// claim order: a batch waits until the batches that claimed earlier
+// positions of that ring have published theirs.  No batch waits on a later
+// one (claims are taken under one lock, so of two batches one is first on
+// every ring), and a waiting batch holds no lock.  tags (may be NULL) are
+// kept with the requests for the load generator.  Returns the number
+// admitted, or the error of the first request when none was.
 static int64_t submit_batch_impl(gvs_server *s, const float *queries, int64_t n, int64_t ldq, const int32_t *ks,
-                                 const int32_t *efs, int64_t *req_ids, const double *t_arrival)
+                                 const int32_t *efs, int64_t *req_ids, const double *t_arrival,
+                                 const int64_t *tags = nullptr)
 {
-    const double t = now_us();
     const uint64_t rmask = (uint64_t)s->args.rcap - 1u;
     int64_t err = 0;
-    s->adm.clear();
+    thread_local std::vector<gvs_server::Admitted> adm;
+    adm.clear();
+    uint64_t first[SERVE_QUEUES], last[SERVE_QUEUES];  // ring positions this batch claimed: [first, last)
     {
-        std::lock_guard<std::mutex> lk(s->mu);
+        std::lock_guard<std::mutex> lk(s->submit_mu);
+        for (int q = 0; q < SERVE_QUEUES; q++) first[q] = s->tails[q];
+        const double t = now_us();
+        std::lock_guard<std::mutex> lk2(s->mu);
         for (int64_t m = 0; m < n; m++) {
             const int32_t k = ks[m], ef = efs[m];
             if (k < 1 || k > s->cfg.max_k || ef < 1 || ef > s->cfg.max_ef) {
@@ -352,6 +366,7 @@ static int64_t submit_batch_impl(gvs_server *s, const float *queries, int64_t n,
             sl.req = s->next_req++;
             sl.t_arrival = t_arrival ? t_arrival[m] : t;
             sl.k = k;
+            sl.tag = tags ? tags[m] : -1;
             sl.busy = true;
             sl.cancelled = false;
             sl.ring_idx = rpos;
@@ -360,32 +375,35 @@ static int64_t submit_batch_impl(gvs_server *s, const float *queries, int64_t n,
             s->inflight.push_back(slot);
             s->c_submitted++;
             if (req_ids) req_ids[m] = sl.req;
-            s->adm.push_back({slot, rpos, m});
+            adm.push_back({slot, rpos, m, s->next_seq++});
         }
+        for (int q = 0; q < SERVE_QUEUES; q++) last[q] = s->tails[q];
     }
     // the admitted slots and ring entries belong to this batch until published
     const int dim = s->ix->dim;
-    for (const auto &a : s->adm) {
+    for (const auto &a : adm) {
         std::memcpy(s->qbuf + (size_t)a.slot * dim, queries + a.m * ldq, sizeof(float) * dim);
         s->qk[a.slot] = ks[a.m];
         s->qef[a.slot] = efs[a.m];
-        s->qseq[a.slot] = s->next_seq++;
+        s->qseq[a.slot] = a.seq;
         s->ring[a.rpos] = (uint32_t)a.slot;
     }
-    if (!s->adm.empty()) {
+    if (!adm.empty()) {
         // request fields and ring entries are globally visible before the tails
         std::atomic_thread_fence(std::memory_order_seq_cst);
-        for (int q = 0; q < SERVE_QUEUES; q++)
-            if (s->tails[q] != s->published[q]) {
-                __atomic_store_n(&s->ctl->tail[q], s->tails[q], __ATOMIC_RELEASE);
-                s->published[q] = s->tails[q];
-            }
+        for (int q = 0; q < SERVE_QUEUES; q++) {
+            if (last[q] == first[q]) continue;
+            for (int spin = 0; s->published[q].load(std::memory_order_acquire) != first[q]; spin++)
+                if (spin > 64) std::this_thread::yield();
+            __atomic_store_n(&s->ctl->tail[q], last[q], __ATOMIC_RELEASE);
+            s->published[q].store(last[q], std::memory_order_release);
+        }
         std::lock_guard<std::mutex> lk(s->mu);
         const int rc = serve_ensure_running(s);
         if (rc) return rc;
     }
-    if (s->adm.empty() && err == GVS_ERR_ARG) return err;
-    return (int64_t)s->adm.size();  // 0 when no slot was free (the message says so)
+    if (adm.empty() && err == GVS_ERR_ARG) return err;
+    return (int64_t)adm.size();  // 0 when no slot was free (the message says so)
 }
 
 extern "C" int64_t gvs_server_submit(gvs_server *s, const float *query, int32_t k, int32_t ef)
@@ -394,7 +412,6 @@ extern "C" int64_t gvs_server_submit(gvs_server *s, const float *query, int32_t 
         gvs::set_error("gvs_server_submit: invalid argument");
         return GVS_ERR_ARG;
     }
-    std::lock_guard<std::mutex> lk(s->submit_mu);
     int64_t id = -1;
     const int64_t r = submit_batch_impl(s, query, 1, s->ix->dim, &k, &ef, &id, nullptr);
     return r == 1 ? id : r == 0 ? GVS_ERR_STATE : r;
@@ -407,16 +424,17 @@ extern "C" int64_t gvs_server_submit_batch(gvs_server *s, const float *queries, 
         gvs::set_error("gvs_server_submit_batch: invalid argument");
         return GVS_ERR_ARG;
     }
-    std::lock_guard<std::mutex> lk(s->submit_mu);
     return submit_batch_impl(s, queries, n, ldq, ks, efs, req_ids, nullptr);
 }
 
-// Deliver up to max_done completions, in the order the device finished
-// them; caller holds s->poll_mu.  The device wrote each request's own k
-// results into its slot's rows; rows are delivered max_k wide, the rest
-// label -1 / dist +inf.
+// Deliver up to max_done completions, each ring's in the order the device
+// finished them, the rings taken round robin.  Claims of completion entries are serialised (poll_mu); the result
+// copies are not, so several threads may poll at once, each receiving its
+// own completions.  The device wrote each request's own k results into its
+// slot's rows; rows are delivered max_k wide, the rest label -1 / dist +inf.
+// tags_out (may be NULL) receives the tags the requests were submitted with.
 static int poll_impl(gvs_server *s, int32_t max_done, int64_t *req_ids, int64_t *labels, float *dists,
-                     double *latency_us, int32_t *stats_out, int32_t *status_out)
+                     double *latency_us, int32_t *stats_out, int32_t *status_out, int64_t *tags_out = nullptr)
 {
     const int K = s->cfg.max_k;
     const uint64_t cmask = (uint64_t)s->args.ccap - 1u;
@@ -425,34 +443,46 @@ static int poll_impl(gvs_server *s, int32_t max_done, int64_t *req_ids, int64_t 
         gvs::set_error("gvs_server_poll: the device read an out-of-range slot index from ring entry %u", err - 1u);
         return GVS_ERR_STATE;
     }
-    // completion ring entry i carries (i + 1) in its high word once it is
-    // written, then the overflow bit and the slot
-    s->comp.clear();
-    while ((int32_t)s->comp.size() < max_done) {
-        const uint64_t v = __atomic_load_n(&s->cring[s->chead & cmask], __ATOMIC_ACQUIRE);
-        if ((uint32_t)(v >> 32) != (uint32_t)(s->chead + 1u)) break;
-        const int slot = (int)((uint32_t)v & 0x7FFFFFFFu);
-        if (slot >= s->cfg.max_inflight) {
-            gvs::set_error("gvs_server_poll: completion ring names slot %d of %d", slot, s->cfg.max_inflight);
-            return GVS_ERR_STATE;
+    // entry i of a completion ring carries (i + 1) in its high word once it
+    // is written, then the overflow bit and the slot
+    thread_local std::vector<gvs_server::Done> comp;
+    comp.clear();
+    {
+        std::lock_guard<std::mutex> lk(s->poll_mu);
+        for (int j = 0; j < SERVE_QUEUES && (int32_t)comp.size() < max_done; j++) {
+            const int q = (s->cnext + j) % SERVE_QUEUES;
+            const uint64_t *cr = s->cring + (size_t)q * s->args.ccap;
+            while ((int32_t)comp.size() < max_done) {
+                const uint64_t h = s->chead[q];
+                const uint64_t v = __atomic_load_n(&cr[h & cmask], __ATOMIC_ACQUIRE);
+                if ((uint32_t)(v >> 32) != (uint32_t)(h + 1u)) break;
+                const int slot = (int)((uint32_t)v & 0x7FFFFFFFu);
+                if (slot >= s->cfg.max_inflight) {
+                    gvs::set_error("gvs_server_poll: completion ring %d names slot %d of %d", q, slot,
+                                   s->cfg.max_inflight);
+                    return GVS_ERR_STATE;
+                }
+                s->chead[q] = h + 1u;
+                comp.push_back({slot, ((uint32_t)v >> 31) != 0u, false, 0, -1, 0.0, 0});
+            }
         }
-        s->chead++;
-        s->comp.push_back({slot, ((uint32_t)v >> 31) != 0u, false, 0, 0.0, 0});
+        s->cnext = (s->cnext + 1) % SERVE_QUEUES;
     }
     {
         std::lock_guard<std::mutex> lk(s->mu);
-        for (auto &c : s->comp) {
+        for (auto &c : comp) {
             const Slot &sl = s->slots[(size_t)c.slot];
             if (!sl.busy) {
                 gvs::set_error("gvs_server_poll: completion ring names slot %d, which is not in flight", c.slot);
                 return GVS_ERR_STATE;
             }
             c.req = sl.req;
+            c.tag = sl.tag;
             c.t_arrival = sl.t_arrival;
             c.cancelled = sl.cancelled;
             c.k = sl.k;
         }
-        if (s->comp.empty()) {
+        if (comp.empty()) {
             if (!s->inflight.empty()) {
                 const int rc = serve_ensure_running(s);  // a launch that went idle leaves work behind
                 if (rc) return rc;
@@ -460,22 +490,23 @@ static int poll_impl(gvs_server *s, int32_t max_done, int64_t *req_ids, int64_t 
             return 0;
         }
     }
-    // copies, outside mu: a slot is freed only below
+    // copies, outside the locks: a slot is freed only below
     const double t = now_us();
     int n = 0;
-    const size_t nc = s->comp.size();
+    const size_t nc = comp.size();
     for (size_t j = 0; j < nc; j++) {
         // the device wrote these rows across the bus: the host reads them from
         // memory, so the rows of a few completions ahead are requested early
         if (j + 6 < nc) {
-            const int ps = s->comp[j + 6].slot;
+            const int ps = comp[j + 6].slot;
             __builtin_prefetch(s->labels + (size_t)ps * K);
             __builtin_prefetch(s->dists + (size_t)ps * K);
         }
-        const auto &c = s->comp[j];
+        const auto &c = comp[j];
         if (c.cancelled) continue;
         const int k = std::min(K, std::max(1, c.k));
         if (req_ids) req_ids[n] = c.req;
+        if (tags_out) tags_out[n] = c.tag;
         if (labels) {
             std::memcpy(labels + (size_t)n * K, s->labels + (size_t)c.slot * K, 8 * (size_t)k);
             std::fill(labels + (size_t)n * K + k, labels + (size_t)(n + 1) * K, (int64_t)-1);
@@ -492,7 +523,7 @@ static int poll_impl(gvs_server *s, int32_t max_done, int64_t *req_ids, int64_t 
         n++;
     }
     std::lock_guard<std::mutex> lk(s->mu);
-    for (const auto &c : s->comp) {
+    for (const auto &c : comp) {
         Slot &sl = s->slots[(size_t)c.slot];
         if (!c.cancelled) s->c_completed++;
         if (c.ovf) s->c_overflowed++;
@@ -520,7 +551,6 @@ extern "C" int gvs_server_poll(gvs_server *s, int32_t max_done, int64_t *req_ids
         gvs::set_error("gvs_server_poll: invalid argument");
         return GVS_ERR_ARG;
     }
-    std::lock_guard<std::mutex> lk(s->poll_mu);
     return poll_impl(s, max_done, req_ids, labels, dists, latency_us, nullptr, nullptr);
 }
 
@@ -531,7 +561,6 @@ extern "C" int gvs_server_poll_stats(gvs_server *s, int32_t max_done, int64_t *r
         gvs::set_error("gvs_server_poll_stats: invalid argument");
         return GVS_ERR_ARG;
     }
-    std::lock_guard<std::mutex> lk(s->poll_mu);
     return poll_impl(s, max_done, req_ids, labels, dists, latency_us, stats, nullptr);
 }
 
@@ -542,7 +571,6 @@ extern "C" int gvs_server_poll_status(gvs_server *s, int32_t max_done, int64_t *
         gvs::set_error("gvs_server_poll_status: invalid argument");
         return GVS_ERR_ARG;
     }
-    std::lock_guard<std::mutex> lk(s->poll_mu);
     return poll_impl(s, max_done, req_ids, labels, dists, latency_us, stats, status);
 }
 
@@ -610,9 +638,12 @@ extern "C" int gvs_server_close(gvs_server *s)
 {
     if (!s) return GVS_ERR_ARG;
     {
+        // no new claims; batches already claimed finish their publication first
         std::lock_guard<std::mutex> lk(s->submit_mu);
-        for (int q = 0; q < SERVE_QUEUES; q++)
-            __atomic_store_n(&s->ctl->tail[q], s->published[q] | SERVE_STOP, __ATOMIC_RELEASE);
+        for (int q = 0; q < SERVE_QUEUES; q++) {
+            while (s->published[q].load(std::memory_order_acquire) != s->tails[q]) std::this_thread::yield();
+            __atomic_store_n(&s->ctl->tail[q], s->tails[q] | SERVE_STOP, __ATOMIC_RELEASE);
+        }
     }
     // the loop leaves within microseconds of seeing stop; a loop that does
     // not is reported and its buffers are left allocated (it may still use them)
@@ -631,20 +662,19 @@ extern "C" int gvs_server_close(gvs_server *s)
 
 // Open-loop load generator: request i arrives at a seeded Poisson time
 // (rate_qps).  As the reference client does with its send and receive
-// threads (networked_FPGA/CPU_programs/CPU_client.cpp:490-620), one thread
-// submits every request that has arrived, in one batch per pass (one
-// publication to the device), as slots allow, and the calling thread
-// collects completions; a request's latency runs from its arrival to the
-// poll that sees it completed.  No other thread may submit to the server
-// meanwhile (request ids map back to query indices by submission order).
-extern "C" int gvs_server_loadgen(gvs_server *s, const float *queries, int64_t nq, int64_t ldq, double rate_qps,
-                                  uint64_t seed, const int32_t *ks, const int32_t *efs, int64_t *out_labels,
-                                  float *out_dists, double *out_latency_us, double *out_wall_s)
+// threads (networked_FPGA/CPU_programs/CPU_client.cpp:490-620), submit
+// threads submit every request that has arrived, in one batch per pass (one
+// publication to the device), as slots allow, and poll threads (the calling
+// thread is one) collect completions; a request's latency runs from its
+// arrival to the poll that sees it completed.  With several submit threads
+// the requests are dealt out in runs of LG_RUN consecutive ones, run r to
+// thread r mod nsub, each thread submitting its runs in arrival order.
+static constexpr int64_t LG_RUN = 64;
+
+static int loadgen_impl(gvs_server *s, const float *queries, int64_t nq, int64_t ldq, double rate_qps, uint64_t seed,
+                        const int32_t *ks, const int32_t *efs, int64_t *out_labels, float *out_dists,
+                        double *out_latency_us, double *out_wall_s, int nsub, int npoll)
 {
-    if (!s || !queries || nq < 0 || ldq < s->ix->dim || rate_qps <= 0.0 || !ks || !efs) {
-        gvs::set_error("gvs_server_loadgen: invalid argument");
-        return GVS_ERR_ARG;
-    }
     const int K = s->cfg.max_k;
     std::vector<double> arrival((size_t)nq);
     std::mt19937_64 rng(seed);
@@ -654,119 +684,147 @@ extern "C" int gvs_server_loadgen(gvs_server *s, const float *queries, int64_t n
         t += gap(rng);
         arrival[(size_t)i] = t;
     }
-    int64_t req_base;
-    {
-        std::lock_guard<std::mutex> lk(s->submit_mu);
-        std::lock_guard<std::mutex> lk2(s->mu);
-        req_base = s->next_req;
-    }
+    std::vector<int64_t> tag((size_t)nq);
+    for (int64_t i = 0; i < nq; i++) tag[(size_t)i] = i;
     const double t0 = now_us();
     std::vector<double> t_arr((size_t)nq);
     for (int64_t i = 0; i < nq; i++) t_arr[(size_t)i] = t0 + arrival[(size_t)i];
-    std::atomic<int> sub_err{0};
+    std::atomic<int> err{0};
     std::atomic<bool> stop{false};
-    std::atomic<int64_t> submitted{0};
-    std::string sub_msg;
-    double sub_us = 0.0, poll_us = 0.0;  // time inside the submit and poll calls (lock included)
-    int64_t sub_batches = 0, poll_calls = 0;
-    // submit thread: everything that has arrived, in one batch per pass
-    std::thread submitter([&] {
-        std::vector<int64_t> ids(4096);
-        int64_t next = 0;
-        while (next < nq && !stop.load(std::memory_order_relaxed)) {
-            const double now = now_us() - t0;
-            int64_t m = 0;
-            while (next + m < nq && m < (int64_t)ids.size() && arrival[(size_t)(next + m)] <= now) m++;
-            if (m == 0) {
-                std::this_thread::yield();
-                continue;
+    std::atomic<int64_t> submitted{0}, done_n{0};
+    std::mutex msg_mu;
+    std::string msg;
+    auto fail = [&](int rc, const char *m) {
+        int z = 0;
+        if (err.compare_exchange_strong(z, rc)) {
+            std::lock_guard<std::mutex> lk(msg_mu);
+            msg = m;
+        }
+        stop.store(true);
+    };
+    std::vector<double> sub_us((size_t)nsub, 0.0), poll_us((size_t)npoll, 0.0);
+    std::vector<int64_t> sub_batches((size_t)nsub, 0), poll_calls((size_t)npoll, 0);
+    // submit thread j: runs j, j + nsub, ... of LG_RUN requests, each run
+    // in arrival order, everything that has arrived in one batch per pass
+    auto submitter = [&](int j) {
+        std::vector<int64_t> ids(LG_RUN);
+        for (int64_t r0 = (int64_t)j * LG_RUN; r0 < nq && !stop.load(std::memory_order_relaxed);
+             r0 += (int64_t)nsub * LG_RUN) {
+            const int64_t r1 = std::min(nq, r0 + LG_RUN);
+            int64_t next = r0;
+            while (next < r1 && !stop.load(std::memory_order_relaxed)) {
+                const double now = now_us() - t0;
+                int64_t m = 0;
+                while (next + m < r1 && arrival[(size_t)(next + m)] <= now) m++;
+                if (m == 0) {
+                    std::this_thread::yield();
+                    continue;
+                }
+                const double a0 = now_us();
+                const int64_t got = submit_batch_impl(s, queries + next * ldq, m, ldq, ks + next, efs + next,
+                                                      ids.data(), t_arr.data() + next, tag.data() + next);
+                sub_us[(size_t)j] += now_us() - a0;
+                if (got < 0) {
+                    fail((int)got, gvs_last_error());
+                    return;
+                }
+                if (got > 0) sub_batches[(size_t)j]++;
+                next += got;
+                submitted.fetch_add(got, std::memory_order_relaxed);
+                if (got < m) std::this_thread::yield();  // no free slot: wait for completions
             }
-            int64_t got;
+        }
+    };
+    auto poller = [&](int j) {
+        const int batch = 1024;
+        std::vector<int64_t> tg(batch), lab((size_t)batch * K);
+        std::vector<float> dd((size_t)batch * K);
+        std::vector<double> lat(batch);
+        double last_progress = now_us();
+        while (done_n.load(std::memory_order_relaxed) < nq && !stop.load(std::memory_order_relaxed)) {
             const double a0 = now_us();
-            {
-                std::lock_guard<std::mutex> lk(s->submit_mu);
-                got = submit_batch_impl(s, queries + next * ldq, m, ldq, ks + next, efs + next, ids.data(),
-                                        t_arr.data() + next);
-                if (got < 0) sub_msg = gvs_last_error();
-            }
-            sub_us += now_us() - a0;
-            if (got > 0) sub_batches++;
-            if (got < 0) {
-                sub_err.store((int)got);
+            const int n = poll_impl(s, batch, nullptr, lab.data(), dd.data(), lat.data(), nullptr, nullptr, tg.data());
+            if (n < 0) {
+                fail(n, gvs_last_error());
                 return;
             }
-            next += got;
-            submitted.store(next, std::memory_order_relaxed);
-            if (got < m) std::this_thread::yield();  // no free slot: wait for completions
-        }
-    });
-    const int batch = 1024;
-    std::vector<int64_t> rid(batch), lab((size_t)batch * K);
-    std::vector<float> dd((size_t)batch * K);
-    std::vector<double> lat(batch);
-    int64_t done_n = 0;
-    double last_progress = now_us();
-    int rc = GVS_OK;
-    while (done_n < nq) {
-        if (sub_err.load()) {
-            rc = sub_err.load();
-            break;
-        }
-        int n;
-        const double a0 = now_us();
-        {
-            std::lock_guard<std::mutex> lk(s->poll_mu);
-            n = poll_impl(s, batch, rid.data(), lab.data(), dd.data(), lat.data(), nullptr, nullptr);
-        }
-        if (n > 0) {
-            poll_us += now_us() - a0;
-            poll_calls++;
-        }
-        if (n < 0) {
-            rc = n;
-            break;
-        }
-        for (int j = 0; j < n; j++) {
-            const int64_t i = rid[(size_t)j] - req_base;  // requests are numbered in submission order
-            if (i < 0 || i >= nq) continue;
-            if (out_labels) std::memcpy(out_labels + i * K, lab.data() + (size_t)j * K, 8 * (size_t)K);
-            if (out_dists) std::memcpy(out_dists + i * K, dd.data() + (size_t)j * K, 4 * (size_t)K);
-            if (out_latency_us) out_latency_us[i] = lat[(size_t)j];
-            done_n++;
-        }
-        const double now = now_us();
-        if (n > 0) {
-            last_progress = now;
-        } else {
-            if (now - last_progress > 30e6) {
-                gvs::set_error("gvs_server_loadgen: no request completed for 30 s (%lld of %lld done, %lld submitted)",
-                               (long long)done_n, (long long)nq, (long long)submitted.load());
-                rc = GVS_ERR_STATE;
-                break;
+            if (n > 0) {
+                poll_us[(size_t)j] += now_us() - a0;
+                poll_calls[(size_t)j]++;
             }
-            std::this_thread::yield();
+            for (int i2 = 0; i2 < n; i2++) {
+                const int64_t i = tg[(size_t)i2];
+                if (i < 0 || i >= nq) continue;
+                if (out_labels) std::memcpy(out_labels + i * K, lab.data() + (size_t)i2 * K, 8 * (size_t)K);
+                if (out_dists) std::memcpy(out_dists + i * K, dd.data() + (size_t)i2 * K, 4 * (size_t)K);
+                if (out_latency_us) out_latency_us[i] = lat[(size_t)i2];
+            }
+            const double now = now_us();
+            if (n > 0) {
+                done_n.fetch_add(n, std::memory_order_relaxed);
+                last_progress = now;
+            } else {
+                if (now - last_progress > 30e6) {
+                    char m[200];
+                    std::snprintf(m, sizeof m,
+                                  "gvs_server_loadgen: no request completed for 30 s (%lld of %lld done, %lld submitted)",
+                                  (long long)done_n.load(), (long long)nq, (long long)submitted.load());
+                    fail(GVS_ERR_STATE, m);
+                    return;
+                }
+                std::this_thread::yield();
+            }
         }
-    }
+    };
+    std::vector<std::thread> th;
+    for (int j = 0; j < nsub; j++) th.emplace_back(submitter, j);
+    for (int j = 1; j < npoll; j++) th.emplace_back(poller, j);
+    poller(0);
     stop.store(true);
-    submitter.join();
-    if (rc == GVS_OK && sub_err.load()) rc = sub_err.load();
-    if (rc != GVS_OK) {
-        if (!sub_msg.empty() && sub_err.load() == rc) gvs::set_error("%s", sub_msg.c_str());
-        return rc;
+    for (auto &x : th) x.join();
+    if (err.load()) {
+        gvs::set_error("%s", msg.c_str());
+        return err.load();
     }
     const double wall_us = now_us() - t0;
     if (out_wall_s) *out_wall_s = wall_us / 1e6;
     {
         std::lock_guard<std::mutex> lk(s->mu);
         const double n = (double)std::max<int64_t>(nq, 1);
-        s->lg_breakdown[0] = 1e3 * sub_us / n;
-        s->lg_breakdown[1] = 1e3 * poll_us / n;
-        s->lg_breakdown[2] = sub_us / wall_us;
-        s->lg_breakdown[3] = poll_us / wall_us;
-        s->lg_breakdown[4] = (double)nq / (double)std::max<int64_t>(sub_batches, 1);
-        s->lg_breakdown[5] = (double)nq / (double)std::max<int64_t>(poll_calls, 1);
+        double su = 0, pu = 0;
+        int64_t sb = 0, pc = 0;
+        for (int j = 0; j < nsub; j++) su += sub_us[(size_t)j], sb += sub_batches[(size_t)j];
+        for (int j = 0; j < npoll; j++) pu += poll_us[(size_t)j], pc += poll_calls[(size_t)j];
+        s->lg_breakdown[0] = 1e3 * su / n;
+        s->lg_breakdown[1] = 1e3 * pu / n;
+        s->lg_breakdown[2] = su / (wall_us * nsub);
+        s->lg_breakdown[3] = pu / (wall_us * npoll);
+        s->lg_breakdown[4] = (double)nq / (double)std::max<int64_t>(sb, 1);
+        s->lg_breakdown[5] = (double)nq / (double)std::max<int64_t>(pc, 1);
     }
     return GVS_OK;
+}
+
+extern "C" int gvs_server_loadgen(gvs_server *s, const float *queries, int64_t nq, int64_t ldq, double rate_qps,
+                                  uint64_t seed, const int32_t *ks, const int32_t *efs, int64_t *out_labels,
+                                  float *out_dists, double *out_latency_us, double *out_wall_s)
+{
+    return gvs_server_loadgen_mt(s, queries, nq, ldq, rate_qps, seed, ks, efs, out_labels, out_dists, out_latency_us,
+                                 out_wall_s, 1, 1);
+}
+
+extern "C" int gvs_server_loadgen_mt(gvs_server *s, const float *queries, int64_t nq, int64_t ldq, double rate_qps,
+                                     uint64_t seed, const int32_t *ks, const int32_t *efs, int64_t *out_labels,
+                                     float *out_dists, double *out_latency_us, double *out_wall_s,
+                                     int32_t submit_threads, int32_t poll_threads)
+{
+    if (!s || !queries || nq < 0 || ldq < s->ix->dim || rate_qps <= 0.0 || !ks || !efs || submit_threads < 1 ||
+        submit_threads > 16 || poll_threads < 1 || poll_threads > 16) {
+        gvs::set_error("gvs_server_loadgen: invalid argument (1 to 16 submit and poll threads)");
+        return GVS_ERR_ARG;
+    }
+    return loadgen_impl(s, queries, nq, ldq, rate_qps, seed, ks, efs, out_labels, out_dists, out_latency_us,
+                        out_wall_s, submit_threads, poll_threads);
 }
 
 extern "C" int gvs_server_loadgen_breakdown(gvs_server *s, double out[6])

@@ -575,6 +575,71 @@ __device__ __forceinline__ void find_entry(const KParams &P, int64_t qi, const W
 // POOL: the query may start from an initial pool (init, init_len) instead
 // of the descent; a compile-time switch, so that the other kernels hold none
 // of that code.
+// The query in the lane layout eval_rows reads (lane p = lane & 7 holds the
+// elements (p + 8 i) * E + t), zero past dim.  A query in host memory
+// (sys_query: system-scope loads, fresh across the bus) is read once: lane l
+// loads element 64 b + l of each 64-element block b (one contiguous 256-byte
+// read per block), and the layout is built with cross-lane permutes, instead
+// of every lane reading its own elements across the bus (8 lanes share each).
+template <int NCH, int E>
+__device__ __forceinline__ void load_query(const KParams &P, const float *Q, bool sys_query, float *qf)
+{
+    const int lane = lane_id();
+    const int p = lane & 7;
+    if constexpr (E == 4) {
+        if (sys_query) {
+            // element 32 i + 4 p + t lies in block i >> 1, at 32 (i & 1) + 4 p + t
+            constexpr int NB = (NCH + 1) / 2;
+            float blk[NB];
+#pragma unroll
+            for (int b = 0; b < NB; b++) {
+                const int idx = 64 * b + lane;
+                blk[b] = idx < P.dim ? __hip_atomic_load(&Q[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) : 0.0f;
+            }
+#pragma unroll
+            for (int i = 0; i < NCH; i++)
+#pragma unroll
+                for (int t = 0; t < 4; t++) {
+                    const int src = 32 * (i & 1) + 4 * p + t;
+                    qf[i * 4 + t] = __int_as_float(__builtin_amdgcn_ds_bpermute(src << 2, __float_as_int(blk[i >> 1])));
+                }
+            return;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < NCH; i++)
+#pragma unroll
+        for (int t = 0; t < E; t++) {
+            const int idx = (p + 8 * i) * E + t;
+            float v = 0.0f;
+            if (idx < P.dim)
+                v = sys_query ? __hip_atomic_load(&Q[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) : Q[idx];
+            qf[i * E + t] = v;
+        }
+}
+
+// One result row: plain stores, or, for a request whose results cross the
+// bus (sys), system-scope stores that write through to host memory, so the
+// completion flag needs only a wait for them, not a write-back of L2.
+__device__ __forceinline__ void put_result(const KParams &P, int64_t at, int64_t lab, float d, bool sys)
+{
+    if (sys) {
+        __hip_atomic_store(&P.out_labels[at], lab, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        __hip_atomic_store(&P.out_dists[at], d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    } else {
+        P.out_labels[at] = lab;
+        P.out_dists[at] = d;
+    }
+}
+
+__device__ __forceinline__ void put_stat(int32_t *row, int i, int32_t v, bool sys)
+{
+    if (sys)
+        __hip_atomic_store(&row[i], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    else
+        row[i] = v;
+}
+
 template <int NCH, int STORE, int METRIC, int MROWS, bool POOL = false>
 __device__ bool search_query_req(const KParams &P, int64_t qi, int64_t slot, unsigned char *lds, int k, int ef,
                                  const float *Q, bool sys_query, const int32_t *init = nullptr, int init_len = 0)
@@ -592,16 +657,7 @@ __device__ bool search_query_req(const KParams &P, int64_t qi, int64_t slot, uns
     if (ef < k) ef = k;
 
     float qf[NCH * E];
-#pragma unroll
-    for (int i = 0; i < NCH; i++)
-#pragma unroll
-        for (int t = 0; t < E; t++) {
-            const int idx = (p + 8 * i) * E + t;
-            float v = 0.0f;
-            if (idx < P.dim)
-                v = sys_query ? __hip_atomic_load(&Q[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) : Q[idx];
-            qf[i * E + t] = v;
-        }
+    load_query<NCH, E>(P, Q, sys_query, qf);
 
     int up_hops = 0, up_evals = 0;
     uint32_t ep = 0u;
@@ -801,18 +857,17 @@ __device__ bool search_query_req(const KParams &P, int64_t qi, int64_t slot, uns
             lab = P.labels ? P.labels[id] : (int64_t)id;
             d = key_dist(key);
         }
-        P.out_labels[qi * P.k + i] = lab;
-        P.out_dists[qi * P.k + i] = d;
+        put_result(P, qi * P.k + i, lab, d, sys_query);
     }
     PROF_MARK(7);
     if (lane == 0) {
         if (P.out_stats) {
             int32_t *row = P.out_stats + qi * GVS_STATS_ROW;
-            row[0] = hops;
-            row[1] = evals;
+            put_stat(row, 0, hops, sys_query);
+            put_stat(row, 1, evals, sys_query);
             if (!P.keep_upper_stats) {
-                row[2] = up_hops;
-                row[3] = up_evals;
+                put_stat(row, 2, up_hops, sys_query);
+                put_stat(row, 3, up_evals, sys_query);
             }
 #ifdef GVS_PROFILE
             for (int ph = 0; ph < 8; ph++) row[4 + ph] = (int32_t)(prof_acc[ph] >> 6);
@@ -982,16 +1037,7 @@ __device__ bool search_query_mw_req(const KParams &P, int64_t qi, int64_t slot, 
 
     if (ef < k) ef = k;
     float qf[NCH * E];
-#pragma unroll
-    for (int i = 0; i < NCH; i++)
-#pragma unroll
-        for (int t = 0; t < E; t++) {
-            const int idx = (p + 8 * i) * E + t;
-            float v = 0.0f;
-            if (idx < P.dim)
-                v = sys_query ? __hip_atomic_load(&Q[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) : Q[idx];
-            qf[i * E + t] = v;
-        }
+    load_query<NCH, E>(P, Q, sys_query, qf);
     // every wave runs the (cheap, deterministic) descent on its own lists
     int up_hops = 0, up_evals = 0;
     uint32_t ep;
@@ -1178,16 +1224,15 @@ __device__ bool search_query_mw_req(const KParams &P, int64_t qi, int64_t slot, 
                 lab = P.labels ? P.labels[id] : (int64_t)id;
                 d = key_dist(key);
             }
-            P.out_labels[qi * P.k + i] = lab;
-            P.out_dists[qi * P.k + i] = d;
+            put_result(P, qi * P.k + i, lab, d, sys_query);
         }
         if (lane == 0) {
             if (P.out_stats) {
                 int32_t *row = P.out_stats + qi * GVS_STATS_ROW;
-                row[0] = hops;
-                row[1] = (int)G.ctl[WG_EVALS];
-                row[2] = up_hops;
-                row[3] = up_evals;
+                put_stat(row, 0, hops, sys_query);
+                put_stat(row, 1, (int)G.ctl[WG_EVALS], sys_query);
+                put_stat(row, 2, up_hops, sys_query);
+                put_stat(row, 3, up_evals, sys_query);
 #ifdef GVS_PROFILE
                 // wave 0's phases: descent, wait at the step barrier, visit,
                 // gather + distances, filter, wait at the list barrier,

@@ -44,12 +44,15 @@ struct ServeArgs {
     const int32_t *qef;      // pinned, per slot
     const uint64_t *qseq;    // pinned, per slot: sequence number of the request in the slot
     uint64_t *done;          // pinned, per slot: sequence number of the last completed request
-    uint64_t *cring;         // pinned completion ring: entry i = (i + 1) << 32 | slot of the i-th completion
-    uint32_t ccap;           // power of two, >= slots
+    uint64_t *cring;         // pinned completion rings, one per request ring: entry i of ring q at
+                             // cring[q * ccap + (i & (ccap-1))] = (i + 1) << 32 | overflow << 31 | slot
+    uint32_t ccap;           // entries per completion ring, a power of two, >= rcap
     uint32_t nslots;         // slots (a ring entry names one below this)
     uint64_t *head;          // device: next request index to claim on ring q at head[q * SERVE_HEAD_STRIDE]
     uint64_t *mirror;        // device: SERVE_MIRROR_WORDS words, the tails as last read from the host
-    uint64_t *served;        // device: completed requests (completion ring position)
+    uint64_t *served;        // device: requests of ring q completed (its completion ring position) at
+                             // served[q * SERVE_HEAD_STRIDE]: one counter per ring, since same-address
+                             // device-scope atomics from all XCDs serialise
     int64_t idle_ticks;      // 100 MHz ticks without work before a wave leaves the loop
 };
 
@@ -205,18 +208,23 @@ __device__ int serve_slot(const ServeArgs &A, int q, uint64_t item)
     return (int)slot;
 }
 
-// Results (plain stores to pinned memory) become visible before the flag: a
-// release fence (write-back, no L2 invalidation); then the per-slot done
-// word and the completion ring entry (position, overflow bit, slot).
-__device__ void serve_complete(const ServeArgs &A, int slot, uint64_t seq, bool ovf)
+// The results were written by this wave with system-scope stores, which
+// write through to host memory: once they are complete (the wait below,
+// also a compiler barrier) the flags can follow, with no release fence (at
+// system scope that writes back the whole L2 of the XCD, once per request).
+// Then the per-slot done word and the entry in the completion ring of the
+// request's ring q (position, overflow bit, slot), also stored through to
+// host memory; the host reads the entry before the rows, and PCIe keeps
+// posted writes in order.
+__device__ void serve_complete(const ServeArgs &A, int q, int slot, uint64_t seq, bool ovf)
 {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+    __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
     if (lane_id() == 0) {
-        __hip_atomic_store(&A.done[slot], seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
-        const uint64_t pos = atomicAdd((unsigned long long *)A.served, 1ull);
-        __hip_atomic_store(&A.cring[pos & (uint64_t)(A.ccap - 1u)],
+        __hip_atomic_store(&A.done[slot], seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        const uint64_t pos = atomicAdd((unsigned long long *)(A.served + (size_t)q * SERVE_HEAD_STRIDE), 1ull);
+        __hip_atomic_store(&A.cring[(size_t)q * A.ccap + (size_t)(pos & (uint64_t)(A.ccap - 1u))],
                            ((pos + 1ull) << 32) | ((uint64_t)(ovf ? 1u : 0u) << 31) | (uint64_t)slot,
-                           __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     }
 }
 
@@ -240,7 +248,7 @@ __global__ __launch_bounds__(256, GVS_MIN_WAVES_PER_EU) void k_serve(ServeArgs A
         const uint64_t seq = ld_sys64r(&A.qseq[slot]);
         const bool ovf = search_query_req<NCH, STORE, METRIC, MROWS>(A.P, (int64_t)slot, gw, lds, k, ef,
                                                                      A.qbuf + (size_t)slot * A.P.ldq, true);
-        serve_complete(A, slot, seq, ovf);
+        serve_complete(A, q, slot, seq, ovf);
         C.last = wall_clock64();
     }
 }
@@ -258,9 +266,9 @@ __global__ __launch_bounds__(512) void k_serve_mw(ServeArgs A)
     const int W = A.P.waves_per_block;
     const int64_t gb = blockIdx.x;
     ClaimState C{(int)(gb % SERVE_QUEUES), 1, gb * W < SERVE_KEEP, ~0ull, (uint64_t)wall_clock64()};
+    int q = 0;  // wave 0: the ring of the request in hand
     for (;;) {
         if (w == 0) {
-            int q;
             uint64_t item;
             uint32_t st = serve_claim(A, C, q, item);
             int slot = -1;
@@ -293,7 +301,7 @@ __global__ __launch_bounds__(512) void k_serve_mw(ServeArgs A)
         // (the search's own barriers order these reads before the next claim's writes)
         const bool ovf = search_query_mw_req<NCH, STORE, METRIC, WROWS>(A.P, (int64_t)slot, gb, smem, k, ef,
                                                                         A.qbuf + (size_t)slot * A.P.ldq, true);
-        if (w == 0) serve_complete(A, slot, seq, ovf);
+        if (w == 0) serve_complete(A, q, slot, seq, ovf);
         C.last = wall_clock64();
     }
 }

@@ -112,6 +112,7 @@ def _load():
         "gvs_server_cancel": (C.c_int, [P, I64]),
         "gvs_server_poll_stats": (C.c_int, [P, I32, P, P, P, P, P]),
         "gvs_server_loadgen": (C.c_int, [P, P, I64, I64, C.c_double, U64, P, P, P, P, P, P]),
+        "gvs_server_loadgen_mt": (C.c_int, [P, P, I64, I64, C.c_double, U64, P, P, P, P, P, P, I32, I32]),
         "gvs_server_stats": (C.c_int, [P, P]),
         "gvs_server_close": (C.c_int, [P]),
         "gvs_net_serve": (C.c_int, [P, P, P, P]),
@@ -144,7 +145,7 @@ def exported_symbols():
              "gvs_search_graph_launch", "gvs_search_graph_destroy", "gvs_device_copy", "gvs_stats_per_query", "gvs_search_plan", "gvs_hash_murmur2_keylen4_device", "gvs_bruteforce_f64_device", "gvs_server_open",
              "gvs_server_submit", "gvs_server_submit_batch", "gvs_server_poll", "gvs_server_poll_stats",
              "gvs_server_poll_status", "gvs_server_counters", "gvs_server_loadgen_breakdown", "gvs_server_cancel",
-             "gvs_server_stats", "gvs_server_close", "gvs_server_loadgen", "gvs_net_serve", "gvs_net_frame_bytes", "gvs_search_pool_device", "gvs_search_pool_host", "gvs_hnsw_build_device", "gvs_hnsw_build_batched"]
+             "gvs_server_stats", "gvs_server_close", "gvs_server_loadgen", "gvs_server_loadgen_mt", "gvs_net_serve", "gvs_net_frame_bytes", "gvs_search_pool_device", "gvs_search_pool_host", "gvs_hnsw_build_device", "gvs_hnsw_build_batched"]
     return [n for n in names if hasattr(lib, n)]
 
 
@@ -543,7 +544,9 @@ class Server:
         return dict(zip(("submitted", "admitted", "completed", "cancelled", "rejected", "device_loops",
                          "overflowed"), c.tolist()))
 
-    def loadgen(self, queries, rate_qps, ks, efs, seed=1):
+    def loadgen(self, queries, rate_qps, ks, efs, seed=1, submit_threads=1, poll_threads=1):
+        """Open-loop Poisson load (gvs_server_loadgen_mt) -> (labels, dists,
+        latency_us per query, wall seconds)."""
         q = np.ascontiguousarray(queries, dtype=np.float32)
         nq = q.shape[0]
         ks = np.ascontiguousarray(ks, dtype=np.int32)
@@ -552,8 +555,9 @@ class Server:
         dist = np.empty((nq, self.max_k), dtype=np.float32)
         lat = np.empty(nq, dtype=np.float64)
         wall = C.c_double(0.0)
-        _check(lib.gvs_server_loadgen(self._h, _ptr(q), nq, q.shape[1], float(rate_qps), seed, _ptr(ks), _ptr(efs),
-                                      _ptr(lab), _ptr(dist), _ptr(lat), C.byref(wall)))
+        _check(lib.gvs_server_loadgen_mt(self._h, _ptr(q), nq, q.shape[1], float(rate_qps), seed, _ptr(ks),
+                                         _ptr(efs), _ptr(lab), _ptr(dist), _ptr(lat), C.byref(wall),
+                                         int(submit_threads), int(poll_threads)))
         return lab, dist, lat, wall.value
 
     def loadgen_breakdown(self):

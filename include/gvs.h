@@ -364,6 +364,10 @@ typedef struct gvs_server_config {
 } gvs_server_config;
 
 GVS_API int gvs_server_open(gvs_index *ix, const gvs_server_config *cfg, gvs_server **out);
+/* Submits and polls may come from several threads at once: the claims of
+ * ring positions and of completion entries are serialised, the request
+ * writes and result copies are not, and each poll returns completions no
+ * other poll returns. */
 /* Enqueue one request (host query of dim floats).  Returns a request id >= 0,
  * GVS_ERR_ARG when k or ef is outside the admitted range (counted as
  * rejected), or GVS_ERR_STATE when all max_inflight device slots are busy
@@ -408,7 +412,9 @@ GVS_API int gvs_server_close(gvs_server *s);
  * Outputs per query i: labels/dists rows of max_k, latency_us[i] from
  * arrival to completion; wall_s = time to drain all nq.  The reference's
  * client measures per-batch latency the same way around its windows
- * (networked_FPGA/CPU_programs/CPU_client.cpp:742-790). */
+ * (networked_FPGA/CPU_programs/CPU_client.cpp:742-790).  One submit thread
+ * and the calling thread polling; no other thread may poll the server
+ * meanwhile. */
 /* Host-side breakdown of the last gvs_server_loadgen on this server: [0] ns
  * per request spent in the submit thread's batched submits (lock
  * included), [1] ns per request in the collecting thread's polls that
@@ -419,6 +425,16 @@ GVS_API int gvs_server_loadgen_breakdown(gvs_server *s, double out[6]);
 GVS_API int gvs_server_loadgen(gvs_server *s, const float *queries, int64_t nq, int64_t ldq, double rate_qps,
                                uint64_t seed, const int32_t *ks, const int32_t *efs, int64_t *labels,
                                float *dists, double *latency_us, double *wall_s);
+/* gvs_server_loadgen with submit_threads threads submitting (requests dealt
+ * out in runs of 64 consecutive ones, each thread's runs in arrival order)
+ * and poll_threads threads polling (the calling thread is one), 1 to 16
+ * each: the host side of a server fed by several connections.  No
+ * reference counterpart beyond the client's one send and one receive
+ * thread (CPU_client.cpp:490-620). */
+GVS_API int gvs_server_loadgen_mt(gvs_server *s, const float *queries, int64_t nq, int64_t ldq, double rate_qps,
+                                  uint64_t seed, const int32_t *ks, const int32_t *efs, int64_t *out_labels,
+                                  float *out_dists, double *out_latency_us, double *out_wall_s,
+                                  int32_t submit_threads, int32_t poll_threads);
 
 /* Network front end with the reference's wire format
  * (networked_FPGA/CPU_programs/README.md:40-72): the node side of the

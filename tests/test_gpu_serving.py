@@ -139,6 +139,33 @@ def test_loadgen_poisson_matches_alone():
     check_against_oracle(s["og"], q, ks, efs, lab, dist)
 
 
+@pytest.mark.parametrize("threads", [(3, 2), (4, 4)])
+def test_loadgen_several_threads_matches_alone(threads):
+    """Several submit and poll threads at once (per-ring ordered publication,
+    completion rings per request ring): every request still gets exactly its
+    own result, saturated and under load, with nothing lost or duplicated."""
+    s = setup()
+    dev, q = s["dev"], s["q"]
+    rng = np.random.default_rng(11)
+    nq = 4 * len(q)
+    qq = np.ascontiguousarray(np.tile(q, (4, 1)))
+    ks = rng.choice([1, 10, 100], size=nq).astype(np.int32)
+    efs = rng.integers(32, 129, size=nq).astype(np.int32)
+    srv = gvs.Server(dev, max_inflight=256, max_k=100, max_ef=128)
+    S, P = threads
+    for rate in (1e9, 30000.0):
+        lab, dist, lat, wall = srv.loadgen(qq, rate, ks, efs, seed=5, submit_threads=S, poll_threads=P)
+        assert np.all(lat > 0)
+        for i in range(nq):
+            k, ef = int(ks[i]), int(efs[i])
+            alone, adist = dev.search(qq[i:i + 1], k=k, ef=ef)
+            np.testing.assert_array_equal(lab[i, :k], alone[0])
+            np.testing.assert_array_equal(dist[i, :k], adist[0])
+    st = srv.stats()
+    srv.close()
+    assert st["completed"] == 2 * nq and st["submitted"] == 2 * nq
+
+
 def test_submit_batch_admits_a_prefix_and_publishes_it():
     """gvs_server_submit_batch admits requests in order up to the free slots;
     the rest are submitted after completions; every result equals the
