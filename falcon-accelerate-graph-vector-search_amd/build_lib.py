@@ -25,6 +25,7 @@ HOST_SRCS = ["hnsw_build.cpp", "formats.cpp", "datagen.cpp", "gvs_net.cpp"]
 HIP_SRCS = ["gvs_device.hip", "gvs_bruteforce.hip", "gvs_server.hip", "gvs_build.hip",
             "k_f32_l2.hip", "k_f32_ip.hip", "k_bf16_l2.hip", "k_bf16_ip.hip", "k_fp8_l2.hip", "k_fp8_ip.hip",
             "ks_f32_l2.hip", "ks_f32_ip.hip", "k_f32seq_l2.hip", "k_f32seq_ip.hip",
+            "ks_bf16_l2.hip", "ks_bf16_ip.hip", "ks_fp8_l2.hip", "ks_fp8_ip.hip",
             "kp_f32_l2.hip", "kp_f32_ip.hip", "kp_bf16_l2.hip", "kp_bf16_ip.hip", "kp_fp8_l2.hip", "kp_fp8_ip.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -160,8 +160,10 @@ extern "C" int gvs_server_open(gvs_index *ix, const gvs_server_config *cfg, gvs_
 {
     const int W = cfg ? (cfg->waves_per_request > 0 ? cfg->waves_per_request : 1) : 1;
     if (!ix || !cfg || !out || cfg->max_inflight < 1 || cfg->max_inflight > (1 << 20) || cfg->max_k < 1 ||
-        cfg->max_ef < 1 || cfg->mc < 0 || cfg->storage != GVS_STORE_F32 || (W != 1 && W != 2 && W != 4 && W != 8)) {
-        gvs::set_error("gvs_server_open: invalid configuration (fp32 storage; 1, 2, 4 or 8 waves per request)");
+        cfg->max_ef < 1 || cfg->mc < 0 || cfg->storage < GVS_STORE_F32 || cfg->storage > GVS_STORE_FP8 ||
+        (W != 1 && W != 2 && W != 4 && W != 8) || (W > 1 && cfg->storage != GVS_STORE_F32)) {
+        gvs::set_error("gvs_server_open: invalid configuration (fp32, BF16 or FP8 storage; 1, 2, 4 or 8 waves per "
+                       "request, several only with fp32)");
         return GVS_ERR_ARG;
     }
     gvs_server *s = new gvs_server();
@@ -174,7 +176,8 @@ extern "C" int gvs_server_open(gvs_index *ix, const gvs_server_config *cfg, gvs_
     p.mc = cfg->mc > 0 ? cfg->mc : 1;
     p.mg = 1;
     p.waves_per_query = W;
-    p.storage = GVS_STORE_F32;
+    p.storage = cfg->storage;
+    p.rerank = cfg->storage != GVS_STORE_F32 && cfg->rerank ? 1 : 0;
     p.persistent = 1;
     // the grid holds at most grid_waves (or max_inflight) waves: the
     // visited hash may use LDS that only waves beyond those would need
@@ -184,11 +187,17 @@ extern "C" int gvs_server_open(gvs_index *ix, const gvs_server_config *cfg, gvs_
         delete s;
         return rc;
     }
-    const int nch = ix->dim_pad / 32;
+    const int E = cfg->storage == GVS_STORE_F32 ? 4 : cfg->storage == GVS_STORE_BF16 ? 8 : 16;
+    const int nch = ix->dim_pad / (8 * E);
     const int rows = p.mc * ((ix->r0 + WAVE - 1) / WAVE);
     const bool l2 = ix->metric == GVS_METRIC_L2;
     if (W == 1) {
-        s->fn = l2 ? gvs_serve_kernel_f32_l2(nch, rows) : gvs_serve_kernel_f32_ip(nch, rows);
+        if (cfg->storage == GVS_STORE_F32)
+            s->fn = l2 ? gvs_serve_kernel_f32_l2(nch, rows) : gvs_serve_kernel_f32_ip(nch, rows);
+        else if (cfg->storage == GVS_STORE_BF16)
+            s->fn = l2 ? gvs_serve_kernel_bf16_l2(nch, rows) : gvs_serve_kernel_bf16_ip(nch, rows);
+        else
+            s->fn = l2 ? gvs_serve_kernel_fp8_l2(nch, rows) : gvs_serve_kernel_fp8_ip(nch, rows);
         s->shm = (size_t)s->lp.wpb * s->lp.lds_per_wave;
     } else {
         // one workgroup of W waves per request (its LDS: lp.lds_per_wave)
@@ -629,7 +638,8 @@ int server_search_setup(gvs_server *s, gvs_index **ix, gvs_params *p, int *max_k
     p->mc = s->cfg.mc > 0 ? s->cfg.mc : 1;
     p->mg = 1;
     p->waves_per_query = s->cfg.waves_per_request > 0 ? s->cfg.waves_per_request : 1;
-    p->storage = GVS_STORE_F32;
+    p->storage = s->cfg.storage;
+    p->rerank = s->cfg.storage != GVS_STORE_F32 && s->cfg.rerank ? 1 : 0;
     return GVS_OK;
 }
 }  // namespace gvs

@@ -332,3 +332,7 @@ gvsk::sfn_t gvs_serve_kernel_f32_l2(int nch, int rows);
 gvsk::sfn_t gvs_serve_kernel_f32_ip(int nch, int rows);
 gvsk::sfn_t gvs_serve_kernel_mw_f32_l2(int nch, int rows_per_wave);
 gvsk::sfn_t gvs_serve_kernel_mw_f32_ip(int nch, int rows_per_wave);
+gvsk::sfn_t gvs_serve_kernel_bf16_l2(int nch, int rows);
+gvsk::sfn_t gvs_serve_kernel_bf16_ip(int nch, int rows);
+gvsk::sfn_t gvs_serve_kernel_fp8_l2(int nch, int rows);
+gvsk::sfn_t gvs_serve_kernel_fp8_ip(int nch, int rows);

@@ -54,7 +54,8 @@ class Params(C.Structure):
 
 class ServerConfig(C.Structure):
     _fields_ = [(n, C.c_int32) for n in (
-        "max_inflight", "max_k", "max_ef", "queue_capacity", "grid_waves", "mc", "storage", "waves_per_request")]
+        "max_inflight", "max_k", "max_ef", "queue_capacity", "grid_waves", "mc", "storage", "waves_per_request",
+        "rerank")]
 
 
 class BuildConfig(C.Structure):
@@ -479,10 +480,11 @@ class Server:
     """Continuous-batching serving loop over a DeviceIndex (C ABI gvs_server_*)."""
 
     def __init__(self, dev, max_inflight=1024, max_k=100, max_ef=512, queue_capacity=0, grid_waves=0, mc=1,
-                 waves_per_request=1):
+                 waves_per_request=1, storage=F32, rerank=False):
         self.dev = dev
         self.max_k = max_k
-        cfg = ServerConfig(max_inflight, max_k, max_ef, queue_capacity, grid_waves, mc, F32, waves_per_request)
+        cfg = ServerConfig(max_inflight, max_k, max_ef, queue_capacity, grid_waves, mc, storage, waves_per_request,
+                           int(bool(rerank)))
         self._h = C.c_void_p()
         _check(lib.gvs_server_open(dev.handle, C.byref(cfg), C.byref(self._h)))
 

@@ -357,10 +357,13 @@ typedef struct gvs_server_config {
                                 * waves, at most max_inflight); heavy
                                 * mixes can saturate higher below that */
     int32_t mc;                /* DST candidates per step used for every request */
-    int32_t storage;           /* vector copy traversed */
+    int32_t storage;           /* vector copy traversed: GVS_STORE_F32, or GVS_STORE_BF16 /
+                                * GVS_STORE_FP8 (built for the index; one wave per request) */
     int32_t waves_per_request; /* 1 (0) = one wave per request; 2, 4, 8 = one workgroup of that many
                                 * waves per request (the intra-query search, DST(mc, 1); lower
                                 * latency for large ef) */
+    int32_t rerank;            /* BF16 / FP8 storage: 1 = exact fp32 rerank of each request's final
+                                * ef candidates, as in gvs_params */
 } gvs_server_config;
 
 GVS_API int gvs_server_open(gvs_index *ix, const gvs_server_config *cfg, gvs_server **out);

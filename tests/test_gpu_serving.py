@@ -166,6 +166,29 @@ def test_loadgen_several_threads_matches_alone(threads):
     assert st["completed"] == 2 * nq and st["submitted"] == 2 * nq
 
 
+@pytest.mark.parametrize("storage,rerank", [(gvs.BF16, True), (gvs.FP8, False), (gvs.FP8, True)])
+def test_low_precision_serving_matches_alone(storage, rerank):
+    """Serving over BF16 / FP8 storage (fp32 accumulation), with or without
+    the exact fp32 rerank of the final ef candidates: each request's result
+    equals the same query searched alone on the batch path with that
+    storage, rerank, k and ef."""
+    s = setup()
+    if "dev_lp" not in s:
+        s["dev_lp"] = gvs.DeviceIndex(s["g"], storages=(gvs.F32, gvs.BF16, gvs.FP8))
+    dev, q = s["dev_lp"], s["q"]
+    rng = np.random.default_rng(21)
+    ks = rng.choice([1, 10, 100], size=len(q)).astype(np.int32)
+    efs = rng.integers(32, 257, size=len(q)).astype(np.int32)
+    srv = gvs.Server(dev, max_inflight=256, max_k=100, max_ef=256, storage=storage, rerank=rerank)
+    lab, dist, lat, wall = srv.loadgen(q, 20000.0, ks, efs, seed=7)
+    srv.close()
+    for i in range(len(q)):
+        k, ef = int(ks[i]), int(efs[i])
+        alone, adist = dev.search(q[i:i + 1], k=k, ef=ef, storage=storage, rerank=rerank)
+        np.testing.assert_array_equal(lab[i, :k], alone[0])
+        np.testing.assert_array_equal(dist[i, :k], adist[0])
+
+
 def test_submit_batch_admits_a_prefix_and_publishes_it():
     """gvs_server_submit_batch admits requests in order up to the free slots;
     the rest are submitted after completions; every result equals the
