@@ -569,12 +569,6 @@ __device__ __forceinline__ void find_entry(const KParams &P, int64_t qi, const W
     find_entry_path<NCH, STORE, METRIC>(P, qi, L, qf, ep, epd, up_hops, up_evals, n2, n3);
 }
 
-
-// Returns true if the query's visited table overflowed (its results are
-// then label -1 / +inf and its status word is 1).
-// POOL: the query may start from an initial pool (init, init_len) instead
-// of the descent; a compile-time switch, so that the other kernels hold none
-// of that code.
 // The query in the lane layout eval_rows reads (lane p = lane & 7 holds the
 // elements (p + 8 i) * E + t), zero past dim.  A query in host memory
 // (sys_query: system-scope loads, fresh across the bus) is read once: lane l
@@ -640,6 +634,11 @@ __device__ __forceinline__ void put_stat(int32_t *row, int i, int32_t v, bool sy
         row[i] = v;
 }
 
+// Returns true if the query's visited table overflowed (its results are
+// then label -1 / +inf and its status word is 1).
+// POOL: the query may start from an initial pool (init, init_len) instead
+// of the descent; a compile-time switch, so that the other kernels hold none
+// of that code.
 template <int NCH, int STORE, int METRIC, int MROWS, bool POOL = false>
 __device__ bool search_query_req(const KParams &P, int64_t qi, int64_t slot, unsigned char *lds, int k, int ef,
                                  const float *Q, bool sys_query, const int32_t *init = nullptr, int init_len = 0)
