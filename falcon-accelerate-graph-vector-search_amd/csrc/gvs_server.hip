@@ -436,6 +436,11 @@ extern "C" int64_t gvs_server_submit_batch(gvs_server *s, const float *queries, 
     return submit_batch_impl(s, queries, n, ldq, ks, efs, req_ids, nullptr);
 }
 
+// completions ahead whose result rows a poll requests early
+#ifndef GVS_POLL_PREFETCH
+#define GVS_POLL_PREFETCH 6
+#endif
+
 // Deliver up to max_done completions, each ring's in the order the device
 // finished them, the rings taken round robin.  Claims of completion entries are serialised (poll_mu); the result
 // copies are not, so several threads may poll at once, each receiving its
@@ -506,8 +511,8 @@ static int poll_impl(gvs_server *s, int32_t max_done, int64_t *req_ids, int64_t 
     for (size_t j = 0; j < nc; j++) {
         // the device wrote these rows across the bus: the host reads them from
         // memory, so the rows of a few completions ahead are requested early
-        if (j + 6 < nc) {
-            const int ps = comp[j + 6].slot;
+        if (j + GVS_POLL_PREFETCH < nc) {
+            const int ps = comp[j + GVS_POLL_PREFETCH].slot;
             __builtin_prefetch(s->labels + (size_t)ps * K);
             __builtin_prefetch(s->dists + (size_t)ps * K);
         }
