@@ -54,6 +54,8 @@ def read_traffic(path, cfg):
         key += f" grid_waves={cfg['grid_waves']}"
     if cfg.get("region_order"):
         key += " region_order=1"
+    if cfg.get("waves_per_query", 1) > 1:
+        key += f" waves_per_query={cfg['waves_per_query']}"
     for line in open(path):
         line = line.strip()
         if line.startswith("traffic ") and " bytes=" in line and line.split(" bytes=")[0] == "traffic " + key:
@@ -576,6 +578,8 @@ def main():
             fixed["grid_waves"] = int(kv["grid_waves"])
         if int(kv.get("region_order", 0)):
             fixed["region_order"] = True
+        if int(kv.get("waves_per_query", 1)) > 1:
+            fixed["waves_per_query"] = int(kv["waves_per_query"])
         efs = [fixed["ef"]]
     plain_ef, plain_rec = None, None
     sweep = []
@@ -601,7 +605,7 @@ def main():
         cfg, ms = dev.autotune(dq.address, a.nq, a.dim, dlab.address, ddist.address, k=a.k, ef=ef,
                                stream=stream.handle)
         cfg = {kk: v for kk, v in cfg.items()
-               if not (kk in ("grid_waves", "region_order") and not v) and kk != "waves_per_query"}
+               if not (kk in ("grid_waves", "region_order") and not v) and not (kk == "waves_per_query" and v == 1)}
         rec = measure_recall(cfg)
         log(f"[bench] autotune ef={ef}: {cfg} recall={rec:.4f} {ms:.3f} ms")
         if rec < a.target:
@@ -734,6 +738,7 @@ def main():
             "ef": tuned_cfg["ef"], "mc": tuned_cfg["mc"], "mg": tuned_cfg.get("mg", 1),
             "persistent": tuned_cfg["persistent"], "grid_waves": tuned_cfg.get("grid_waves", 0),
             "region_order": bool(tuned_cfg.get("region_order", False)),
+            "waves_per_query": tuned_cfg.get("waves_per_query", 1),
             "recall_at_10": round(tuned_rec, 4), "plain_ef": plain_ef, "plain_recall_at_10": round(plain_rec, 4),
             "plain_qps": round(plain_qps, 1), "speedup_vs_plain": round(qps / plain_qps, 3),
             "avg_hops": round(float(avg[0]), 2), "avg_dist_evals": round(float(avg[1]), 2),

@@ -445,7 +445,7 @@ int plan_search(const gvs_index *ix, const gvs_params *p, int max_ef, int max_k,
     }
     int efe = std::max(max_ef, max_k);
     lp->fn = (void *)fn;
-    lp->dfn = W == 1 ? (void *)pick_descend(p->storage, ix->metric, nch, seq) : nullptr;
+    lp->dfn = (void *)pick_descend(p->storage, ix->metric, nch, seq);
     lp->mg = p->mg;
     lp->ef_cap = (efe + 63) / 64 * 64;
     lp->lcap = p->mc * rpc * WAVE;
@@ -507,12 +507,17 @@ int plan_search(const gvs_index *ix, const gvs_params *p, int max_ef, int max_k,
             set_error("gvs_search: per-query LDS %d bytes exceeds 160 KiB (ef too large)", lp->lds_per_wave);
             return GVS_ERR_UNSUPPORTED;
         }
-        lp->resident_waves = ix->cu_count * W;
         if (lp->lds_per_wave > 65536 &&
             !hip_ok(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                         lp->lds_per_wave),
                     "hipFuncSetAttribute"))
             return GVS_ERR_HIP;
+        // resident workgroups (the persistent grid), in waves
+        int bpc = 0;
+        if (!hip_ok(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, (const void *)fn, W * WAVE, lp->lds_per_wave),
+                    "occupancy"))
+            return GVS_ERR_HIP;
+        lp->resident_waves = std::max(1, bpc) * ix->cu_count * W;
         return GVS_OK;
     }
     const int rc = fit_waves(ix, (const void *)fn, lp);
@@ -621,8 +626,14 @@ void fill_params(const gvs_index *ix, const gvs_params *p, const LaunchPlan &lp,
 // launches.
 int64_t scratch_slots(const gvs_params *p, const LaunchPlan &lp, int64_t nq)
 {
-    if (p->waves_per_query > 1)  // one workgroup (one table) per query, in chunks
+    if (p->waves_per_query > 1) {
+        if (p->persistent) {  // the resident workgroups, one table each
+            const int64_t gw = p->grid_waves > 0 ? std::min<int64_t>(p->grid_waves, lp.resident_waves) : lp.resident_waves;
+            return std::max<int64_t>(1, std::min<int64_t>(nq, gw / lp.wpb));
+        }
+        // one workgroup (one table) per query, in chunks
         return std::min<int64_t>(std::min<int64_t>(nq, 65536), std::max<int64_t>(1, ((int64_t)1 << 30) >> lp.vlog2));
+    }
     if (p->persistent) {
         int64_t gw = p->grid_waves > 0 ? p->grid_waves : lp.resident_waves;
         gw = std::max<int64_t>(lp.wpb, std::min<int64_t>(gw, (nq + lp.wpb - 1) / lp.wpb * lp.wpb));
@@ -641,8 +652,8 @@ int64_t scratch_slots(const gvs_params *p, const LaunchPlan &lp, int64_t nq)
 static bool region_order(const gvs_index *ix, const gvs_params *p, const LaunchPlan &lp, const KParams &P,
                          int64_t nq)
 {
-    return (p->flags & GVS_FLAG_REGION_ORDER) && p->persistent && p->waves_per_query == 1 && !P.entries &&
-           !P.q_index && ix->max_level >= 2 && lp.dfn && nq >= 2 * lp.wpb;
+    return (p->flags & GVS_FLAG_REGION_ORDER) && p->persistent && !P.entries && !P.q_index && ix->max_level >= 2 &&
+           lp.dfn && nq >= 2 * lp.wpb;
 }
 
 // entries, keys and permutation (double-buffered for the sort) for nq queries
@@ -686,7 +697,7 @@ int launch_search(gvs_index *ix, const gvs_params *p, const LaunchPlan &lp, KPar
     const int64_t slots = scratch_slots(p, lp, nq);
     P.vtab = vtab;
     P.status = status;
-    if (p->waves_per_query > 1) {
+    if (p->waves_per_query > 1 && !p->persistent) {
         // one workgroup of waves_per_query waves per query, chunks of `slots`
         P.persistent = 0;
         hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, stream, ix->d_counters);
@@ -723,7 +734,9 @@ int launch_search(gvs_index *ix, const gvs_params *p, const LaunchPlan &lp, KPar
             P.keep_upper_stats = 1;
             P.xcd_queues = 1;
         }
-        hipLaunchKernelGGL(fn, dim3((unsigned)(slots / lp.wpb)), dim3(lp.wpb * WAVE), shm, stream, P);
+        // a grid of resident waves, or of resident workgroups with several waves per query
+        const int64_t blocks = p->waves_per_query > 1 ? slots : slots / lp.wpb;
+        hipLaunchKernelGGL(fn, dim3((unsigned)blocks), dim3(lp.wpb * WAVE), shm, stream, P);
         if (!hip_ok(hipGetLastError(), "launch k_search (persistent)")) return GVS_ERR_HIP;
         return GVS_OK;
     }
@@ -1117,7 +1130,10 @@ extern "C" int gvs_autotune(gvs_index *ix, const float *queries, int64_t nq, int
                     }
                 }
             }
-        // several waves per query pay off only when there are few queries
+        // several waves per query: for few queries, every shape; for full
+        // batches, two or four waves on the persistent queue (one query's
+        // waves share its pool and visited hash, so a query needs less LDS
+        // per wave and finishes sooner, which shortens the batch's tail)
         if (nq <= 64 && p->storage == GVS_STORE_F32)
             for (int W : {2, 4, 8})
                 for (int mg : {1, 2, 4}) {
@@ -1130,6 +1146,21 @@ extern "C" int gvs_autotune(gvs_index *ix, const float *queries, int64_t nq, int
                     c.flags = (p->flags | GVS_FLAG_ASYNC);
                     cand.push_back(c);
                 }
+        if (nq > 64 && mc >= 2 && p->storage == GVS_STORE_F32)
+            for (int W : {2, 4}) {
+                gvs_params c = *p;
+                c.mc = mc;
+                c.mg = 1;
+                c.persistent = 1;
+                c.waves_per_query = W;
+                c.grid_waves = 0;
+                c.flags = (p->flags | GVS_FLAG_ASYNC);
+                cand.push_back(c);
+                if (ix->max_level >= 2 && nq >= 4096) {
+                    c.flags |= GVS_FLAG_REGION_ORDER;
+                    cand.push_back(c);
+                }
+            }
     }
     // Above ef 128 the visited set's placement is timed too, unless the
     // caller fixed it: the LDS hash costs fewer instructions and HBM atomics

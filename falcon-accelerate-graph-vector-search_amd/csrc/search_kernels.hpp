@@ -116,9 +116,45 @@ template <int NCH, int STORE, int METRIC, int WROWS>
 __global__ __launch_bounds__(512) void k_search_mw(KParams P)
 {
     extern __shared__ __align__(16) unsigned char smem[];
-    const int64_t qi = P.q_base + blockIdx.x;
-    if (qi >= P.nq) return;
-    search_query_mw<NCH, STORE, METRIC, WROWS>(P, qi, blockIdx.x, smem);
+    if (!P.persistent) {
+        const int64_t qi = P.q_base + blockIdx.x;
+        if (qi >= P.nq) return;
+        search_query_mw<NCH, STORE, METRIC, WROWS>(P, qi, blockIdx.x, smem);
+        return;
+    }
+    // Persistent: the first lane takes the next query from the work queues
+    // the one-wave kernel uses (region order: its XCD's segment of the
+    // order, then the others'), and the workgroup searches it.  Every wave
+    // reads the claim after a barrier and leaves on the same value; the
+    // search's own barriers separate this read from the next claim.
+    __shared__ int64_t claim;
+    const int x0 = (int)(xcc_id() & 7u);
+    int seg = 0;
+    for (;;) {
+        if (threadIdx.x == 0) {
+            int64_t item = -1;
+            if (P.xcd_queues) {
+                while (seg < 8) {
+                    const int x = (x0 + seg) & 7;
+                    const int64_t s0 = P.nq * x / 8, s1 = P.nq * (x + 1) / 8;
+                    const uint32_t w = atomicAdd(&P.counters[4 + x], 1u);
+                    if (s0 + (int64_t)w < s1) {
+                        item = s0 + (int64_t)w;
+                        break;
+                    }
+                    seg++;
+                }
+            } else {
+                const uint32_t w = atomicAdd(&P.counters[0], 1u);
+                if ((int64_t)w < P.nq) item = (int64_t)w;
+            }
+            claim = item < 0 ? -1 : (P.q_perm ? (int64_t)P.q_perm[item] : item);
+        }
+        wg_sync();
+        const int64_t qi = claim;
+        if (qi < 0) break;
+        search_query_mw<NCH, STORE, METRIC, WROWS>(P, qi, blockIdx.x, smem);
+    }
 }
 
 // rows per wave = ceil(mc * ceil(r0/64) / waves) rounded up to 1, 2 or 4

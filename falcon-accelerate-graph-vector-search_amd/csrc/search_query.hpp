@@ -1230,8 +1230,10 @@ __device__ bool search_query_mw_req(const KParams &P, int64_t qi, int64_t slot, 
                 int32_t *row = P.out_stats + qi * GVS_STATS_ROW;
                 put_stat(row, 0, hops, sys_query);
                 put_stat(row, 1, (int)G.ctl[WG_EVALS], sys_query);
-                put_stat(row, 2, up_hops, sys_query);
-                put_stat(row, 3, up_evals, sys_query);
+                if (!P.keep_upper_stats) {  // region order: the descent kernel wrote them
+                    put_stat(row, 2, up_hops, sys_query);
+                    put_stat(row, 3, up_evals, sys_query);
+                }
 #ifdef GVS_PROFILE
                 // wave 0's phases: descent, wait at the step barrier, visit,
                 // gather + distances, filter, wait at the list barrier,
@@ -1258,7 +1260,8 @@ __device__ bool search_query_mw_req(const KParams &P, int64_t qi, int64_t slot, 
 template <int NCH, int STORE, int METRIC, int WROWS>
 __device__ __forceinline__ void search_query_mw(const KParams &P, int64_t qi, int64_t slot, unsigned char *lds)
 {
-    search_query_mw_req<NCH, STORE, METRIC, WROWS>(P, qi, slot, lds, P.k, max(P.ef, P.k), P.queries + qi * P.ldq,
+    const int64_t qrow = P.q_index ? P.q_index[qi] : qi;  // a retry row re-runs query q_index[qi]
+    search_query_mw_req<NCH, STORE, METRIC, WROWS>(P, qi, slot, lds, P.k, max(P.ef, P.k), P.queries + qrow * P.ldq,
                                                    false);
 }
 

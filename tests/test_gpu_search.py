@@ -482,6 +482,23 @@ def test_intra_query_waves_match_single_wave_dst(W, mc, mg):
         check_equal(dev, og, q, 10, ef, mc=mc, mg=mg, waves_per_query=W)
 
 
+@pytest.mark.parametrize("W,mc", [(2, 2), (2, 4), (4, 4), (4, 8)])
+def test_intra_query_waves_persistent_and_region_order(W, mc):
+    """Several waves per query on the persistent work queue, in arrival order
+    and in region order (descent kernel, sort, XCD segments): the ids,
+    distances and all four counters of the oracle's DST(mc, 1), with the
+    upper-layer counters from the descent kernel."""
+    x, g, dev = graph(20000, 128, 16, 100)
+    og = orc.OracleGraph(g.arrays())
+    q = queries(400, 128, seed=54)
+    assert g.arrays()["max_level"] >= 2
+    for kw in (dict(persistent=True), dict(persistent=True, region_order=True),
+               dict(persistent=True, region_order=True, grid_waves=8 * W)):
+        print(f"case intra persistent W={W} mc={mc} {kw}", flush=True)
+        check_equal(dev, og, q, 10, 64, mc=mc, waves_per_query=W, **kw)
+    check_equal(dev, og, q, 100, 300, mc=mc, waves_per_query=W, persistent=True, region_order=True)
+
+
 @pytest.mark.parametrize("W", [2, 8])
 def test_intra_query_waves_hbm_spill_ip_and_entries(W):
     """The intra-query path through the HBM visited table (forced at large
@@ -532,7 +549,7 @@ def test_autotuner_returns_a_working_configuration(nq):
     print(f"case autotune nq={nq}", flush=True)
     cfg, ms = dev.autotune(dq.address, nq, 128, dl.address, dd.address, k=10, ef=64)
     assert ms > 0 and cfg["ef"] == 64 and cfg["mc"] in (1, 2, 4, 8) and cfg["mg"] in (1, 2, 4)
-    assert cfg["waves_per_query"] in ((1, 2, 4, 8) if nq <= 64 else (1,))
+    assert cfg["waves_per_query"] in ((1, 2, 4, 8) if nq <= 64 else (1, 2, 4))
     kw = {kk: v for kk, v in cfg.items() if kk not in ("ef", "mc", "mg")}
     check_equal(dev, og, q, 10, 64, mc=cfg["mc"], mg=cfg["mg"], **kw)
     # above ef 128 the visited-set placement is one of the timed choices
