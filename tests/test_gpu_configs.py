@@ -3,8 +3,9 @@ oracle on the same seeded inputs and the same graph):
 
 * 1M x 96, M=32 (layer-0 degree 64), efC=200, 10k queries, ef 32/64/128/256
   (the plain inter-query configuration), every query;
-* 1M x 128, M=32, efC=200, 10k queries at the tuned point of the benchmark
-  (ef 58, mc 4, persistent, region order), every query;
+* 1M x 128, M=32, efC=200, 10k queries at the tuned points of the benchmark
+  (ef 56 and 58, mc 4, persistent, region order, one and two waves per
+  query), every query;
 * batch-1 intra-query DST on a degree-64 graph: 1/2/4/8 waves per query x
   mc 1/2/4/8 x ef 64/256/512, eager and as a captured graph;
 * an NSG-style flat graph of out-degree 64 traversed in BF16 and FP8-e4m3
@@ -51,11 +52,12 @@ def test_c2_full_size_every_query(ef):
     print(f"  mean hops {st[:, 0].mean():.1f} evals {st[:, 1].mean():.1f}", flush=True)
 
 
-def test_headline_tuned_config_every_query():
+@pytest.mark.parametrize("ef,W", [(58, 1), (56, 2)])
+def test_headline_tuned_config_every_query(ef, W):
     c = bench_graph(1_000_000, 128)
     assert c["g"].view.max_level >= 2
-    print("case 1M x 128 ef=58 mc=4 persistent region order: 10k queries", flush=True)
-    lab, st = check(c["dev"], c["og"], c["q"], 10, 58, mc=4, persistent=True, region_order=True)
+    print(f"case 1M x 128 ef={ef} mc=4 W={W} persistent region order: 10k queries", flush=True)
+    lab, st = check(c["dev"], c["og"], c["q"], 10, ef, mc=4, persistent=True, region_order=True, waves_per_query=W)
     gt, _ = orc.bruteforce_f64(c["x"], c["q"][:1000], 10, nthreads=16)
     rec = orc.recall_at(lab[:1000], gt, 10)
     print(f"  recall@10 on 1k queries {rec:.4f}", flush=True)
