@@ -509,10 +509,13 @@ def main():
                         grid_waves=max(a.serve_grid, 0), nreq=a.serve_requests, waves_per_request=a.serve_w,
                         mc=a.serve_mc)
             return
-        # one wave per request (throughput), then one workgroup of 8 waves
-        # running DST(8, 1) per request (latency of the heavy requests)
+        # one wave per request (the plain search's configuration), one
+        # workgroup of 4 waves running DST(8, 1) per request (the highest rate
+        # and p99 under 1 ms at half load), and of 8 waves (the lowest latency)
         serve_child(a, gvs, g, q, ks=ks, efs=efs, metric_name="inner product, unit-norm", max_inflight=4096,
                     grid_waves=768, nreq=30000)
+        serve_child(a, gvs, g, q, ks=ks, efs=efs, metric_name="inner product, unit-norm", max_inflight=4096,
+                    grid_waves=0, nreq=30000, waves_per_request=4, mc=8)
         serve_child(a, gvs, g, q, ks=ks, efs=efs, metric_name="inner product, unit-norm", max_inflight=4096,
                     grid_waves=1024, nreq=30000, waves_per_request=8, mc=8)
         return
