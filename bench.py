@@ -248,6 +248,21 @@ def precision_c4(a, gvs, g, x, q):
         r = {"storage": names[storage], "rerank": rerank, "recall_at_10": round(recall_at(lab, gt, a.k), 4),
              "qps": round(nq / (ms / 1e3), 1), "ms_per_batch": round(ms, 3),
              "dist_rel_err_mean": float(f"{rel.mean():.3e}"), "dist_rel_err_max": float(f"{rel.max():.3e}")}
+        # the library autotuner's launch for this storage (DST mc / mg, persistent grid, waves per query)
+        cfg, _ = dev.autotune(dq.address, nq, a.dim, dl.address, dd.address, stream=st.handle, **kw)
+        tkw = dict(kw, **cfg)
+        dev.search_device(dq.address, nq, a.dim, dl.address, dd.address, stream=st.handle, **tkw)
+        st.synchronize()
+        tlab = dl.to_host()
+        e0.record(st)
+        for _ in range(3):
+            dev.search_device(dq.address, nq, a.dim, dl.address, dd.address, stream=st.handle, asynchronous=True,
+                              **tkw)
+        e1.record(st)
+        e1.synchronize()
+        tms = e0.elapsed_ms(e1) / 3
+        r["tuned"] = {"config": {kk: v for kk, v in cfg.items() if kk not in ("k", "ef")},
+                      "recall_at_10": round(recall_at(tlab, gt, a.k), 4), "qps": round(nq / (tms / 1e3), 1)}
         rows.append(r)
         log(f"[c4] {r}")
     # NSG search from a pool (nsg.py:62-125): the navigating node's first
