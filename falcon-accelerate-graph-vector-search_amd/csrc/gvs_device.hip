@@ -503,7 +503,7 @@ int plan_search(const gvs_index *ix, const gvs_params *p, int max_ef, int max_k,
                     (lp->hlog2 > 0 ? 4 * (1 << lp->hlog2) : 0);
         lp->lds_per_wave = (bytes + 15) / 16 * 16;  // per workgroup
         lp->wpb = W;
-        if (lp->lds_per_wave > 160 * 1024) {
+        if (lp->lds_per_wave > 160 * 1024 - 64) {  // (the persistent loop's claim word is static LDS)
             set_error("gvs_search: per-query LDS %d bytes exceeds 160 KiB (ef too large)", lp->lds_per_wave);
             return GVS_ERR_UNSUPPORTED;
         }

@@ -10,6 +10,10 @@ template <int NCH, int STORE, int METRIC, int MROWS, bool POOL = false>
 #ifndef GVS_MIN_WAVES_PER_EU
 #define GVS_MIN_WAVES_PER_EU 1
 #endif
+// waves per SIMD the descent kernel is compiled to allow (its registers)
+#ifndef GVS_DESCEND_WAVES_PER_EU
+#define GVS_DESCEND_WAVES_PER_EU 1
+#endif
 __global__ __launch_bounds__(256, GVS_MIN_WAVES_PER_EU) void k_search(KParams P)
 {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -63,7 +67,7 @@ __global__ __launch_bounds__(256, GVS_MIN_WAVES_PER_EU) void k_search(KParams P)
 // upper-layer counters (into out_stats[.., 2..3]) and a region key from the
 // nodes the descent settled on at levels 3 and 2, for ordering the batch.
 template <int NCH, int STORE, int METRIC>
-__global__ __launch_bounds__(256) void k_descend(KParams P, int32_t *ep_out, uint64_t *key_out, int32_t *perm_out)
+__global__ __launch_bounds__(256, GVS_DESCEND_WAVES_PER_EU) void k_descend(KParams P, int32_t *ep_out, uint64_t *key_out, int32_t *perm_out)
 {
     constexpr int E = StoreTraits<STORE>::E;
     __shared__ uint64_t keys[4][WAVE];
