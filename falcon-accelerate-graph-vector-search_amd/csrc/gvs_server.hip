@@ -104,9 +104,17 @@ struct gvs_server {
     uint64_t next_seq = 1;
     int64_t next_req = 0;
     int64_t c_submitted = 0, c_completed = 0, c_cancelled = 0, c_rejected = 0, c_loops = 0, c_overflowed = 0;
+    int64_t inflight_bytes = 0;  // mu: queue memory held by admitted requests (config queue_bytes)
     // host-side breakdown of the last gvs_server_loadgen (see gvs.h)
     double lg_breakdown[6] = {0, 0, 0, 0, 0, 0};
 };
+
+// Queue memory a request of k results holds while in flight: its query,
+// its k result rows (id and distance), its counters and control words.
+static int64_t request_bytes(const gvs_server *s, int k)
+{
+    return 4 * (int64_t)s->ix->dim + 12 * (int64_t)k + 4 * GVS_STATS_ROW + 16;
+}
 
 static void serve_free(gvs_server *s)
 {
@@ -161,7 +169,7 @@ extern "C" int gvs_server_open(gvs_index *ix, const gvs_server_config *cfg, gvs_
     const int W = cfg ? (cfg->waves_per_request > 0 ? cfg->waves_per_request : 1) : 1;
     if (!ix || !cfg || !out || cfg->max_inflight < 1 || cfg->max_inflight > (1 << 20) || cfg->max_k < 1 ||
         cfg->max_ef < 1 || cfg->mc < 0 || cfg->storage < GVS_STORE_F32 || cfg->storage > GVS_STORE_FP8 ||
-        (W != 1 && W != 2 && W != 4 && W != 8) || (W > 1 && cfg->storage != GVS_STORE_F32)) {
+        (W != 1 && W != 2 && W != 4 && W != 8) || (W > 1 && cfg->storage != GVS_STORE_F32) || cfg->queue_bytes < 0) {
         gvs::set_error("gvs_server_open: invalid configuration (fp32, BF16 or FP8 storage; 1, 2, 4 or 8 waves per "
                        "request, several only with fp32)");
         return GVS_ERR_ARG;
@@ -169,6 +177,12 @@ extern "C" int gvs_server_open(gvs_index *ix, const gvs_server_config *cfg, gvs_
     gvs_server *s = new gvs_server();
     s->ix = ix;
     s->cfg = *cfg;
+    if (cfg->queue_bytes > 0 && cfg->queue_bytes < request_bytes(s, cfg->max_k)) {
+        gvs::set_error("gvs_server_open: queue_bytes %d cannot hold one request of k = max_k (%lld bytes)",
+                       cfg->queue_bytes, (long long)request_bytes(s, cfg->max_k));
+        delete s;
+        return GVS_ERR_ARG;
+    }
     gvs_params p;
     std::memset(&p, 0, sizeof p);
     p.k = cfg->max_k;
@@ -366,6 +380,14 @@ static int64_t submit_batch_impl(gvs_server *s, const float *queries, int64_t n,
                 err = GVS_ERR_STATE;
                 break;
             }
+            const int64_t rb = request_bytes(s, k);
+            if (s->cfg.queue_bytes > 0 && s->inflight_bytes + rb > s->cfg.queue_bytes) {
+                gvs::set_error("gvs_server_submit: queue memory taken (%lld of %d bytes in flight)",
+                               (long long)s->inflight_bytes, s->cfg.queue_bytes);
+                err = GVS_ERR_STATE;
+                break;
+            }
+            s->inflight_bytes += rb;
             s->rr = (q + 1) % SERVE_QUEUES;
             const int slot = s->free_slots.back();
             s->free_slots.pop_back();
@@ -542,6 +564,7 @@ static int poll_impl(gvs_server *s, int32_t max_done, int64_t *req_ids, int64_t 
         if (!c.cancelled) s->c_completed++;
         if (c.ovf) s->c_overflowed++;
         sl.busy = false;
+        s->inflight_bytes -= request_bytes(s, sl.k);
         s->ring_busy[sl.ring_idx] = 0;
         s->free_slots.push_back(c.slot);
         const int ip = s->inflight_pos[(size_t)c.slot];

@@ -54,7 +54,7 @@ class Params(C.Structure):
 
 class ServerConfig(C.Structure):
     _fields_ = [(n, C.c_int32) for n in (
-        "max_inflight", "max_k", "max_ef", "queue_capacity", "grid_waves", "mc", "storage", "waves_per_request",
+        "max_inflight", "max_k", "max_ef", "queue_bytes", "grid_waves", "mc", "storage", "waves_per_request",
         "rerank")]
 
 
@@ -479,11 +479,11 @@ class SearchGraph:
 class Server:
     """Continuous-batching serving loop over a DeviceIndex (C ABI gvs_server_*)."""
 
-    def __init__(self, dev, max_inflight=1024, max_k=100, max_ef=512, queue_capacity=0, grid_waves=0, mc=1,
+    def __init__(self, dev, max_inflight=1024, max_k=100, max_ef=512, queue_bytes=0, grid_waves=0, mc=1,
                  waves_per_request=1, storage=F32, rerank=False):
         self.dev = dev
         self.max_k = max_k
-        cfg = ServerConfig(max_inflight, max_k, max_ef, queue_capacity, grid_waves, mc, storage, waves_per_request,
+        cfg = ServerConfig(max_inflight, max_k, max_ef, queue_bytes, grid_waves, mc, storage, waves_per_request,
                            int(bool(rerank)))
         self._h = C.c_void_p()
         _check(lib.gvs_server_open(dev.handle, C.byref(cfg), C.byref(self._h)))
@@ -496,7 +496,8 @@ class Server:
         return r
 
     def try_submit(self, query, k, ef):
-        """Like submit, but returns None when all device slots are busy."""
+        """Like submit, but returns None when all device slots are busy or the
+        queue memory is taken."""
         q = np.ascontiguousarray(query, dtype=np.float32)
         r = lib.gvs_server_submit(self._h, _ptr(q), k, ef)
         if r == -7:

@@ -352,7 +352,10 @@ typedef struct gvs_server_config {
     int32_t max_inflight;      /* device query slots */
     int32_t max_k;             /* largest k accepted */
     int32_t max_ef;            /* largest ef accepted */
-    int32_t queue_capacity;    /* reserved, 0 (requests wait in the caller until a slot frees) */
+    int32_t queue_bytes;       /* queue memory the in-flight requests may hold, in bytes (0 = only the
+                                * slots limit them): a request holds its query (4 dim), its k result
+                                * rows (12 k), its counters and control words; one that does not fit
+                                * waits in the caller, as for a busy slot */
     int32_t grid_waves;        /* persistent workers, 0 = auto (resident
                                 * waves, at most max_inflight); heavy
                                 * mixes can saturate higher below that */
@@ -371,10 +374,13 @@ GVS_API int gvs_server_open(gvs_index *ix, const gvs_server_config *cfg, gvs_ser
  * ring positions and of completion entries are serialised, the request
  * writes and result copies are not, and each poll returns completions no
  * other poll returns. */
-/* Enqueue one request (host query of dim floats).  Returns a request id >= 0,
- * GVS_ERR_ARG when k or ef is outside the admitted range (counted as
- * rejected), or GVS_ERR_STATE when all max_inflight device slots are busy
- * (the caller polls and retries). */
+/* Enqueue one request (host query of dim floats).  Admission is by memory:
+ * the request's ef and k must fit the LDS the serving waves were planned
+ * with (ef <= max_ef, k <= max_k), and a slot and its queue memory must be
+ * free.  Returns a request id >= 0, GVS_ERR_ARG when k or ef is outside the
+ * admitted range (counted as rejected), or GVS_ERR_STATE when all
+ * max_inflight device slots are busy or the queue memory is taken (the
+ * caller polls and retries). */
 GVS_API int64_t gvs_server_submit(gvs_server *s, const float *query, int32_t k, int32_t ef);
 /* Enqueue up to n requests (host queries, row stride ldq floats, with their
  * own ks[i] and efs[i]) in order, stopping at the first that finds no free

@@ -189,6 +189,36 @@ def test_low_precision_serving_matches_alone(storage, rerank):
         np.testing.assert_array_equal(dist[i, :k], adist[0])
 
 
+def test_admission_by_queue_memory():
+    """queue_bytes bounds the memory the in-flight requests hold (query, k
+    result rows, counters, control words): three k=100 requests fill a
+    budget of three, and the same budget then admits eight k=1 requests;
+    memory comes back as completions are polled, and every result equals the
+    query searched alone."""
+    s = setup()
+    dev, q = s["dev"], s["q"]
+    rb = lambda k: 4 * 128 + 12 * k + 4 * gvs.STATS_ROW + 16
+    with pytest.raises(gvs.GvsError):
+        gvs.Server(dev, max_inflight=64, max_k=100, max_ef=128, queue_bytes=rb(100) - 1)
+    budget = 3 * rb(100)
+    srv = gvs.Server(dev, max_inflight=64, max_k=100, max_ef=128, queue_bytes=budget)
+    big = [srv.try_submit(q[i], 100, 64) for i in range(4)]
+    assert all(r is not None for r in big[:3]) and big[3] is None
+    assert srv.try_submit(q[4], 1, 64) is None  # the memory is taken until the completions are polled
+    got = drain(srv, 3)
+    for j, i in enumerate(range(3)):
+        alone, _ = dev.search(q[i:i + 1], k=100, ef=64)
+        np.testing.assert_array_equal(got[big[j]][0][:100], alone[0])
+    n_small = budget // rb(1)
+    small = [srv.try_submit(q[10 + i], 1, 64) for i in range(n_small + 1)]
+    assert all(r is not None for r in small[:n_small]) and small[n_small] is None
+    got = drain(srv, n_small)
+    for j in range(n_small):
+        alone, _ = dev.search(q[10 + j:11 + j], k=1, ef=64)
+        assert got[small[j]][0][0] == alone[0][0]
+    srv.close()
+
+
 def test_submit_batch_admits_a_prefix_and_publishes_it():
     """gvs_server_submit_batch admits requests in order up to the free slots;
     the rest are submitted after completions; every result equals the
