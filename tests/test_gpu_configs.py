@@ -2,7 +2,8 @@
 oracle on the same seeded inputs and the same graph):
 
 * 1M x 96, M=32 (layer-0 degree 64), efC=200, 10k queries, ef 32/64/128/256
-  (the plain inter-query configuration), every query;
+  (the plain inter-query configuration) and the bench's tuned point (ef 52,
+  mc 4, two waves per query, persistent, region order), every query;
 * 1M x 128, M=32, efC=200, 10k queries at the tuned points of the benchmark
   (ef 56 and 58, mc 4, persistent, region order, one and two waves per
   query), every query;
@@ -50,6 +51,14 @@ def test_c2_full_size_every_query(ef):
     print(f"case 1M x 96 ef={ef}: 10k queries", flush=True)
     lab, st = check(c["dev"], c["og"], c["q"], 10, ef)
     print(f"  mean hops {st[:, 0].mean():.1f} evals {st[:, 1].mean():.1f}", flush=True)
+
+
+def test_c2_tuned_point_every_query():
+    """The tuned point the bench found on 1M x 96: ef 52, DST mc 4, two waves
+    per query, persistent, region order."""
+    c = bench_graph(1_000_000, 96)
+    print("case 1M x 96 ef=52 mc=4 W=2 persistent region order: 10k queries", flush=True)
+    check(c["dev"], c["og"], c["q"], 10, 52, mc=4, persistent=True, region_order=True, waves_per_query=2)
 
 
 @pytest.mark.parametrize("ef,W", [(58, 1), (56, 2)])

@@ -497,6 +497,10 @@ def test_intra_query_waves_persistent_and_region_order(W, mc):
         print(f"case intra persistent W={W} mc={mc} {kw}", flush=True)
         check_equal(dev, og, q, 10, 64, mc=mc, waves_per_query=W, **kw)
     check_equal(dev, og, q, 100, 300, mc=mc, waves_per_query=W, persistent=True, region_order=True)
+    xi, gi, devi = graph(8000, 96, 12, 100, metric=gvs.IP, normalize=True)
+    ogi = orc.OracleGraph(gi.arrays())
+    qi = queries(200, 96, seed=55, normalize=True)
+    check_equal(devi, ogi, qi, 10, 64, mc=mc, waves_per_query=W, persistent=True, region_order=True)
 
 
 @pytest.mark.parametrize("W", [2, 8])
