@@ -465,8 +465,8 @@ def main():
     a = ap.parse_args()
 
     if a.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for --gpus {a.gpus}); "
-              "the graph search is replicated per GPU with queries split round-robin (see DESIGN.md)",
+        print(f"bench.py: runs on one GPU only (asked for --gpus {a.gpus}); on several, each GPU would "
+              "hold a replica of the graph and vectors and take a round-robin share of the queries",
               file=sys.stderr)
         sys.exit(2)
 

@@ -40,7 +40,7 @@ def load_case(name):
 
 def divergent_queries(ids_a, d_a, ids_b, d_b, rel=REL):
     """Rows whose ids differ at a position where the distances do not tie
-    within `rel` (the north-star tie rule)."""
+    within `rel` (ties within 1e-5 relative may order either way)."""
     bad = []
     for r in range(ids_a.shape[0]):
         for c in np.nonzero(ids_a[r] != ids_b[r])[0]:
