@@ -133,7 +133,7 @@ def _load():
 
 
 lib = _load()
-# stats row width of this build (4 counters; more in a phase-profiling build)
+# stats row width of the loaded library (4 counters; 16 when compiled with phase profiling)
 STATS_ROW = lib.gvs_stats_per_query()
 
 
