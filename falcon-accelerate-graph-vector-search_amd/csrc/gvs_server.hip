@@ -7,13 +7,13 @@
 // slots free up instead of in fixed batches, each with its own k and ef.
 //
 // Host and device share pinned (fine-grained) host memory: the host writes a
-// request into a free slot (query, k, ef), appends the slot to a ring and
-// bumps its `tail`; device waves claim ring entries with a CAS on a device
-// counter, search, write the results into the slot's pinned result record
-// and append the slot to a completion ring, which the host consumes.  Device
-// waves leave the loop on the stop flag or after an idle timeout; the host
-// relaunches the kernel when work is pending and the previous launch has
-// finished.
+// request into a free slot (query, k, ef, sequence number), appends the slot
+// to the ring and bumps `tail`; device waves claim ring entries with a CAS on
+// a device counter, search, write the results into the slot's pinned result
+// rows and store the sequence number into done[slot].  The host polls the
+// done flags of its in-flight slots.  Device waves leave the loop on the stop
+// flag or after an idle timeout; the host relaunches the kernel when work is
+// pending and the previous launch has finished.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -72,13 +72,9 @@ struct gvs_server {
     uint32_t *ring = nullptr;
     float *qbuf = nullptr;
     int32_t *qk = nullptr, *qef = nullptr;
-    // result records, one per slot, of rec_bytes (whole cache lines): the
-    // slot's max_k labels, then its max_k distances, so that the host reads a
-    // request's results from the fewest lines the device wrote
-    unsigned char *res = nullptr;
-    size_t rec_bytes = 0;
-    int64_t *labels = nullptr;  // res: labels of slot i at labels + i * rec_bytes / 8
-    float *dists = nullptr;     // res + 8 max_k: distances of slot i at dists + i * rec_bytes / 4
+    uint64_t *qseq = nullptr, *done = nullptr;
+    int64_t *labels = nullptr;
+    float *dists = nullptr;
     int32_t *stats = nullptr;
     // device
     uint64_t *d_head = nullptr, *d_served = nullptr, *d_mirror = nullptr;
@@ -95,16 +91,17 @@ struct gvs_server {
     std::vector<Slot> slots;
     std::vector<int> free_slots, inflight;
     std::vector<int> inflight_pos;   // slot -> index in inflight
-    uint32_t *cring = nullptr;       // pinned completion rings (written by the device)
-    uint64_t chead[SERVE_CQUEUES] = {}; // poll_mu: completions consumed per ring
+    uint64_t *cring = nullptr;       // pinned completion rings, one per request ring (written by the device)
+    uint64_t chead[SERVE_QUEUES] = {};  // poll_mu: completions consumed per ring
     int cnext = 0;                      // poll_mu: ring a poll looks at first
     double last_query_us = -1e30;    // last hipStreamQuery
     std::vector<uint8_t> ring_busy;  // ring entry (q * rcap + i) still names an incomplete request
-    struct Admitted { int slot; size_t rpos; int64_t m; };
+    struct Admitted { int slot; size_t rpos; int64_t m; uint64_t seq; };
     struct Done { int slot; bool ovf, cancelled; int64_t req, tag; double t_arrival; int k; };
     uint64_t tails[SERVE_QUEUES] = {};                  // submit_mu: ring positions claimed
     std::atomic<uint64_t> published[SERVE_QUEUES]{};    // ... and published to the device, in claim order
     int rr = 0;                              // ring the next request tries first
+    uint64_t next_seq = 1;
     int64_t next_req = 0;
     int64_t c_submitted = 0, c_completed = 0, c_cancelled = 0, c_rejected = 0, c_loops = 0, c_overflowed = 0;
     int64_t inflight_bytes = 0;  // mu: queue memory held by admitted requests (config queue_bytes)
@@ -128,8 +125,11 @@ static void serve_free(gvs_server *s)
     (void)hipHostFree(s->qbuf);
     (void)hipHostFree(s->qk);
     (void)hipHostFree(s->qef);
+    (void)hipHostFree(s->qseq);
+    (void)hipHostFree(s->done);
     (void)hipHostFree(s->cring);
-    (void)hipHostFree(s->res);
+    (void)hipHostFree(s->labels);
+    (void)hipHostFree(s->dists);
     (void)hipHostFree(s->stats);
     (void)hipFree(s->d_head);
     (void)hipFree(s->d_mirror);
@@ -251,31 +251,28 @@ extern "C" int gvs_server_open(gvs_index *ix, const gvs_server_config *cfg, gvs_
     // Requests go to the rings round robin.
     uint32_t rcap = 16;
     while (rcap < 4u * (uint32_t)((R + SERVE_QUEUES - 1) / SERVE_QUEUES)) rcap <<= 1;
-    // completion ring c takes the completions of request rings c, c + 4, ...:
-    // at most rcap entries of each are unread (a request's ring entry is
-    // reused only after its completion was polled), and the ring is twice
-    // that, so that the entry a poll reads is either the one it waits for or
-    // the one of the previous lap
-    const uint32_t ccap = 2u * rcap * (uint32_t)(SERVE_QUEUES / SERVE_CQUEUES);
-    uint32_t clog2 = 0;
-    while ((1u << clog2) < ccap) clog2++;
+    // completion ring q: at most rcap entries are unconsumed (a request's
+    // ring entry is reused only after its completion was polled)
+    const uint32_t ccap = 2u * rcap;
     const int dim = ix->dim;
-    s->rec_bytes = ((size_t)12 * cfg->max_k + 63) / 64 * 64;
     bool ok = gvs::hip_ok(hipHostMalloc((void **)&s->ctl, sizeof(ServeCtl), hipHostMallocCoherent), "hipHostMalloc") &&
               gvs::hip_ok(hipHostMalloc((void **)&s->ring, 4 * (size_t)rcap * SERVE_QUEUES, hipHostMallocCoherent),
                           "hipHostMalloc") &&
               gvs::hip_ok(hipHostMalloc((void **)&s->qbuf, 4 * (size_t)R * dim, hipHostMallocCoherent), "hipHostMalloc") &&
               gvs::hip_ok(hipHostMalloc((void **)&s->qk, 4 * (size_t)R, hipHostMallocCoherent), "hipHostMalloc") &&
               gvs::hip_ok(hipHostMalloc((void **)&s->qef, 4 * (size_t)R, hipHostMallocCoherent), "hipHostMalloc") &&
-              gvs::hip_ok(hipHostMalloc((void **)&s->cring, 4 * (size_t)ccap * SERVE_CQUEUES, hipHostMallocCoherent), "hipHostMalloc") &&
-              gvs::hip_ok(hipHostMalloc((void **)&s->res, s->rec_bytes * (size_t)R, hipHostMallocCoherent), "hipHostMalloc") &&
+              gvs::hip_ok(hipHostMalloc((void **)&s->qseq, 8 * (size_t)R, hipHostMallocCoherent), "hipHostMalloc") &&
+              gvs::hip_ok(hipHostMalloc((void **)&s->done, 8 * (size_t)R, hipHostMallocCoherent), "hipHostMalloc") &&
+              gvs::hip_ok(hipHostMalloc((void **)&s->cring, 8 * (size_t)ccap * SERVE_QUEUES, hipHostMallocCoherent), "hipHostMalloc") &&
+              gvs::hip_ok(hipHostMalloc((void **)&s->labels, 8 * (size_t)R * cfg->max_k, hipHostMallocCoherent), "hipHostMalloc") &&
+              gvs::hip_ok(hipHostMalloc((void **)&s->dists, 4 * (size_t)R * cfg->max_k, hipHostMallocCoherent), "hipHostMalloc") &&
               gvs::hip_ok(hipHostMalloc((void **)&s->stats, 4 * GVS_STATS_ROW * (size_t)R, hipHostMallocCoherent), "hipHostMalloc") &&
               gvs::hip_ok(hipMalloc(&s->d_head, 8 * SERVE_QUEUES * SERVE_HEAD_STRIDE), "hipMalloc") &&
-              gvs::hip_ok(hipMalloc(&s->d_served, 8 * SERVE_CQUEUES * SERVE_HEAD_STRIDE), "hipMalloc") &&
+              gvs::hip_ok(hipMalloc(&s->d_served, 8 * SERVE_QUEUES * SERVE_HEAD_STRIDE), "hipMalloc") &&
               gvs::hip_ok(hipMalloc(&s->d_mirror, 8 * SERVE_MIRROR_WORDS), "hipMalloc") &&
               gvs::hip_ok(hipMemset(s->d_mirror, 0, 8 * SERVE_MIRROR_WORDS), "hipMemset") &&
               gvs::hip_ok(hipMemset(s->d_head, 0, 8 * SERVE_QUEUES * SERVE_HEAD_STRIDE), "hipMemset") &&
-              gvs::hip_ok(hipMemset(s->d_served, 0, 8 * SERVE_CQUEUES * SERVE_HEAD_STRIDE), "hipMemset") &&
+              gvs::hip_ok(hipMemset(s->d_served, 0, 8 * SERVE_QUEUES * SERVE_HEAD_STRIDE), "hipMemset") &&
               gvs::hip_ok(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "hipStreamCreateWithFlags");
     const size_t vwords = (size_t)s->grid_waves << s->lp.vlog2;
     if (ok)
@@ -286,11 +283,11 @@ extern "C" int gvs_server_open(gvs_index *ix, const gvs_server_config *cfg, gvs_
         serve_free(s);
         return GVS_ERR_HIP;
     }
-    s->labels = (int64_t *)s->res;
-    s->dists = (float *)(s->res + 8 * (size_t)cfg->max_k);
     std::memset(s->ctl, 0, sizeof(ServeCtl));
-    std::memset(s->cring, 0, 4 * (size_t)ccap * SERVE_CQUEUES);
+    std::memset(s->done, 0, 8 * (size_t)R);
+    std::memset(s->cring, 0, 8 * (size_t)ccap * SERVE_QUEUES);
     s->inflight_pos.assign((size_t)R, -1);
+    std::memset(s->qseq, 0, 8 * (size_t)R);
     s->slots.assign((size_t)R, Slot());
     s->ring_busy.assign((size_t)rcap * SERVE_QUEUES, 0);
     for (int64_t i = R - 1; i >= 0; i--) s->free_slots.push_back((int)i);
@@ -301,8 +298,6 @@ extern "C" int gvs_server_open(gvs_index *ix, const gvs_server_config *cfg, gvs_
     P.ldq = dim;
     P.out_labels = s->labels;
     P.out_dists = s->dists;
-    P.out_ld_lab = (int)(s->rec_bytes / 8);
-    P.out_ld_dist = (int)(s->rec_bytes / 4);
     P.out_stats = s->stats;
     P.out_k_only = 1;  // only a request's own k results cross the bus
     P.persistent = 1;
@@ -316,9 +311,10 @@ extern "C" int gvs_server_open(gvs_index *ix, const gvs_server_config *cfg, gvs_
     s->args.qbuf = s->qbuf;
     s->args.qk = s->qk;
     s->args.qef = s->qef;
+    s->args.qseq = s->qseq;
+    s->args.done = s->done;
     s->args.cring = s->cring;
     s->args.ccap = ccap;
-    s->args.clog2 = clog2;
     s->args.nslots = (uint32_t)R;
     s->args.head = s->d_head;
     s->args.mirror = s->d_mirror;
@@ -410,7 +406,7 @@ static int64_t submit_batch_impl(gvs_server *s, const float *queries, int64_t n,
             s->inflight.push_back(slot);
             s->c_submitted++;
             if (req_ids) req_ids[m] = sl.req;
-            adm.push_back({slot, rpos, m});
+            adm.push_back({slot, rpos, m, s->next_seq++});
         }
         for (int q = 0; q < SERVE_QUEUES; q++) last[q] = s->tails[q];
     }
@@ -420,6 +416,7 @@ static int64_t submit_batch_impl(gvs_server *s, const float *queries, int64_t n,
         std::memcpy(s->qbuf + (size_t)a.slot * dim, queries + a.m * ldq, sizeof(float) * dim);
         s->qk[a.slot] = ks[a.m];
         s->qef[a.slot] = efs[a.m];
+        s->qseq[a.slot] = a.seq;
         s->ring[a.rpos] = (uint32_t)a.slot;
     }
     if (!adm.empty()) {
@@ -472,11 +469,8 @@ extern "C" int64_t gvs_server_submit_batch(gvs_server *s, const float *queries, 
 // own completions.  The device wrote each request's own k results into its
 // slot's rows; rows are delivered max_k wide, the rest label -1 / dist +inf.
 // tags_out (may be NULL) receives the tags the requests were submitted with.
-// With tag_rows > 0 (the load generator), a request's labels, dists and
-// latency go to row `tag` instead of row n, for tags in [0, tag_rows) only.
 static int poll_impl(gvs_server *s, int32_t max_done, int64_t *req_ids, int64_t *labels, float *dists,
-                     double *latency_us, int32_t *stats_out, int32_t *status_out, int64_t *tags_out = nullptr,
-                     int64_t tag_rows = 0)
+                     double *latency_us, int32_t *stats_out, int32_t *status_out, int64_t *tags_out = nullptr)
 {
     const int K = s->cfg.max_k;
     const uint64_t cmask = (uint64_t)s->args.ccap - 1u;
@@ -485,35 +479,30 @@ static int poll_impl(gvs_server *s, int32_t max_done, int64_t *req_ids, int64_t 
         gvs::set_error("gvs_server_poll: the device read an out-of-range slot index from ring entry %u", err - 1u);
         return GVS_ERR_STATE;
     }
-    // Entry i of a completion ring is written once its top bit is
-    // ((i >> clog2) + 1) & 1 (the lap); then come the overflow bit and the
-    // slot.  Four rings of 4-byte entries keep the lines a poll touches few:
-    // the first read of a line after the device wrote it takes one to three
-    // microseconds, and each ring's newest line is read on every poll.
-    const uint32_t clog2 = s->args.clog2;
+    // entry i of a completion ring carries (i + 1) in its high word once it
+    // is written, then the overflow bit and the slot
     thread_local std::vector<gvs_server::Done> comp;
     comp.clear();
     {
         std::lock_guard<std::mutex> lk(s->poll_mu);
-        for (int j = 0; j < SERVE_CQUEUES && (int32_t)comp.size() < max_done; j++) {
-            const int c = (s->cnext + j) % SERVE_CQUEUES;
-            const uint32_t *cr = s->cring + (size_t)c * s->args.ccap;
+        for (int j = 0; j < SERVE_QUEUES && (int32_t)comp.size() < max_done; j++) {
+            const int q = (s->cnext + j) % SERVE_QUEUES;
+            const uint64_t *cr = s->cring + (size_t)q * s->args.ccap;
             while ((int32_t)comp.size() < max_done) {
-                const uint64_t h = s->chead[c];
-                const uint32_t v = __atomic_load_n(&cr[h & cmask], __ATOMIC_ACQUIRE);
-                if ((v >> 31) != (uint32_t)(((h >> clog2) + 1u) & 1u)) break;
-                if ((h & 15u) == 0u) __builtin_prefetch(&cr[(h + 16u) & cmask]);
-                const int slot = (int)(v & 0x3FFFFFFFu);
+                const uint64_t h = s->chead[q];
+                const uint64_t v = __atomic_load_n(&cr[h & cmask], __ATOMIC_ACQUIRE);
+                if ((uint32_t)(v >> 32) != (uint32_t)(h + 1u)) break;
+                const int slot = (int)((uint32_t)v & 0x7FFFFFFFu);
                 if (slot >= s->cfg.max_inflight) {
-                    gvs::set_error("gvs_server_poll: completion ring %d names slot %d of %d", c, slot,
+                    gvs::set_error("gvs_server_poll: completion ring %d names slot %d of %d", q, slot,
                                    s->cfg.max_inflight);
                     return GVS_ERR_STATE;
                 }
-                s->chead[c] = h + 1u;
-                comp.push_back({slot, ((v >> 30) & 1u) != 0u, false, 0, -1, 0.0, 0});
+                s->chead[q] = h + 1u;
+                comp.push_back({slot, ((uint32_t)v >> 31) != 0u, false, 0, -1, 0.0, 0});
             }
         }
-        s->cnext = (s->cnext + 1) % SERVE_CQUEUES;
+        s->cnext = (s->cnext + 1) % SERVE_QUEUES;
     }
     {
         std::lock_guard<std::mutex> lk(s->mu);
@@ -545,25 +534,24 @@ static int poll_impl(gvs_server *s, int32_t max_done, int64_t *req_ids, int64_t 
         // the device wrote these rows across the bus: the host reads them from
         // memory, so the rows of a few completions ahead are requested early
         if (j + GVS_POLL_PREFETCH < nc) {
-            const unsigned char *r = s->res + (size_t)comp[j + GVS_POLL_PREFETCH].slot * s->rec_bytes;
-            for (size_t b = 0; b < s->rec_bytes; b += 64) __builtin_prefetch(r + b);
+            const int ps = comp[j + GVS_POLL_PREFETCH].slot;
+            __builtin_prefetch(s->labels + (size_t)ps * K);
+            __builtin_prefetch(s->dists + (size_t)ps * K);
         }
         const auto &c = comp[j];
         if (c.cancelled) continue;
         const int k = std::min(K, std::max(1, c.k));
-        const size_t row = tag_rows > 0 ? (size_t)c.tag : (size_t)n;
-        const bool put = tag_rows <= 0 || (c.tag >= 0 && c.tag < tag_rows);
         if (req_ids) req_ids[n] = c.req;
         if (tags_out) tags_out[n] = c.tag;
-        if (labels && put) {
-            std::memcpy(labels + row * K, s->labels + (size_t)c.slot * (s->rec_bytes / 8), 8 * (size_t)k);
-            std::fill(labels + row * K + k, labels + (row + 1) * K, (int64_t)-1);
+        if (labels) {
+            std::memcpy(labels + (size_t)n * K, s->labels + (size_t)c.slot * K, 8 * (size_t)k);
+            std::fill(labels + (size_t)n * K + k, labels + (size_t)(n + 1) * K, (int64_t)-1);
         }
-        if (dists && put) {
-            std::memcpy(dists + row * K, s->dists + (size_t)c.slot * (s->rec_bytes / 4), 4 * (size_t)k);
-            std::fill(dists + row * K + k, dists + (row + 1) * K, INFINITY);
+        if (dists) {
+            std::memcpy(dists + (size_t)n * K, s->dists + (size_t)c.slot * K, 4 * (size_t)k);
+            std::fill(dists + (size_t)n * K + k, dists + (size_t)(n + 1) * K, INFINITY);
         }
-        if (latency_us && put) latency_us[row] = t - c.t_arrival;
+        if (latency_us) latency_us[n] = t - c.t_arrival;
         if (stats_out)
             std::memcpy(stats_out + (size_t)n * GVS_STATS_ROW, s->stats + (size_t)c.slot * GVS_STATS_ROW,
                         4 * GVS_STATS_ROW);
@@ -785,15 +773,15 @@ static int loadgen_impl(gvs_server *s, const float *queries, int64_t nq, int64_t
             }
         }
     };
-    // poll thread j: the rows go straight to each request's own output row
-    // (its tag), completions of requests not made here are skipped
     auto poller = [&](int j) {
         const int batch = 1024;
+        std::vector<int64_t> tg(batch), lab((size_t)batch * K);
+        std::vector<float> dd((size_t)batch * K);
+        std::vector<double> lat(batch);
         double last_progress = now_us();
         while (done_n.load(std::memory_order_relaxed) < nq && !stop.load(std::memory_order_relaxed)) {
             const double a0 = now_us();
-            const int n = poll_impl(s, batch, nullptr, out_labels, out_dists, out_latency_us, nullptr, nullptr,
-                                    nullptr, nq);
+            const int n = poll_impl(s, batch, nullptr, lab.data(), dd.data(), lat.data(), nullptr, nullptr, tg.data());
             if (n < 0) {
                 fail(n, gvs_last_error());
                 return;
@@ -801,6 +789,13 @@ static int loadgen_impl(gvs_server *s, const float *queries, int64_t nq, int64_t
             if (n > 0) {
                 poll_us[(size_t)j] += now_us() - a0;
                 poll_calls[(size_t)j]++;
+            }
+            for (int i2 = 0; i2 < n; i2++) {
+                const int64_t i = tg[(size_t)i2];
+                if (i < 0 || i >= nq) continue;
+                if (out_labels) std::memcpy(out_labels + i * K, lab.data() + (size_t)i2 * K, 8 * (size_t)K);
+                if (out_dists) std::memcpy(out_dists + i * K, dd.data() + (size_t)i2 * K, 4 * (size_t)K);
+                if (out_latency_us) out_latency_us[i] = lat[(size_t)i2];
             }
             const double now = now_us();
             if (n > 0) {

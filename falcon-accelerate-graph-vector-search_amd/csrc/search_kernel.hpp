@@ -115,11 +115,6 @@ struct KParams {
     int64_t init_ld;
     const int32_t *init_ids;
     int64_t n;
-    // serving (results written across the bus): row strides of out_labels and
-    // out_dists in elements, so that a slot's labels and distances can share
-    // one record of whole cache lines; the batch path uses stride k
-    int out_ld_lab;
-    int out_ld_dist;
 };
 
 __device__ __forceinline__ uint32_t ord_f(float f)

@@ -612,17 +612,14 @@ __device__ __forceinline__ void load_query(const KParams &P, const float *Q, boo
         }
 }
 
-// One result row: plain stores at index `at` (rows of stride k), or, for a
-// request whose results cross the bus (sys), system-scope stores that write
-// through to host memory, so the completion flag needs only a wait for them,
-// not a write-back of L2; those rows have the strides out_ld_lab / out_ld_dist
-// (result i of slot qi).
-__device__ __forceinline__ void put_result(const KParams &P, int64_t at, int64_t lab, float d, bool sys, int64_t qi,
-                                           int i)
+// One result row: plain stores, or, for a request whose results cross the
+// bus (sys), system-scope stores that write through to host memory, so the
+// completion flag needs only a wait for them, not a write-back of L2.
+__device__ __forceinline__ void put_result(const KParams &P, int64_t at, int64_t lab, float d, bool sys)
 {
     if (sys) {
-        __hip_atomic_store(&P.out_labels[qi * P.out_ld_lab + i], lab, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-        __hip_atomic_store(&P.out_dists[qi * P.out_ld_dist + i], d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        __hip_atomic_store(&P.out_labels[at], lab, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        __hip_atomic_store(&P.out_dists[at], d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     } else {
         P.out_labels[at] = lab;
         P.out_dists[at] = d;
@@ -859,7 +856,7 @@ __device__ bool search_query_req(const KParams &P, int64_t qi, int64_t slot, uns
             lab = P.labels ? P.labels[id] : (int64_t)id;
             d = key_dist(key);
         }
-        put_result(P, qi * P.k + i, lab, d, sys_query, qi, i);
+        put_result(P, qi * P.k + i, lab, d, sys_query);
     }
     PROF_MARK(7);
     if (lane == 0) {
@@ -1226,7 +1223,7 @@ __device__ bool search_query_mw_req(const KParams &P, int64_t qi, int64_t slot, 
                 lab = P.labels ? P.labels[id] : (int64_t)id;
                 d = key_dist(key);
             }
-            put_result(P, qi * P.k + i, lab, d, sys_query, qi, i);
+            put_result(P, qi * P.k + i, lab, d, sys_query);
         }
         if (lane == 0) {
             if (P.out_stats) {

@@ -1,7 +1,7 @@
 // Persistent serving kernel: waves claim published requests from a ring in
 // pinned host memory, search them with each request's own k and ef, write
-// the results back to pinned host memory and append (overflow bit, slot) to a
-// completion ring.
+// the results back to pinned host memory, raise a per-slot done flag and
+// append (position, overflow bit, slot) to a completion ring.
 #pragma once
 #include "search_kernels.hpp"
 
@@ -12,11 +12,6 @@ namespace gvsk {
 // head, and waves racing for new work spread over the rings instead of
 // serialising on one counter.
 constexpr int SERVE_QUEUES = 16;
-// Completions go to SERVE_CQUEUES rings (request ring q to ring q % 4), each
-// with its own device counter: same-address device-scope atomics from all
-// XCDs serialise (about 3.7M per second), and the host, which reads every
-// ring's newest line on each poll, touches fewer lines the fewer rings there are.
-constexpr int SERVE_CQUEUES = 4;
 // heads in device memory, one per 128-byte line (16 uint64)
 constexpr int SERVE_HEAD_STRIDE = 16;
 // Device-memory mirror of the host's tails (uint64 words): [0, QUEUES) the
@@ -47,15 +42,17 @@ struct ServeArgs {
     const float *qbuf;       // pinned: slot-major queries, stride ldq
     const int32_t *qk;       // pinned, per slot
     const int32_t *qef;      // pinned, per slot
-    uint32_t *cring;         // pinned completion rings: entry i of ring c at cring[c * ccap + (i & (ccap-1))]
-                             // = lap << 31 | overflow << 30 | slot, lap = ((i >> clog2) + 1) & 1, so that
-                             // neither a zeroed entry nor one of the previous lap reads as entry i
-    uint32_t ccap;           // entries per completion ring, 1 << clog2, at least twice what can be unread
-    uint32_t clog2;
+    const uint64_t *qseq;    // pinned, per slot: sequence number of the request in the slot
+    uint64_t *done;          // pinned, per slot: sequence number of the last completed request
+    uint64_t *cring;         // pinned completion rings, one per request ring: entry i of ring q at
+                             // cring[q * ccap + (i & (ccap-1))] = (i + 1) << 32 | overflow << 31 | slot
+    uint32_t ccap;           // entries per completion ring, a power of two, >= rcap
     uint32_t nslots;         // slots (a ring entry names one below this)
     uint64_t *head;          // device: next request index to claim on ring q at head[q * SERVE_HEAD_STRIDE]
     uint64_t *mirror;        // device: SERVE_MIRROR_WORDS words, the tails as last read from the host
-    uint64_t *served;        // device: entries written to completion ring c, at served[c * SERVE_HEAD_STRIDE]
+    uint64_t *served;        // device: requests of ring q completed (its completion ring position) at
+                             // served[q * SERVE_HEAD_STRIDE]: one counter per ring, since same-address
+                             // device-scope atomics from all XCDs serialise
     int64_t idle_ticks;      // 100 MHz ticks without work before a wave leaves the loop
 };
 
@@ -215,19 +212,19 @@ __device__ int serve_slot(const ServeArgs &A, int q, uint64_t item)
 // write through to host memory: once they are complete (the wait below,
 // also a compiler barrier) the flags can follow, with no release fence (at
 // system scope that writes back the whole L2 of the XCD, once per request).
-// Then the entry in the completion ring of the request's ring q (lap,
-// overflow bit, slot), also stored through to host memory; the host reads
-// the entry before the rows, and PCIe keeps posted writes in order.
-__device__ void serve_complete(const ServeArgs &A, int q, int slot, bool ovf)
+// Then the per-slot done word and the entry in the completion ring of the
+// request's ring q (position, overflow bit, slot), also stored through to
+// host memory; the host reads the entry before the rows, and PCIe keeps
+// posted writes in order.
+__device__ void serve_complete(const ServeArgs &A, int q, int slot, uint64_t seq, bool ovf)
 {
     __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
     if (lane_id() == 0) {
-        const int c = q % SERVE_CQUEUES;
-        const uint64_t pos = atomicAdd((unsigned long long *)(A.served + (size_t)c * SERVE_HEAD_STRIDE), 1ull);
-        const uint32_t lap = (uint32_t)((pos >> A.clog2) + 1ull) & 1u;
-        __hip_atomic_store(&A.cring[(size_t)c * A.ccap + (size_t)(pos & (uint64_t)(A.ccap - 1u))],
-                           (lap << 31) | ((ovf ? 1u : 0u) << 30) | (uint32_t)slot, __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_SYSTEM);
+        __hip_atomic_store(&A.done[slot], seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        const uint64_t pos = atomicAdd((unsigned long long *)(A.served + (size_t)q * SERVE_HEAD_STRIDE), 1ull);
+        __hip_atomic_store(&A.cring[(size_t)q * A.ccap + (size_t)(pos & (uint64_t)(A.ccap - 1u))],
+                           ((pos + 1ull) << 32) | ((uint64_t)(ovf ? 1u : 0u) << 31) | (uint64_t)slot,
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     }
 }
 
@@ -248,9 +245,10 @@ __global__ __launch_bounds__(256, GVS_MIN_WAVES_PER_EU) void k_serve(ServeArgs A
         if (slot < 0) break;
         const int k = ld_sys32i(&A.qk[slot]);
         const int ef = ld_sys32i(&A.qef[slot]);
+        const uint64_t seq = ld_sys64r(&A.qseq[slot]);
         const bool ovf = search_query_req<NCH, STORE, METRIC, MROWS>(A.P, (int64_t)slot, gw, lds, k, ef,
                                                                      A.qbuf + (size_t)slot * A.P.ldq, true);
-        serve_complete(A, q, slot, ovf);
+        serve_complete(A, q, slot, seq, ovf);
         C.last = wall_clock64();
     }
 }
@@ -263,7 +261,7 @@ template <int NCH, int STORE, int METRIC, int WROWS>
 __global__ __launch_bounds__(512) void k_serve_mw(ServeArgs A)
 {
     extern __shared__ __align__(16) unsigned char smem[];
-    __shared__ int64_t claim[3];  // state, slot, k | ef << 32
+    __shared__ int64_t claim[4];  // state, slot, k | ef << 32, seq
     const int w = threadIdx.x >> 6;
     const int W = A.P.waves_per_block;
     const int64_t gb = blockIdx.x;
@@ -275,6 +273,7 @@ __global__ __launch_bounds__(512) void k_serve_mw(ServeArgs A)
             uint32_t st = serve_claim(A, C, q, item);
             int slot = -1;
             int k = 0, ef = 0;
+            uint64_t seq = 0ull;
             if (st == 0u) {
                 slot = serve_slot(A, q, item);
                 if (slot < 0) {
@@ -282,12 +281,14 @@ __global__ __launch_bounds__(512) void k_serve_mw(ServeArgs A)
                 } else {
                     k = ld_sys32i(&A.qk[slot]);
                     ef = ld_sys32i(&A.qef[slot]);
+                    seq = ld_sys64r(&A.qseq[slot]);
                 }
             }
             if (lane_id() == 0) {
                 claim[0] = st;
                 claim[1] = slot;
                 claim[2] = (int64_t)(uint32_t)k | ((int64_t)ef << 32);
+                claim[3] = (int64_t)seq;
             }
         }
         wg_sync();
@@ -295,11 +296,12 @@ __global__ __launch_bounds__(512) void k_serve_mw(ServeArgs A)
         const int slot = (int)claim[1];
         const int k = (int)(uint32_t)claim[2];
         const int ef = (int)(claim[2] >> 32);
+        const uint64_t seq = (uint64_t)claim[3];
         if (st != 0u) break;  // the same for every wave of the workgroup
         // (the search's own barriers order these reads before the next claim's writes)
         const bool ovf = search_query_mw_req<NCH, STORE, METRIC, WROWS>(A.P, (int64_t)slot, gb, smem, k, ef,
                                                                         A.qbuf + (size_t)slot * A.P.ldq, true);
-        if (w == 0) serve_complete(A, q, slot, ovf);
+        if (w == 0) serve_complete(A, q, slot, seq, ovf);
         C.last = wall_clock64();
     }
 }
