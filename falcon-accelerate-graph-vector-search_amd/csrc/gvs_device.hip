@@ -162,14 +162,16 @@ static int noalloc_scratch(int device, unsigned char **out)
 static void free_index(gvs_index *ix)
 {
     if (!ix) return;
-    (void)hipFree(ix->d_f32);
-    (void)hipFree(ix->d_bf16);
-    (void)hipFree(ix->d_fp8);
-    (void)hipFree(ix->d_scale);
-    (void)hipFree(ix->d_adj0);
-    (void)hipFree(ix->d_adj_up);
-    (void)hipFree(ix->d_upper_off);
-    (void)hipFree(ix->d_labels);
+    if (!ix->parent) {  // a fork's arrays are its root's
+        (void)hipFree(ix->d_f32);
+        (void)hipFree(ix->d_bf16);
+        (void)hipFree(ix->d_fp8);
+        (void)hipFree(ix->d_scale);
+        (void)hipFree(ix->d_adj0);
+        (void)hipFree(ix->d_adj_up);
+        (void)hipFree(ix->d_upper_off);
+        (void)hipFree(ix->d_labels);
+    }
     (void)hipFree(ix->d_vtab);
     (void)hipFree(ix->d_counters);
     (void)hipFree(ix->d_status);
@@ -333,11 +335,65 @@ extern "C" int gvs_index_create(const gvs_graph *g, uint32_t storage_mask, gvs_i
     return GVS_OK;
 }
 
+// guards the fork counts of root indexes
+static std::mutex g_fork_mu;
+
 extern "C" int gvs_index_destroy(gvs_index *ix)
 {
     if (!ix) return GVS_ERR_ARG;
+    {
+        std::lock_guard<std::mutex> lk(g_fork_mu);
+        if (ix->forks > 0) {
+            gvs::set_error("gvs_index_destroy: %d forks of this index are still open", ix->forks);
+            return GVS_ERR_STATE;
+        }
+        if (ix->parent) ix->parent->forks--;
+    }
     (void)hipDeviceSynchronize();
     free_index(ix);
+    return GVS_OK;
+}
+
+extern "C" int gvs_index_fork(gvs_index *ix, gvs_index **out)
+{
+    if (!ix || !out) {
+        gvs::set_error("gvs_index_fork: invalid argument");
+        return GVS_ERR_ARG;
+    }
+    gvs_index *root = ix->parent ? ix->parent : ix;
+    int dev = -1;
+    if (!hip_ok(hipGetDevice(&dev), "hipGetDevice")) return GVS_ERR_HIP;
+    if (dev != root->device) {
+        gvs::set_error("gvs_index_fork: the index lives on device %d, the current device is %d", root->device, dev);
+        return GVS_ERR_ARG;
+    }
+    gvs_index *f = new gvs_index(*root);
+    // own scratch only (allocated on demand, as for a new index)
+    f->parent = root;
+    f->forks = 0;
+    f->d_vtab = nullptr;
+    f->vtab_words = 0;
+    f->d_counters = nullptr;
+    f->d_status = nullptr;
+    f->status_cap = 0;
+    f->d_xnorm64 = nullptr;
+    f->d_rep = nullptr;
+    f->d_rkey = nullptr;
+    f->d_rperm = nullptr;
+    f->d_rtmp = nullptr;
+    f->rcap = 0;
+    f->rtmp_bytes = 0;
+    if (!hip_ok(hipMalloc(&f->d_counters, 64), "hipMalloc counters") ||
+        !hip_ok(hipMemset(f->d_counters, 0, 64), "memset counters")) {
+        (void)hipFree(f->d_counters);
+        delete f;
+        return GVS_ERR_HIP;
+    }
+    {
+        std::lock_guard<std::mutex> lk(g_fork_mu);
+        root->forks++;
+    }
+    *out = f;
     return GVS_OK;
 }
 

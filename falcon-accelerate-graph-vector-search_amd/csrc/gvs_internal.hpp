@@ -22,6 +22,10 @@ struct gvs_index {
     int32_t *d_adj_up = nullptr;
     int64_t *d_upper_off = nullptr;
     int64_t *d_labels = nullptr;
+    // A fork (gvs_index_fork) shares the arrays above with its root index and
+    // owns only what follows; the root counts its open forks.
+    gvs_index *parent = nullptr;
+    int forks = 0;
     // scratch: per-slot visited tables, work-queue counters, per-query status
     uint32_t *d_vtab = nullptr;
     size_t vtab_words = 0;

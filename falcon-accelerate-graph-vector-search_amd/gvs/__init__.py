@@ -88,6 +88,7 @@ def _load():
         "gvs_gen_mixture": (C.c_int, [U64, U64, I64, I32, I32, C.c_float, C.c_float, I32, P, I32]),
         "gvs_index_create": (C.c_int, [P, U32, P]),
         "gvs_index_destroy": (C.c_int, [P]),
+        "gvs_index_fork": (C.c_int, [P, P]),
         "gvs_index_device_bytes": (C.c_int, [P, P, P]),
         "gvs_search_device": (C.c_int, [P, P, I64, I64, P, P, P, P, P, P]),
         "gvs_search_host": (C.c_int, [P, P, I64, I64, P, P, P, P, P]),
@@ -141,7 +142,7 @@ def exported_symbols():
     """Names of the C ABI functions this binding knows, present in the library."""
     names = ["gvs_last_error", "gvs_version", "gvs_hnsw_build", "gvs_nsg_from_hnsw", "gvs_hnswlib_load",
              "gvs_hnswlib_save", "gvs_nsg_load", "gvs_fpga_load", "gvs_fpga_save", "gvs_host_graph_view", "gvs_host_graph_free",
-             "gvs_gen_mixture", "gvs_index_create", "gvs_index_destroy", "gvs_index_device_bytes",
+             "gvs_gen_mixture", "gvs_index_create", "gvs_index_destroy", "gvs_index_fork", "gvs_index_device_bytes",
              "gvs_search_device", "gvs_search_host", "gvs_search_device_noalloc", "gvs_search_overflow_count", "gvs_search_counters", "gvs_autotune", "gvs_search_graph_create",
              "gvs_search_graph_launch", "gvs_search_graph_destroy", "gvs_device_copy", "gvs_stats_per_query", "gvs_search_plan", "gvs_hash_murmur2_keylen4_device", "gvs_bruteforce_f64_device", "gvs_server_open",
              "gvs_server_submit", "gvs_server_submit_batch", "gvs_server_poll", "gvs_server_poll_stats",
@@ -325,6 +326,18 @@ class DeviceIndex:
     @property
     def handle(self):
         return self._h
+
+    def fork(self):
+        """A second handle on the same device arrays with scratch of its own
+        (gvs_index_fork): searches through different handles may run at once
+        on different streams."""
+        f = DeviceIndex.__new__(DeviceIndex)
+        f._graph = self._graph
+        f._root = getattr(self, "_root", None) or self  # destroyed after its forks
+        f._h = C.c_void_p()
+        _check(lib.gvs_index_fork(self._h, C.byref(f._h)))
+        f.dim, f.n = self.dim, self.n
+        return f
 
     def search(self, queries, k=10, ef=64, entries=None, with_stats=False, **kw):
         """searchKnn over a batch of host queries -> (labels, dists[, stats])."""

@@ -221,6 +221,17 @@ GVS_API int gvs_gen_mixture(uint64_t seed_centers, uint64_t seed_points, int64_t
  * (F32 is always kept; BF16 / FP8-e4m3 with per-vector scale are optional). */
 GVS_API int gvs_index_create(const gvs_graph *g, uint32_t storage_mask, gvs_index **out);
 GVS_API int gvs_index_destroy(gvs_index *ix);
+/* A second handle on the same device index: it shares the resident arrays
+ * (no copy) and owns its own search scratch (work-queue counters, statuses,
+ * visited tables, region-order buffers).  Searches through one handle share
+ * its scratch and must not overlap; searches through different handles may
+ * run at the same time on different streams, so that one batch's descent,
+ * sort and ramp run under the previous batch's drain.  Destroy the forks
+ * before the index (gvs_index_destroy returns GVS_ERR_STATE while forks are
+ * open); a fork of a fork is a fork of the first index.  No counterpart in
+ * the reference: its host runs one batch at a time on the FPGA kernel
+ * (INTER/host.cpp:851-995). */
+GVS_API int gvs_index_fork(gvs_index *ix, gvs_index **out);
 /* device pointers of the resident arrays (for tests / introspection) */
 GVS_API int gvs_index_device_bytes(const gvs_index *ix, int64_t *graph_bytes, int64_t *vector_bytes);
 
@@ -232,7 +243,9 @@ GVS_API int gvs_index_device_bytes(const gvs_index *ix, int64_t *graph_bytes, in
  * per-query layer-0 entry point (skips the upper-layer descent, as
  * INTER/host.cpp:711-713 does) or NULL.  Outputs (device): labels[nq*k],
  * dists[nq*k] ascending (closer first, HNSWLIB/hnswlib.h:171-185); missing
- * results are label -1 / dist +inf.  stats: see GVS_STATS_PER_QUERY. */
+ * results are label -1 / dist +inf.  stats: see GVS_STATS_PER_QUERY.
+ * Searches through one index handle share its scratch and must not overlap
+ * (one stream at a time); see gvs_index_fork for concurrent batches. */
 GVS_API int gvs_search_device(gvs_index *ix, const float *queries, int64_t nq, int64_t ldq,
                               const int32_t *entries, const gvs_params *p,
                               int64_t *labels, float *dists, int32_t *stats, void *stream);

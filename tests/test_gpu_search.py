@@ -392,6 +392,42 @@ def test_async_search_has_no_overflow_at_default_sizes():
         np.testing.assert_array_equal(dl.to_host(), lab_sync)
 
 
+def test_fork_batches_on_two_streams_equal_one_handle():
+    """gvs_index_fork: a second handle on the same device arrays with scratch
+    of its own.  Batches alternating between the index and its fork on two
+    streams (two waves per query, persistent, region order; all queued before
+    any is waited on) return the oracle's results, as one handle does on one
+    stream; the index cannot be destroyed while its fork is open."""
+    from gvs import hip as H
+    x, g, _ = graph(20000, 128, 16, 100)
+    og = orc.OracleGraph(g.arrays())
+    dev = gvs.DeviceIndex(g)
+    fork = dev.fork()
+    nq = 1000
+    q = queries(nq, 128, seed=61)
+    cfg = dict(mc=4, waves_per_query=2, persistent=True, region_order=True)
+    print("case fork: one handle against the oracle", flush=True)
+    lab, dist, _ = check_equal(dev, og, q, 10, 64, **cfg)
+    dq = H.DeviceArray.from_host(q)
+    streams = [H.Stream(), H.Stream()]
+    outs = [(H.DeviceArray((nq, 10), np.int64), H.DeviceArray((nq, 10), np.float32)) for _ in range(6)]
+    print("case fork: six batches over two handles and two streams", flush=True)
+    for b in range(6):
+        (dev, fork)[b % 2].search_device(dq.address, nq, 128, outs[b][0].address, outs[b][1].address, k=10, ef=64,
+                                         stream=streams[b % 2].handle, asynchronous=True, **cfg)
+    H.synchronize()
+    for b in range(6):
+        np.testing.assert_array_equal(outs[b][0].to_host(), lab)
+        np.testing.assert_array_equal(outs[b][1].to_host(), dist)
+    assert dev.overflow_count() == 0 and fork.overflow_count() == 0
+    err_state = -7  # GVS_ERR_STATE
+    assert gvs.lib.gvs_index_destroy(dev.handle) == err_state
+    assert gvs.lib.gvs_index_destroy(fork.handle) == 0
+    fork._h = None
+    assert gvs.lib.gvs_index_destroy(dev.handle) == 0
+    dev._h = None
+
+
 @pytest.mark.parametrize("storage", [gvs.BF16, gvs.FP8])
 def test_low_precision_traversal_and_fp32_rerank(storage):
     """BF16 / FP8-e4m3 (per-vector scale) storage, fp32 accumulation: the
