@@ -392,38 +392,43 @@ def test_async_search_has_no_overflow_at_default_sizes():
         np.testing.assert_array_equal(dl.to_host(), lab_sync)
 
 
-def test_fork_batches_on_two_streams_equal_one_handle():
-    """gvs_index_fork: a second handle on the same device arrays with scratch
-    of its own.  Batches alternating between the index and its fork on two
-    streams (two waves per query, persistent, region order; all queued before
-    any is waited on) return the oracle's results, as one handle does on one
-    stream; the index cannot be destroyed while its fork is open."""
+def test_fork_batches_on_three_streams_equal_one_handle():
+    """gvs_index_fork: handles on the same device arrays with scratch of their
+    own.  Batches going round the index, a fork and a fork of the fork on
+    three streams (two waves per query, persistent, region order; all queued
+    before any is waited on) return the oracle's results, as one handle does
+    on one stream; the index cannot be destroyed while a fork is open."""
     from gvs import hip as H
     x, g, _ = graph(20000, 128, 16, 100)
     og = orc.OracleGraph(g.arrays())
     dev = gvs.DeviceIndex(g)
     fork = dev.fork()
+    fork2 = fork.fork()
+    handles = (dev, fork, fork2)
     nq = 1000
     q = queries(nq, 128, seed=61)
     cfg = dict(mc=4, waves_per_query=2, persistent=True, region_order=True)
     print("case fork: one handle against the oracle", flush=True)
     lab, dist, _ = check_equal(dev, og, q, 10, 64, **cfg)
     dq = H.DeviceArray.from_host(q)
-    streams = [H.Stream(), H.Stream()]
-    outs = [(H.DeviceArray((nq, 10), np.int64), H.DeviceArray((nq, 10), np.float32)) for _ in range(6)]
-    print("case fork: six batches over two handles and two streams", flush=True)
-    for b in range(6):
-        (dev, fork)[b % 2].search_device(dq.address, nq, 128, outs[b][0].address, outs[b][1].address, k=10, ef=64,
-                                         stream=streams[b % 2].handle, asynchronous=True, **cfg)
+    streams = [H.Stream() for _ in handles]
+    outs = [(H.DeviceArray((nq, 10), np.int64), H.DeviceArray((nq, 10), np.float32)) for _ in range(9)]
+    print("case fork: nine batches over three handles and three streams", flush=True)
+    for b in range(9):
+        handles[b % 3].search_device(dq.address, nq, 128, outs[b][0].address, outs[b][1].address, k=10, ef=64,
+                                     stream=streams[b % 3].handle, asynchronous=True, **cfg)
     H.synchronize()
-    for b in range(6):
+    for b in range(9):
         np.testing.assert_array_equal(outs[b][0].to_host(), lab)
         np.testing.assert_array_equal(outs[b][1].to_host(), dist)
-    assert dev.overflow_count() == 0 and fork.overflow_count() == 0
+    assert all(h.overflow_count() == 0 for h in handles)
     err_state = -7  # GVS_ERR_STATE
     assert gvs.lib.gvs_index_destroy(dev.handle) == err_state
     assert gvs.lib.gvs_index_destroy(fork.handle) == 0
     fork._h = None
+    assert gvs.lib.gvs_index_destroy(dev.handle) == err_state  # fork2 is the index's fork too
+    assert gvs.lib.gvs_index_destroy(fork2.handle) == 0
+    fork2._h = None
     assert gvs.lib.gvs_index_destroy(dev.handle) == 0
     dev._h = None
 
