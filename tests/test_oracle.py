@@ -513,6 +513,11 @@ def test_invalid_arguments_return_errors():
         gvs.HostGraph.load_hnswlib("/nonexistent/file.bin", 16)
     with pytest.raises(gvs.GvsError):
         gvs.HostGraph.build_hnsw(np.zeros((10, 4), np.float32), M=1)
+    # index handles: NULL is refused before any device call
+    import ctypes as C
+    h = C.c_void_p()
+    assert gvs.lib.gvs_index_fork(None, C.byref(h)) == -1 and not h.value  # GVS_ERR_ARG
+    assert gvs.lib.gvs_index_destroy(None) == -1
 
 
 def test_vectorised_test_encoders_match_oracle():
